@@ -1,0 +1,125 @@
+"""ctypes declarations of libiccg_hip.so (include/iccg_hip.h).
+
+The library is loaded from iccg_amd/lib/ (built in-tree by
+`make -C iccg_amd/csrc`).  A missing library is an error: there is no CPU
+fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libiccg_hip.so")
+
+ICCG_OK = 0
+ICCG_ERR_ARG, ICCG_ERR_IO, ICCG_ERR_PIVOT, ICCG_ERR_SCHEDULE = 1, 2, 3, 4
+ICCG_ERR_DEVICE, ICCG_ERR_NOMEM, ICCG_ERR_STATE = 5, 6, 7
+PRECOND_IDENTITY, PRECOND_JACOBI, PRECOND_IC0 = 0, 1, 2
+FP64, MIXED = 0, 1
+SCHED_MC, SCHED_ABMC = 0, 1
+SOLVE_TRUE_RESID = 1
+
+_vp = C.c_void_p
+_i = C.c_int
+_l = C.c_long
+_d = C.c_double
+_ip = C.POINTER(C.c_int)
+_lp = C.POINTER(C.c_long)
+_dp = C.POINTER(C.c_double)
+
+
+class CGResultC(C.Structure):
+    _fields_ = [("iters", C.c_int), ("rel_resid", C.c_double), ("converged", C.c_int),
+                ("breakdown", C.c_int), ("true_rel_resid", C.c_double),
+                ("restarts", C.c_int), ("solve_ms", C.c_double)]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "iccg_last_error": (C.c_char_p, []),
+    "iccg_status_string": (C.c_char_p, [_i]),
+    "iccg_device_count": (_i, [_ip]),
+    "iccg_device_set": (_i, [_i]),
+    "iccg_mm_read": (_i, [C.c_char_p, _ip, _lp, C.POINTER(_ip), C.POINTER(_ip), C.POINTER(_dp)]),
+    "iccg_free_host": (None, [_vp]),
+    "iccg_coo_to_crs": (_i, [_i, _l, _vp, _vp, _vp, _vp, _vp, _vp, _lp]),
+    "iccg_expand_lower_to_full": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _lp]),
+    "iccg_read_color_file": (_i, [C.c_char_p, _i, _vp, _ip]),
+    "iccg_read_block_file": (_i, [C.c_char_p, _i, _vp, _ip]),
+    "iccg_read_block_color_file": (_i, [C.c_char_p, _i, _vp, _ip]),
+    "iccg_write_color_file": (_i, [C.c_char_p, _i, _vp, _i]),
+    "iccg_write_block_file": (_i, [C.c_char_p, _i, _vp, _i]),
+    "iccg_write_block_color_file": (_i, [C.c_char_p, _i, _vp, _i]),
+    "iccg_perm_by_color": (_i, [_i, _vp, _vp]),
+    "iccg_perm_by_block_color": (_i, [_i, _vp, _i, _vp, _vp]),
+    "iccg_ptap_lower": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _lp]),
+    "iccg_color_ptr_rows": (_i, [_i, _vp, _i, _vp, _vp, _vp]),
+    "iccg_abmc_schedule": (_i, [_i, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp]),
+    "iccg_check_interblock": (_i, [_i, _vp, _vp, _vp, _i, _vp, _i, _vp, C.POINTER(C.c_longlong)]),
+    "iccg_check_hazards": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _i, _ip, _vp, _vp, _vp,
+                                C.POINTER(C.c_longlong)]),
+    "iccg_greedy_color": (_i, [_i, _vp, _vp, _vp, _ip]),
+    "iccg_abmc_blocking": (_i, [_i, _vp, _vp, _i, _vp, _ip, _vp, _ip]),
+    "iccg_abmc_blocking_grid": (_i, [_i, _i, _i, _i, _i, _i, _vp, _ip, _vp, _ip]),
+    "iccg_gen_grid": (_i, [_i, _i, _i, _d, _d, _d, C.c_ulonglong, _ip, _lp, _vp, _vp, _vp]),
+    "iccg_gen_normal": (_i, [_l, C.c_ulonglong, _vp]),
+    "iccg_csr_create": (_i, [_i, _l, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "iccg_csr_destroy": (_i, [_vp]),
+    "iccg_spmv": (_i, [_vp, _vp, _vp]),
+    "iccg_schedule_create_mc": (_i, [_i, _i, _vp, _vp, C.POINTER(_vp)]),
+    "iccg_schedule_create_abmc": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "iccg_schedule_destroy": (_i, [_vp]),
+    "iccg_precond_create": (_i, [_vp, _i, _d, _vp, _i, C.POINTER(_vp)]),
+    "iccg_precond_destroy": (_i, [_vp]),
+    "iccg_precond_set_schedule": (_i, [_vp, _vp]),
+    "iccg_precond_apply": (_i, [_vp, _vp, _vp]),
+    "iccg_precond_factor_nnz": (_i, [_vp, _lp]),
+    "iccg_precond_get_factor": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "iccg_cg_solve": (_i, [_vp, _vp, _vp, _vp, _i, _d, _i, C.POINTER(CGResultC)]),
+    "iccg_solver_create": (_i, [_vp, _vp, C.POINTER(_vp)]),
+    "iccg_solver_destroy": (_i, [_vp]),
+    "iccg_solver_set": (_i, [_vp, _vp, _vp]),
+    "iccg_solver_run": (_i, [_vp, _i, _d, _i, C.POINTER(CGResultC)]),
+    "iccg_solver_get_x": (_i, [_vp, _vp]),
+    "iccg_solver_set_timing": (_i, [_vp, _i]),
+    "iccg_solver_get_timing": (_i, [_vp, _i, _lp, _dp]),
+    "iccg_solver_reset_timing": (_i, [_vp]),
+    "iccg_bench_copy": (_i, [_l, _i, _dp]),
+}
+
+_lib = None
+
+
+class IccgError(RuntimeError):
+    """Error raised for a non-zero status; .status holds the ICCG_* code."""
+
+    def __init__(self, status, msg):
+        super().__init__(msg)
+        self.status = status
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is not built: run `make -C iccg_amd/csrc` (or "
+                "__graft_entry__.build()); the HIP library is required")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def check(status):
+    if status != ICCG_OK:
+        msg = lib().iccg_last_error().decode(errors="replace")
+        raise IccgError(status, msg)

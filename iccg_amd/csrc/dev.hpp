@@ -1,0 +1,130 @@
+// dev.hpp -- device data layout and kernel launchers of libiccg_hip.so.
+//
+// HBM layout (one system):
+//   A   : CSR, int32 rowptr[n+1], int32 colind[nnz], f64 val[nnz]
+//   L   : lower+diag CSR of the IC(0) pattern (rows sorted, diagonal LAST),
+//         f64 values (+ f32 copy in mixed precision)
+//   U   : L^T as CSR (= CSC of L), diagonal FIRST in each row
+//   vectors x, r, p, Ap, b : f64[n];  y, z : f64[n] or f32[n] (mixed)
+//   Scal: the solver's scalars (alpha, beta, rz, norms, flags) -- device
+//         resident, read back by the host only every few iterations
+//   part: per-workgroup double-double partial sums of the dot products
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace iccg {
+
+constexpr int kWG = 256;          // threads per workgroup of streaming kernels
+constexpr int kMaxRedBlocks = 2048;  // workgroups of reducing kernels (8 per CU)
+
+// Device-resident solver scalars.  Written only by the last-arriving
+// workgroup of a reducing kernel; read by the others at kernel start.
+struct alignas(128) Scal {
+  double normb;    // ||b|| (1.0 when ||b|| == 0, cg_crs.cpp:108-110)
+  double rz_old;   // r.z of the previous iteration
+  double alpha, beta;
+  double rel;      // recurrence ||r|| / ||b||
+  double pAp;
+  double rr_true;  // ||b - A x||^2 (true-residual check)
+  double tol;
+  int iters;       // completed iterations
+  int done;        // stop flag: converged / breakdown / max_iter
+  int converged;
+  int breakdown;
+  int max_iter;
+  int pad0;
+  unsigned int arrive[8];  // last-workgroup counters, one per kernel kind
+  unsigned long long err;  // IC(0) failure key (row << 32 | row_j), ~0 = none
+};
+
+enum PrecondKind { PK_IDENT = 0, PK_JAC = 1, PK_IC = 2 };
+
+// arrival-counter slots
+enum { SLOT_SPMV = 0, SLOT_XR = 1, SLOT_RESID = 2, SLOT_RZ = 3, SLOT_PINIT = 4, SLOT_TRUE = 5 };
+
+struct DevCsr {
+  int n = 0;
+  long nnz = 0;
+  int *rp = nullptr, *ci = nullptr;
+  double *v = nullptr;
+};
+
+// Triangular-solve / factorization schedule on the device.
+//   MC   : colors as contiguous row ranges (rows == nullptr) or row lists
+//   ABMC : one workgroup per block; bstart[kb] = first position of block kb
+//          (blocks in blocks_of_color order), rows listed by brows (nullptr
+//          when positions are the new row ids); levels inside each block
+//   LEVEL: natural order, level sets (lvl_ptr / lvl_rows)
+struct DevSched {
+  int kind = 0;  // 0 = natural levels, 1 = MC, 2 = ABMC
+  int nc = 0;
+  int *d_rows = nullptr;           // MC: rows_of_color (nullptr if identity)
+  int *d_bstart = nullptr;         // ABMC: nb+1
+  int *d_brows = nullptr;          // ABMC: n (nullptr if contiguous)
+  uint16_t *d_lev_f = nullptr;     // ABMC/levels: per-row forward level
+  uint16_t *d_lev_b = nullptr;     // per-row backward level
+  int *d_nlev_f = nullptr, *d_nlev_b = nullptr;  // ABMC: per block
+  int max_block = 0;
+  // natural levels: level sets and launch segments of the forward and
+  // backward plans.  A segment {0, l0, l1} runs levels [l0, l1) inside one
+  // workgroup (narrow levels); {1, l, 0} runs level l over many workgroups.
+  std::vector<int> h_lvl_ptr_f, h_lvl_ptr_b;
+  int *d_lvl_ptr_f = nullptr, *d_lvl_rows_f = nullptr;
+  int *d_lvl_ptr_b = nullptr, *d_lvl_rows_b = nullptr;
+  std::vector<int> seg_f, seg_b;  // triples
+  // host copies of the range bounds
+  std::vector<int> h_color_ptr_v, h_color_blk_v;
+};
+
+struct Factor {
+  int n = 0;
+  long nnz = 0;
+  int *lrp = nullptr, *lci = nullptr;
+  double *lv = nullptr;
+  float *lv32 = nullptr;
+  int *urp = nullptr, *uci = nullptr;
+  double *uv = nullptr;
+  float *uv32 = nullptr;
+  int *l2u = nullptr;
+};
+
+// ---- launchers (kernels.hip) ----------------------------------------------
+void launch_spmv_plain(const DevCsr &A, const double *x, double *y, hipStream_t st);
+void launch_spmv_cg(const DevCsr &A, const double *p, double *Ap, Scal *S, double *part,
+                    hipStream_t st);
+// r = b - A x with ||b|| and ||r||.
+//   mode 1: start of a solve -- resets S (iters 0), stores tol/max_iter,
+//           sets normb, rel and done (rel < tol or max_iter <= 0)
+//   mode 2: restart from the true residual -- keeps iters
+//   mode 0: true-residual check only: S->rr_true, r not written
+void launch_resid(const DevCsr &A, const double *x, const double *b, double *r, Scal *S,
+                  double *part, int mode, double tol, int max_iter, hipStream_t st);
+void launch_xr(int pk, int n, double *x, const double *p, double *r, const double *Ap,
+               const double *inv_diag, double *z, Scal *S, double *part, hipStream_t st);
+void launch_precond_diag_init(int pk, int n, const double *r, const double *inv_diag, double *z,
+                              double *p, Scal *S, double *part, hipStream_t st);
+void launch_rz(int n, const double *r, const void *z, bool z32, Scal *S, double *part, int init,
+               hipStream_t st);
+void launch_update_p(int n, const void *z, bool z32, double *p, const Scal *S, int init,
+                     hipStream_t st);
+void launch_jacobi_apply(int n, const double *inv_diag, const double *r, double *z,
+                         hipStream_t st);
+void launch_jacobi_setup(const DevCsr &A, double *inv_diag, hipStream_t st);
+
+// sweeps: skip when S != nullptr && S->done
+void launch_fwd(const DevSched &sc, const Factor &F, bool mixed, const double *r, void *y,
+                const Scal *S, hipStream_t st);
+void launch_bwd(const DevSched &sc, const Factor &F, bool mixed, const void *y, void *z,
+                const Scal *S, hipStream_t st);
+void launch_ic0_factor(const DevSched &sc, const Factor &F, double shift,
+                       unsigned long long *err, hipStream_t st);
+void launch_scatter_u(const Factor &F, bool mixed, hipStream_t st);
+void launch_cast_f32(const double *a, float *b, long n, hipStream_t st);
+void launch_cast_f64(const float *a, double *b, long n, hipStream_t st);
+void launch_copy(const double *a, double *b, long n, hipStream_t st);
+
+}  // namespace iccg

@@ -1,0 +1,256 @@
+/*
+ * iccg_hip.h -- C ABI of the MI355X ICCG solve path (libiccg_hip.so).
+ *
+ * Plain C: int / long / double / float pointers and sizes, opaque handles,
+ * int status codes.  No exception crosses this boundary; on failure a call
+ * returns a non-zero ICCG_* status and iccg_last_error() (thread-local) holds
+ * the message, worded like the reference's std::runtime_error text.
+ *
+ * Each entry point names the reference interface it replaces
+ * (stomo0511/ICCG, file:line).  The reference's C++ surface (CRS,
+ * ColorSchedule, Identity/Jacobi/IC0, conjugate_gradient) is mirrored on top
+ * of this header by the headers under include/iccg/.
+ */
+#ifndef ICCG_HIP_H
+#define ICCG_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+enum {
+  ICCG_OK = 0,
+  ICCG_ERR_ARG = 1,        /* bad argument / size mismatch                   */
+  ICCG_ERR_IO = 2,         /* file open / parse failure (crs_io.cpp, color.cpp) */
+  ICCG_ERR_PIVOT = 3,      /* IC(0) missing diagonal / zero / non-positive pivot */
+  ICCG_ERR_SCHEDULE = 4,   /* coloring leaves a same-color dependency (a race) */
+  ICCG_ERR_DEVICE = 5,     /* HIP runtime error                                */
+  ICCG_ERR_NOMEM = 6,      /* host or device allocation failure                */
+  ICCG_ERR_STATE = 7       /* object used in a state that does not allow it    */
+};
+
+/* preconditioner kinds: precond.hpp:8-45 (Identity, Jacobi), ic0.hpp:11 (IC0) */
+enum { ICCG_PRECOND_IDENTITY = 0, ICCG_PRECOND_JACOBI = 1, ICCG_PRECOND_IC0 = 2 };
+
+/* precision of the preconditioner (the outer CG recurrence is always FP64):
+ *   ICCG_FP64   L values, triangular solves and their vectors in FP64 -- the
+ *               reference's arithmetic;
+ *   ICCG_MIXED  IC(0) computed in FP64 (pivots FP64), then stored and applied
+ *               in FP32 (triangular solves, y/z vectors in FP32).            */
+enum { ICCG_FP64 = 0, ICCG_MIXED = 1 };
+
+/* schedule modes: color.hpp:15 ColorSchedule::Mode */
+enum { ICCG_SCHED_MC = 0, ICCG_SCHED_ABMC = 1 };
+
+/* solve flags */
+enum {
+  ICCG_SOLVE_TRUE_RESID = 1 /* stop only when the FP64 true residual
+                               ||b - A x|| / ||b|| < tol (restart CG from the
+                               true residual otherwise).  Default in MIXED.  */
+};
+
+typedef struct iccg_csr_s *iccg_csr_t;
+typedef struct iccg_sched_s *iccg_sched_t;
+typedef struct iccg_precond_s *iccg_precond_t;
+typedef struct iccg_solver_s *iccg_solver_t;
+
+/* CGResult (cg_crs.cpp:82-86) plus what the GPU path reports beside it */
+typedef struct {
+  int iters;              /* iterations taken (cg_crs.cpp:131,144)          */
+  double rel_resid;       /* recurrence ||r||/||b|| at return               */
+  int converged;          /* cg_crs.cpp:120,147                             */
+  int breakdown;          /* pAp <= eps ||p|| ||Ap|| (cg_crs.cpp:127-133)   */
+  double true_rel_resid;  /* ||b - A x|| / ||b|| recomputed in FP64 (<0: not computed) */
+  int restarts;           /* true-residual restarts (ICCG_SOLVE_TRUE_RESID) */
+  double solve_ms;        /* CG phase wall time, device-resident data       */
+} iccg_cg_result;
+
+/* ---- errors --------------------------------------------------------------- */
+const char *iccg_last_error(void);
+const char *iccg_status_string(int status);
+
+/* ---- device --------------------------------------------------------------- */
+int iccg_device_count(int *count);
+/* Selects the HIP device used by objects created afterwards on this thread. */
+int iccg_device_set(int device);
+
+/* ---- L0: CRS I/O (crs_io.hpp:10-56) ---------------------------------------- */
+
+/* read_matrix_market_crs (crs_io.hpp:52, crs_io.cpp:73-159).  Arrays are
+ * allocated by the library; release each with iccg_free_host. */
+int iccg_mm_read(const char *path, int *n, long *nnz, int **rowptr,
+                 int **colind, double **val);
+void iccg_free_host(void *p);
+
+/* coo_to_crs (crs_io.cpp:19-70): sort by (i,j), sum duplicates.
+ * rowptr[nrows+1]; colind/val capacity nin; *nnz_out entries written. */
+int iccg_coo_to_crs(int nrows, long nin, const int *I, const int *J,
+                    const double *V, int *rowptr, int *colind, double *val,
+                    long *nnz_out);
+
+/* expand_lower_to_full (crs_io.hpp:56, crs_io.cpp:163-227).  Output capacity
+ * 2*nnz_lower. */
+int iccg_expand_lower_to_full(int n, const int *rp, const int *ci,
+                              const double *v, int *frp, int *fci, double *fv,
+                              long *nnz_out);
+
+/* ---- L1: orderings (color.hpp:4-7, block.hpp:10-39) ------------------------ */
+
+/* ReadColorFile_1Based (color.cpp:14-47): color[n] 0-based, *nc. */
+int iccg_read_color_file(const char *path, int n, int *color, int *nc);
+/* ReadBlockFile_1Based (block.cpp:15-51): block_of_old[n], *nb. */
+int iccg_read_block_file(const char *path, int n, int *block_of_old, int *nb);
+/* ReadBlockColorFile_1Based (block.cpp:53-89): block_color[nb], *nc. */
+int iccg_read_block_color_file(const char *path, int nb, int *block_color,
+                               int *nc);
+/* Writers of the same 1-based formats (header count line, then "id value"). */
+int iccg_write_color_file(const char *path, int n, const int *color, int nc);
+int iccg_write_block_file(const char *path, int n, const int *block_of_old,
+                          int nb);
+int iccg_write_block_color_file(const char *path, int nb,
+                                const int *block_color, int nc);
+
+/* BuildPermutationByColor (color.cpp:50-63) */
+int iccg_perm_by_color(int n, const int *color, int *new_of_old);
+/* BuildPermutationByBlockColor (block.cpp:92-114) */
+int iccg_perm_by_block_color(int n, const int *block_of_old, int nb,
+                             const int *block_color, int *new_of_old);
+/* Permute_PtAP_LowerCRS_to_LowerCRS (color.cpp:66-141).  Output capacity =
+ * number of stored entries with j<=i. */
+int iccg_ptap_lower(int n, const int *rp, const int *ci, const double *v,
+                    const int *new_of_old, int *brp, int *bci, double *bv,
+                    long *nnz_out);
+/* BuildColorPtrRows_FromPermutationAndColor (color.cpp:164-197) */
+int iccg_color_ptr_rows(int n, const int *color, int nc, const int *new_of_old,
+                        int *color_ptr, int *rows_of_color);
+/* BuildABMCSchedule (block.cpp:116-174) */
+int iccg_abmc_schedule(int n, const int *new_of_old, const int *block_of_old,
+                       const int *block_color, int nb, int nc,
+                       int *color_ptr_blk, int *blocks_of_color,
+                       int *block_ptr, int *rows_of_block);
+
+/* DebugCheck_InterBlockSameColorEdges (cg_crs.cpp:171-224): same-color
+ * inter-block edges, per color (conflicts[nc]) and in total (*total). */
+int iccg_check_interblock(int n, const int *rp, const int *ci,
+                          const int *block_of_old, int nb,
+                          const int *block_color, int nc,
+                          long long *conflicts, long long *total);
+/* DebugCheck_ScheduleHazards_OnLower (cg_crs.cpp:259-336).  Arrays sized
+ * nc_cap; *nc_out = max color + 1.  block_of_old may be NULL. */
+int iccg_check_hazards(int n, const int *rp, const int *ci,
+                       const int *new_of_old, const int *color_old,
+                       const int *block_of_old, int nc_cap, int *nc_out,
+                       long long *hazards, long long *same_block,
+                       long long *cross_block, long long *total);
+
+/* ---- ordering generators (no reference counterpart: the reference reads
+ *      its .col/.blk/.bcol from an external tool, run_mc.sh:14,
+ *      run_abmc.sh:19-20) ------------------------------------------------- */
+
+/* First-fit greedy coloring in index order of the graph of A (i~j when
+ * a_ij or a_ji is stored, i != j).  color[n] 0-based, *nc. */
+int iccg_greedy_color(int n, const int *rp, const int *ci, int *color, int *nc);
+/* ABMC blocking: breadth-first aggregation of up to block_size nodes from the
+ * smallest unassigned node (block ids in creation order), then first-fit
+ * greedy coloring of the block graph in block-id order.  block_of_old[n],
+ * block_color[n] (first *nb used), *nb, *nc. */
+int iccg_abmc_blocking(int n, const int *rp, const int *ci, int block_size,
+                       int *block_of_old, int *nb, int *block_color, int *nc);
+/* ABMC blocking of a structured nx*ny*nz grid (natural index x fastest)
+ * into bx*by*bz boxes, boxes colored first-fit in box order. */
+int iccg_abmc_blocking_grid(int nx, int ny, int nz, int bx, int by, int bz,
+                            int *block_of_old, int *nb, int *block_color,
+                            int *nc);
+
+/* ---- synthetic problems (no reference counterpart) ------------------------ */
+
+/* Symmetric 2D 5-point (nz==1) or 3D 7-point finite-volume matrix on an
+ * nx*ny*nz Dirichlet grid, natural order.  coef_lo_exp == coef_hi_exp == 0
+ * gives the constant-coefficient Laplacian (diag 2d, off-diagonal -1);
+ * otherwise every face gets c = 10^(lo + (hi-lo) u), u ~ U[0,1) drawn from
+ * splitmix64(seed) in face order (x faces, y faces, z faces; each
+ * lexicographic with x fastest), off-diagonal -c, diagonal = sum of the
+ * cell's face coefficients + diag_shift.  Pass rowptr == NULL to get the
+ * sizes only. */
+int iccg_gen_grid(int nx, int ny, int nz, double coef_lo_exp,
+                  double coef_hi_exp, double diag_shift, unsigned long long seed,
+                  int *n_out, long *nnz_out, int *rowptr, int *colind,
+                  double *val);
+/* out[i] ~ N(0,1): splitmix64(seed) uniforms, Box-Muller pairs in FP64. */
+int iccg_gen_normal(long n, unsigned long long seed, double *out);
+
+/* ---- device objects ------------------------------------------------------- */
+
+/* struct CRS (crs_io.hpp:10-15) uploaded to HBM (int32 rowptr/colind, FP64
+ * values).  Arrays are copied; the caller keeps ownership. */
+int iccg_csr_create(int n, long nnz, const int *rowptr, const int *colind,
+                    const double *val, iccg_csr_t *out);
+int iccg_csr_destroy(iccg_csr_t A);
+/* spmv (cg_crs.cpp:54-79) on host buffers (copies in/out; test path). */
+int iccg_spmv(iccg_csr_t A, const double *x, double *y);
+
+/* struct ColorSchedule (color.hpp:9-23), MC_NODE form. */
+int iccg_schedule_create_mc(int n, int nc, const int *color_ptr,
+                            const int *rows_of_color, iccg_sched_t *out);
+/* struct ColorSchedule, ABMC_BLOCK form (block.hpp:30-39 arrays). */
+int iccg_schedule_create_abmc(int n, int nc, int nb, const int *color_ptr_blk,
+                              const int *blocks_of_color, const int *block_ptr,
+                              const int *rows_of_block, iccg_sched_t *out);
+int iccg_schedule_destroy(iccg_sched_t s);
+
+/* Identity(const CRS&) / Jacobi(const CRS&) (precond.hpp:10,23) and
+ * IC0(const CRS&, double shift) (ic0.hpp:13) built on the device.  For IC0,
+ * sched selects the triangular-solve schedule exactly as IC0::apply does
+ * (ic0.cpp:366-393): NULL or nc<=0 -> natural order; ABMC_BLOCK -> ABMC;
+ * MC_NODE -> multicolor.  The schedule is checked against A: a same-color
+ * dependency the reference would race on is rejected (ICCG_ERR_SCHEDULE).
+ * Pivot failures return ICCG_ERR_PIVOT with the reference's message. */
+int iccg_precond_create(iccg_csr_t A, int kind, double shift,
+                        iccg_sched_t sched, int precision,
+                        iccg_precond_t *out);
+int iccg_precond_destroy(iccg_precond_t M);
+/* Re-plans the triangular solves of an IC0 preconditioner for another
+ * schedule (IC0::apply takes the schedule per call, ic0.hpp:34).  The factor
+ * is kept: under every valid schedule it is the same, bit for bit. */
+int iccg_precond_set_schedule(iccg_precond_t M, iccg_sched_t sched);
+/* apply(r, z, sched) (precond.hpp:12,38; ic0.hpp:34) on host buffers. */
+int iccg_precond_apply(iccg_precond_t M, const double *r, double *z);
+/* Copies the FP64 factor out: L (lower+diag CSR) and L^T (CSC of L), in the
+ * layout of IC0's members (ic0.hpp:38-49).  Sizes: n+1 / nnz_L. */
+int iccg_precond_factor_nnz(iccg_precond_t M, long *nnz_L);
+int iccg_precond_get_factor(iccg_precond_t M, int *Lrp, int *Lci, double *Lv,
+                            int *LTcp, int *LTri, double *LTv);
+
+/* conjugate_gradient<Precond> (cg_crs.cpp:89-162) on host buffers:
+ * b[n] in, x[n] in/out (x0 in, solution out). */
+int iccg_cg_solve(iccg_csr_t A, iccg_precond_t M, const double *b, double *x,
+                  int max_iter, double tol, int flags, iccg_cg_result *res);
+
+/* ---- device-resident solver (timing, repeated solves) --------------------- */
+int iccg_solver_create(iccg_csr_t A, iccg_precond_t M, iccg_solver_t *out);
+int iccg_solver_destroy(iccg_solver_t S);
+/* Uploads b and x0 (NULL x0 -> zeros) into the solver's HBM buffers. */
+int iccg_solver_set(iccg_solver_t S, const double *b, const double *x0);
+/* Runs the CG phase on the resident buffers (nothing crosses PCIe inside). */
+int iccg_solver_run(iccg_solver_t S, int max_iter, double tol, int flags,
+                    iccg_cg_result *res);
+int iccg_solver_get_x(iccg_solver_t S, double *x);
+/* Per-kernel-class event timing inside iccg_solver_run (0 = off). */
+int iccg_solver_set_timing(iccg_solver_t S, int on);
+/* Kernel classes: 0 SpMV(+dots), 1 forward sweep, 2 backward sweep,
+ * 3 vector updates.  Returns launches and summed milliseconds since the last
+ * reset. */
+int iccg_solver_get_timing(iccg_solver_t S, int cls, long *launches,
+                           double *total_ms);
+int iccg_solver_reset_timing(iccg_solver_t S);
+
+/* ---- measurement helpers -------------------------------------------------- */
+/* Device copy of nbytes (HBM -> HBM) repeated reps times; average ms per
+ * launch through *avg_ms.  Calibrates the hardware byte counters. */
+int iccg_bench_copy(long nbytes, int reps, double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ICCG_HIP_H */

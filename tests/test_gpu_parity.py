@@ -1,0 +1,244 @@
+"""GPU parity of the HIP solve path against the CPU oracle (same inputs).
+
+Bar: bit-exact for SpMV, the Jacobi scale, the IC(0) factor and the
+triangular solves (every row sum is one lane's ascending fma chain, as the
+reference's loops compile); CG matches the oracle run with double-double
+dots bit for bit, and the oracle with the reference's 80-bit dots to within
+one iteration, with the final residual within 1e-12 when the iteration
+counts agree.
+"""
+import numpy as np
+import pytest
+
+import iccg_amd as I
+from oracle import oracle as O
+from tests import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases():
+    return {
+        "saad": I.read_matrix_market_crs(P.SAAD),
+        "lap2d_30": P.grid(30, 30),
+        "lap3d_10": P.grid(10, 10, 10),
+        "var3d_9": P.grid(9, 8, 7, coef=(-2.0, 2.0), shift=1e-6),
+        "var2d_17": P.grid(17, 13, coef=(-1.0, 1.0)),
+    }
+
+
+@pytest.mark.parametrize("name", ["saad", "lap2d_30", "lap3d_10", "var3d_9"])
+def test_spmv_bitwise(name):
+    A = _cases()[name]
+    x = I.gen_normal(A.n, 11)
+    y = I.DeviceCSR(A).spmv(x)
+    assert np.array_equal(y, O.spmv(P.orc(A), x))
+
+
+def test_spmv_empty_rows():
+    # a row with no entries gives y_i = 0 (sum starts at 0.0)
+    A = I.CRS(3, np.array([0, 1, 1, 3], np.int32), np.array([0, 0, 2], np.int32),
+              np.array([2.0, 1.0, 3.0]))
+    y = I.DeviceCSR(A).spmv(np.array([1.0, 5.0, 2.0]))
+    assert np.array_equal(y, [2.0, 0.0, 7.0])
+
+
+@pytest.mark.parametrize("name", ["saad", "var3d_9"])
+def test_jacobi_bitwise(name):
+    A = _cases()[name]
+    r = I.gen_normal(A.n, 3)
+    z = I.Jacobi(A).apply(r)
+    assert np.array_equal(z, O.jacobi_inv_diag(P.orc(A)) * r)
+
+
+def _factor_equal(M, f):
+    Lrp, Lci, Lv, LTcp, LTri, LTv = M.factor()
+    assert np.array_equal(Lrp, f.Lrp) and np.array_equal(Lci, f.Lci)
+    assert np.array_equal(LTcp, f.LTcp) and np.array_equal(LTri, f.LTri)
+    assert np.array_equal(Lv, f.Lv), np.max(np.abs(Lv - f.Lv))
+    assert np.array_equal(LTv, f.LTv)
+
+
+@pytest.mark.parametrize("name", ["saad", "lap2d_30", "var3d_9", "var2d_17"])
+@pytest.mark.parametrize("order", ["natural", "mc", "abmc"])
+def test_ic0_factor_and_apply_bitwise(name, order):
+    A = _cases()[name]
+    if order == "natural":
+        Ar, s = A, None
+    elif order == "mc":
+        o = P.mc_order(A)
+        Ar, s = o.A, o.sched
+    else:
+        o = P.abmc_order(A, block=8)
+        Ar, s = o.A, o.sched
+    f = O.ic0_factor(P.orc(Ar), 0.0)
+    M = I.IC0(Ar, 0.0, sched=s)
+    _factor_equal(M, f)
+    r = I.gen_normal(Ar.n, 5)
+    z = M.apply(r, s)
+    assert np.array_equal(z, O.ic0_apply(f, r, P.orc_sched(s)))
+    # the MC / ABMC result equals the sequential one bit for bit (valid coloring)
+    assert np.array_equal(z, O.ic0_apply(f, r, None))
+
+
+@pytest.mark.parametrize("B", [4, 16, 64])
+def test_abmc_block_sizes(B):
+    A = P.grid(12, 11, 10, coef=(-2.0, 2.0), shift=1e-6)
+    o = P.abmc_order(A, block=B)
+    f = O.ic0_factor(P.orc(o.A), 0.0)
+    M = I.IC0(o.A, 0.0, sched=o.sched)
+    r = I.gen_normal(A.n, 8)
+    assert np.array_equal(M.apply(r, o.sched), O.ic0_apply(f, r, None))
+
+
+def test_abmc_grid_boxes():
+    A = P.grid(16, 12, 8, coef=(-2.0, 2.0), shift=1e-6)
+    o = P.abmc_order(A, grid_dims=(16, 12, 8), box=(4, 4, 4))
+    assert o.extra["nc"] == 2
+    f = O.ic0_factor(P.orc(o.A), 0.0)
+    M = I.IC0(o.A, 0.0, sched=o.sched)
+    _factor_equal(M, f)
+    r = I.gen_normal(A.n, 9)
+    assert np.array_equal(M.apply(r, o.sched), O.ic0_apply(f, r, None))
+
+
+def _solve(A, kind, sched=None, b=None, tol=1e-8, max_iter=10000, precision="fp64"):
+    if b is None:
+        b = np.ones(A.n)
+    x = np.zeros(A.n)
+    if kind == "none":
+        M = I.Identity(A)
+    elif kind == "jacobi":
+        M = I.Jacobi(A)
+    else:
+        M = I.IC0(A, 0.0, sched=sched, precision=precision)
+    res = I.conjugate_gradient(A, b, x, M, max_iter, tol, sched)
+    return res, x
+
+
+def _oracle(A, kind, sched=None, b=None, tol=1e-8, max_iter=10000, dot_mode=0):
+    if b is None:
+        b = np.ones(A.n)
+    Ao = P.orc(A)
+    f = O.ic0_factor(Ao, 0.0) if kind == "ic0" else None
+    return O.cg(Ao, b, kind, ic=f, sched=P.orc_sched(sched), tol=tol, max_iter=max_iter,
+                dot_mode=dot_mode)
+
+
+@pytest.mark.parametrize("kind", ["none", "jacobi", "ic0"])
+@pytest.mark.parametrize("name", ["saad", "lap2d_30", "var3d_9"])
+def test_cg_matches_oracle(kind, name):
+    A = _cases()[name]
+    res, x = _solve(A, kind)
+    dd = _oracle(A, kind, dot_mode=1)
+    ld = _oracle(A, kind, dot_mode=0)
+    # same arithmetic as the double-double oracle: identical run
+    assert res.iters == dd.iters and res.converged == dd.converged
+    assert res.rel_resid == dd.rel_resid
+    assert np.array_equal(x, dd.x)
+    # against the reference's 80-bit dots: +-1 iteration
+    assert abs(res.iters - ld.iters) <= 1
+    if res.iters == ld.iters:
+        assert abs(res.rel_resid - ld.rel_resid) <= 1e-12
+    err = np.linalg.norm(x - ld.x) / np.linalg.norm(ld.x)
+    assert err < 1e-6
+
+
+def test_cg_c1_shape():
+    """2D 100x100 Laplacian, b = A*1, x0 = 0, tol 1e-8."""
+    A = P.grid(100, 100)
+    b = P.rhs_ones_times_A(A)
+    for kind in ("none", "ic0"):
+        res, x = _solve(A, kind, b=b)
+        ld = _oracle(A, kind, b=b)
+        assert res.converged and abs(res.iters - ld.iters) <= 1
+        if res.iters == ld.iters:
+            assert abs(res.rel_resid - ld.rel_resid) <= 1e-12
+        assert np.linalg.norm(x - ld.x) / np.linalg.norm(ld.x) < 1e-6
+
+
+@pytest.mark.parametrize("order", ["mc", "abmc8", "abmcbox"])
+def test_cg_colored_matches_oracle(order):
+    A = P.grid(20, 18, 16, coef=(-2.0, 2.0), shift=1e-6)
+    if order == "mc":
+        o = P.mc_order(A)
+    elif order == "abmc8":
+        o = P.abmc_order(A, block=8)
+    else:
+        o = P.abmc_order(A, grid_dims=(20, 18, 16), box=(4, 4, 4))
+    b = I.gen_normal(A.n, P.SEED + 1)
+    res, x = _solve(o.A, "ic0", o.sched, b=b)
+    dd = _oracle(o.A, "ic0", o.sched, b=b, dot_mode=1)
+    ld = _oracle(o.A, "ic0", o.sched, b=b, dot_mode=0)
+    assert res.iters == dd.iters and np.array_equal(x, dd.x)
+    assert abs(res.iters - ld.iters) <= 1
+    assert res.converged
+
+
+def test_mixed_precision_true_residual():
+    A = P.grid(24, 20, 16, coef=(-2.0, 2.0), shift=1e-6)
+    o = P.abmc_order(A, grid_dims=(24, 20, 16))
+    b = I.gen_normal(A.n, 4)
+    res, x = _solve(o.A, "ic0", o.sched, b=b, precision="mixed")
+    assert res.converged
+    assert res.true_rel_resid < 1e-8
+    assert O.true_resid(P.orc(o.A), x, b) < 1e-8
+    ld = _oracle(o.A, "ic0", o.sched, b=b)
+    assert np.linalg.norm(x - ld.x) / np.linalg.norm(ld.x) < 1e-5
+
+
+def test_solver_resident_repeat():
+    A = P.grid(32, 32)
+    M = I.IC0(A, 0.0)
+    S = I.Solver(A, M)
+    b = I.gen_normal(A.n, 2)
+    S.set(b)
+    r1 = S.run(10000, 1e-8)
+    x1 = S.x()
+    S.set(b)
+    r2 = S.run(10000, 1e-8)
+    assert r1.iters == r2.iters and np.array_equal(x1, S.x())
+
+
+def test_max_iter_and_zero_rhs():
+    A = P.grid(20, 20)
+    res, _ = _solve(A, "none", max_iter=5)
+    ld = _oracle(A, "none", max_iter=5)
+    assert res.iters == 5 == ld.iters and not res.converged
+    assert abs(res.rel_resid - ld.rel_resid) <= 1e-12
+    # b = 0: normb -> 1, r0 = 0 -> converged at iteration 0 (cg_crs.cpp:109-123)
+    res, x = _solve(A, "ic0", b=np.zeros(A.n))
+    assert res.iters == 0 and res.converged and not x.any()
+
+
+def test_breakdown_indefinite():
+    # indefinite diagonal matrix with Identity: p.Ap <= eps |p||Ap| -> breakdown
+    n = 4
+    A = I.CRS(n, np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32),
+              np.array([1.0, -1.0, 1.0, -1.0]))
+    res, _ = _solve(A, "none")
+    ld = _oracle(A, "none")
+    assert res.iters == ld.iters and res.converged == ld.converged == False  # noqa: E712
+    assert res.breakdown
+
+
+def test_pivot_errors_match_reference_messages():
+    A = P.grid(6, 6)
+    v = A.val.copy()
+    d = np.where(A.colind == np.repeat(np.arange(A.n), np.diff(A.rowptr)))[0]
+    v[d[7]] = -1.0
+    B = I.CRS(A.n, A.rowptr, A.colind, v)
+    with pytest.raises(I.IccgError, match="IC\\(0\\): non-positive pivot at row 7"):
+        I.IC0(B, 0.0)
+    with pytest.raises(O.PivotError, match="non-positive pivot at row 7"):
+        O.ic0_factor(P.orc(B), 0.0)
+
+
+def test_invalid_schedule_rejected():
+    A = P.grid(8, 8)
+    o = P.mc_order(A)
+    bad = np.zeros(A.n, np.int32)  # one color: every dependency is a race
+    s = I.ColorSchedule.mc(1, np.array([0, A.n], np.int32), np.arange(A.n, dtype=np.int32))
+    with pytest.raises(I.IccgError, match="not a valid MC schedule"):
+        I.IC0(o.A, 0.0, sched=s)
+    del bad
