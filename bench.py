@@ -1,0 +1,248 @@
+#!/usr/bin/env python3
+"""Headline benchmark: ABMC IC(0)-CG time to ||r||/||b|| < 1e-8 on one MI355X.
+
+Workload (the reference's abmc_crs path at the configuration its metric is
+quoted on): 3D 7-point variable-coefficient diffusion on a 192^3 Dirichlet
+grid (n = 7,077,888), face coefficients 10^U[-2,2], diagonal = sum of the
+cell's face coefficients + 1e-6, b ~ N(0,1), x0 = 0; ABMC ordering with
+block size 64 (breadth-first blocks) and first-fit greedy block coloring;
+the permuted system P^T A P is solved as abmc_crs does (cg_crs.cpp:413-508).
+
+A step is one complete solve to tolerance with every input resident in
+HBM (the CG phase, as the reference times it, cg_crs.cpp:507-510).  The
+headline runs the mixed-precision preconditioner (FP32 triangular solves,
+FP64 CG recurrence, stop on the FP64 true residual); the plain FP64 path
+runs beside it.  Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("ABMC-ICCG time-to-1e-8 rel.res & HBM GB/s; served solves/s, p50/p99 latency")
+HBM_PEAK_GBS = 8000.0
+SEED = 0x1CC6
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def spmv_bytes(n, nnz, s=8, v=8):
+    return nnz * (s + 4) + 4 * (n + 1) + 2 * n * v
+
+
+def trsv_bytes(n, nnz_l, s, v_in, v):
+    return nnz_l * (s + 4) + 4 * (n + 1) + n * v_in + 2 * n * v
+
+
+def iter_bytes(n, nnz, nnz_l, mixed):
+    if mixed:
+        return (spmv_bytes(n, nnz) + trsv_bytes(n, nnz_l, 4, 8, 4) +
+                trsv_bytes(n, nnz_l, 4, 4, 4) + 80 * n)
+    return spmv_bytes(n, nnz) + 2 * trsv_bytes(n, nnz_l, 8, 8, 8) + 80 * n
+
+
+def build_problem(N, block):
+    import iccg_amd as I
+    t0 = time.time()
+    A = I.gen_grid(N, N, N, coef_exp=(-2.0, 2.0), diag_shift=1e-6, seed=SEED)
+    log(f"grid {N}^3: n={A.n} nnz={A.nnz} ({time.time() - t0:.1f}s)")
+    t0 = time.time()
+    if block == "box":
+        bo, nb, bc, nc = I.abmc_blocking_grid(N, N, N, 4, 4, 4)
+    else:
+        bo, nb, bc, nc = I.abmc_blocking(A, int(block))
+    p = I.BuildPermutationByBlockColor(bo, bc)
+    Ap = I.expand_lower_to_full(I.Permute_PtAP_LowerCRS_to_LowerCRS(A, p))
+    cpb, boc, bp, rob = I.BuildABMCSchedule(p, bo, bc, nb, nc)
+    sched = I.ColorSchedule.abmc(nc, nb, cpb, boc, bp, rob)
+    log(f"ABMC blocks={nb} colors={nc} ({time.time() - t0:.1f}s)")
+    b_old = I.gen_normal(A.n, SEED + 1)
+    b = np.empty_like(b_old)
+    b[p] = b_old  # b'[new_of_old[i]] = b[i]
+    return Ap, sched, b, {"nb": nb, "nc": nc}
+
+
+def time_solves(S, steps, warmup, max_iter, tol, true_res):
+    for _ in range(warmup):
+        S.run(max_iter, tol, true_residual=true_res, zero_x0=True)
+    S.reset_timing()
+    res = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res.append(S.run(max_iter, tol, true_residual=true_res, zero_x0=True))
+    wall = time.perf_counter() - t0
+    return res, wall
+
+
+def kernel_table(S, n, nnz, nnz_l, mixed, nc):
+    names = ["spmv_dots", "fwd_sweep", "bwd_sweep", "vector_ops"]
+    fwd = trsv_bytes(n, nnz_l, 4, 8, 4) if mixed else trsv_bytes(n, nnz_l, 8, 8, 8)
+    bwd = trsv_bytes(n, nnz_l, 4, 4, 4) if mixed else trsv_bytes(n, nnz_l, 8, 8, 8)
+    per_launch = [spmv_bytes(n, nnz), fwd, bwd, 80 * n]
+    out = {}
+    for c, nm in enumerate(names):
+        cnt, ms = S.timing(c)
+        if cnt == 0:
+            continue
+        avg = ms / cnt
+        gbs = per_launch[c] / (avg * 1e-3) / 1e9
+        out[nm] = {"launches": cnt, "avg_ms": round(avg, 5), "bytes": int(per_launch[c]),
+                   "gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                   "unit": "one launch" if c == 0 else
+                   (f"one sweep ({nc} color launches)" if c in (1, 2) else
+                    "x/r update + r.z + p update (3 launches)")}
+    return out
+
+
+def cpu_baseline(Ap, sched, b, iters_to_tol, budget_s):
+    """The reference's own abmc_crs CG (oracle/_ref, compiled from its
+    sources with OpenMP) on the same permuted system, timed over a bounded
+    number of iterations; time to tolerance extrapolated with the GPU FP64
+    iteration count (the same method, so the same count to within one)."""
+    from oracle import ref as R
+    from oracle.oracle import Csr, Schedule
+    if not R.available():
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    R.set_threads(threads)
+    A = Csr(Ap.n, Ap.rowptr, Ap.colind, Ap.val)
+    s = Schedule(nc=sched.nc, mode=1, nb=sched.nb, color_ptr_blk=sched.color_ptr_blk,
+                 blocks_of_color=sched.blocks_of_color, block_ptr=sched.block_ptr,
+                 rows_of_block=sched.rows_of_block)
+    t0 = time.time()
+    R.ic0_factor_only(A, 0.0)
+    log(f"reference IC(0) factorization (serial): {time.time() - t0:.1f}s")
+    k0 = 3
+    it0, _, _, _, t_small = R.cg_timed_ic0(A, b, s, k0, tol=0.0, build_factor=False)
+    per = max(t_small / max(it0, 1), 1e-4)
+    k1 = int(min(max(budget_s / per, k0 + 5), 2000))
+    it1, _, _, _, t_big = R.cg_timed_ic0(A, b, s, k1, tol=0.0, build_factor=False)
+    per_iter = (t_big - t_small) / max(it1 - it0, 1)
+    log(f"reference CG: {it1} iterations in {t_big:.2f}s on {threads} threads "
+        f"({per_iter * 1e3:.2f} ms/iter)")
+    return {"value": round(per_iter * iters_to_tol, 3), "unit": "s (time to 1e-8, extrapolated)",
+            "cores": threads, "kind": "reference",
+            "sample": (f"{it1} iterations of the reference's abmc_crs IC(0)-CG (oracle/_ref, "
+                       f"OpenMP {threads} threads) on the same permuted 192^3 system; "
+                       f"{per_iter * 1e3:.2f} ms/iter x {iters_to_tol} iterations"),
+            "ms_per_iter": round(per_iter * 1e3, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=192)
+    ap.add_argument("--block", default="64", help="ABMC block size, or 'box' (4x4x4 boxes)")
+    ap.add_argument("--max-iter", type=int, default=20000)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--skip-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU design "
+              "is described in DESIGN.md and is not run here", file=sys.stderr)
+        sys.exit(2)
+
+    import iccg_amd as I
+    if I.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    N = args.size
+    Ap, sched, b, info = build_problem(N, args.block)
+    n, nnz = Ap.n, Ap.nnz
+    nnz_l = (nnz + n) // 2
+
+    t0 = time.time()
+    dA = I.DeviceCSR(Ap)
+    M32 = I.IC0(dA, 0.0, sched=sched, precision="mixed")
+    log(f"device setup + IC(0) (mixed): {time.time() - t0:.1f}s")
+    S32 = I.Solver(dA, M32)
+    S32.set(b)
+    S32.set_timing(True)
+    res32, wall32 = time_solves(S32, args.steps, args.warmup, args.max_iter, args.tol, True)
+    ms32 = wall32 / args.steps * 1e3
+    r32 = res32[-1]
+    x32 = S32.x()
+    k32 = kernel_table(S32, n, nnz, nnz_l, True, info["nc"])
+    log(f"mixed: {r32.iters} iters, true rel {r32.true_rel_resid:.3e}, {ms32:.1f} ms/solve, "
+        f"restarts {r32.restarts}")
+    del S32
+
+    fp64 = None
+    if not args.skip_fp64:
+        t0 = time.time()
+        M64 = I.IC0(dA, 0.0, sched=sched, precision="fp64")
+        log(f"IC(0) (fp64): {time.time() - t0:.1f}s")
+        S64 = I.Solver(dA, M64)
+        S64.set(b)
+        S64.set_timing(True)
+        res64, wall64 = time_solves(S64, args.steps, args.warmup, args.max_iter, args.tol, False)
+        ms64 = wall64 / args.steps * 1e3
+        r64 = res64[-1]
+        x64 = S64.x()
+        k64 = kernel_table(S64, n, nnz, nnz_l, False, info["nc"])
+        err = float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))
+        log(f"fp64: {r64.iters} iters, rel {r64.rel_resid:.3e}, true {r64.true_rel_resid:.3e}, "
+            f"{ms64:.1f} ms/solve; mixed vs fp64 rel err {err:.2e}")
+        fp64 = {"time_s": round(ms64 / 1e3, 4), "iters": r64.iters,
+                "rel_resid": r64.rel_resid, "true_rel_resid": r64.true_rel_resid,
+                "gbs": round(iter_bytes(n, nnz, nnz_l, False) * r64.iters / (ms64 * 1e-3) / 1e9, 1),
+                "kernels": k64}
+        del S64, M64
+
+    bytes_it = iter_bytes(n, nnz, nnz_l, True)
+    gbs32 = bytes_it * r32.iters / (ms32 * 1e-3) / 1e9
+    # dominant kernel class of the headline solve
+    dom = max(k32.items(), key=lambda kv: kv[1]["avg_ms"] * kv[1]["launches"])
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{dom[0]}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    roofline = {"bound": "hbm", "kernel": dom[0], "unit_of_launch": dom[1]["unit"],
+                "achieved": dom[1]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": dom[1]["frac"], "traffic": traffic,
+                "algorithmic_bytes_per_launch": dom[1]["bytes"], "avg_ms": dom[1]["avg_ms"]}
+
+    cpu = None
+    if not args.no_cpu:
+        iters_ref = fp64["iters"] if fp64 else r32.iters
+        cpu = cpu_baseline(Ap, sched, b, iters_ref, args.cpu_budget)
+
+    line = {
+        "metric": METRIC, "value": round(ms32 / 1e3, 4), "unit": "s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms32, 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 CG + f32 preconditioner", "data": "synthetic",
+        "config": {"workload": (f"abmc_crs: 3D 7-pt variable-coefficient diffusion {N}^3 "
+                                f"(n={n}, nnz={nnz}), ABMC B={args.block} greedy "
+                                f"({info['nb']} blocks, {info['nc']} colors), b~N(0,1), x0=0, "
+                                f"tol {args.tol:g} on the FP64 true residual"),
+                   "n": n, "nnz": nnz, "tol": args.tol},
+        "iters": r32.iters, "true_rel_resid": r32.true_rel_resid, "restarts": r32.restarts,
+        "hbm_gbs": round(gbs32, 1), "hbm_frac": round(gbs32 / HBM_PEAK_GBS, 4),
+        "bytes_per_iter": int(bytes_it), "kernels": k32,
+        "fp64": fp64,
+        "speedup_mixed_vs_fp64": round(fp64["time_s"] / (ms32 / 1e3), 3) if fp64 else None,
+        "err_vs_fp64": (float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))
+                        if fp64 else None),
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

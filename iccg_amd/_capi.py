@@ -19,6 +19,7 @@ PRECOND_IDENTITY, PRECOND_JACOBI, PRECOND_IC0 = 0, 1, 2
 FP64, MIXED = 0, 1
 SCHED_MC, SCHED_ABMC = 0, 1
 SOLVE_TRUE_RESID = 1
+SOLVE_ZERO_X0 = 2
 
 _vp = C.c_void_p
 _i = C.c_int

@@ -492,13 +492,13 @@ class Solver:
         x0 = None if x0 is None else _f64(x0)
         check(lib().iccg_solver_set(self._h.h, _p(b), _p(x0)))
 
-    def run(self, max_iter, tol, true_residual=None) -> CGResult:
+    def run(self, max_iter, tol, true_residual=None, zero_x0=False) -> CGResult:
         if true_residual is None:
             true_residual = self.M.precision == _capi.MIXED
+        flags = (_capi.SOLVE_TRUE_RESID if true_residual else 0) | \
+            (_capi.SOLVE_ZERO_X0 if zero_x0 else 0)
         res = _capi.CGResultC()
-        check(lib().iccg_solver_run(self._h.h, max_iter, tol,
-                                    _capi.SOLVE_TRUE_RESID if true_residual else 0,
-                                    C.byref(res)))
+        check(lib().iccg_solver_run(self._h.h, max_iter, tol, flags, C.byref(res)))
         return _result(res)
 
     def x(self):

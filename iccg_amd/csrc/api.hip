@@ -50,6 +50,50 @@ static void dfree(void *p) {
   if (p) (void)hipFree(p);
 }
 
+static void free_sell(Sell &S) {
+  dfree(S.off);
+  dfree(S.col);
+  dfree(S.v64);
+  dfree(S.v32);
+  S = Sell();
+}
+
+// Host layout of a 64-row sliced ELL over row ranges [beg(i), end(i)) of a
+// CSR-like index array ci: off[nslice+1], col (padding -1), and for every
+// included CSR position k the ELL position (pos[k]).
+template <class Range>
+static void sell_layout(int n, Range range, const int *ci, std::vector<int> &off,
+                        std::vector<int> &col, int &maxw, std::vector<int> *pos) {
+  const int ns = (n + 63) / 64;
+  off.assign((size_t)ns + 1, 0);
+  maxw = 0;
+  long total = 0;
+  for (int s = 0; s < ns; ++s) {
+    int w = 0;
+    for (int i = s * 64; i < std::min(n, s * 64 + 64); ++i) {
+      int b, e;
+      range(i, b, e);
+      w = std::max(w, e - b);
+    }
+    maxw = std::max(maxw, w);
+    total += 64L * w;
+    require(total < 0x7fffffffL, ICCG_ERR_ARG, "matrix too large for the sliced layout");
+    off[s + 1] = (int)total;
+  }
+  col.assign((size_t)total, -1);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    int b, e;
+    range(i, b, e);
+    const int base = off[i >> 6] + (i & 63);
+    for (int k = b; k < e; ++k) {
+      const int q = base + 64 * (k - b);
+      col[q] = ci[k];
+      if (pos) (*pos)[k] = q;
+    }
+  }
+}
+
 }  // namespace iccg
 
 using namespace iccg;
@@ -67,6 +111,7 @@ struct iccg_csr_s {
     dfree(d.rp);
     dfree(d.ci);
     dfree(d.v);
+    free_sell(d.sell);
     if (st) (void)hipStreamDestroy(st);
   }
 };
@@ -100,6 +145,12 @@ struct iccg_precond_s {
     dfree(F.uv);
     dfree(F.uv32);
     dfree(F.l2u);
+    free_sell(F.ls);
+    free_sell(F.us);
+    dfree(F.d64);
+    dfree(F.d32);
+    dfree(F.l2ls);
+    dfree(F.l2us);
     dfree(sc.d_rows);
     dfree(sc.d_bstart);
     dfree(sc.d_brows);
@@ -406,6 +457,60 @@ static void build_schedule(iccg_precond_s &M, const iccg_sched_s *s, const std::
   HIPCHK(hipStreamSynchronize(st));
 }
 
+// Sliced-ELL copies of the strictly lower / strictly upper factor and its
+// diagonal, in the precision the sweeps use, filled on the device from the
+// FP64 factor.
+static void build_sliced_factor(iccg_precond_s &M, const std::vector<int> &lrp,
+                                const std::vector<int> &lci, const std::vector<int> &urp,
+                                const std::vector<int> &uci, const std::vector<int> &l2u) {
+  Factor &F = M.F;
+  const int n = F.n;
+  const long nnz = F.nnz;
+  hipStream_t st = M.st;
+  std::vector<int> loff, lcol, uoff, ucol;
+  std::vector<int> l2ls((size_t)nnz), upos((size_t)nnz, -1), l2us((size_t)nnz);
+  int lw = 0, uw = 0;
+  sell_layout(
+      n, [&](int i, int &b, int &e) { b = lrp[i]; e = lrp[i + 1] - 1; }, lci.data(), loff, lcol,
+      lw, &l2ls);
+  sell_layout(
+      n, [&](int i, int &b, int &e) { b = urp[i] + 1; e = urp[i + 1]; }, uci.data(), uoff, ucol,
+      uw, &upos);
+  if (std::max(lw, uw) > 16) return;  // wide rows: CSR sweeps only
+  for (int i = 0; i < n; ++i) l2ls[(size_t)lrp[i + 1] - 1] = -1 - i;
+  for (long k = 0; k < nnz; ++k) l2us[k] = upos[(size_t)l2u[k]];
+  auto up = [&](Sell &S, std::vector<int> &off, std::vector<int> &col, int w) {
+    S.n = n;
+    S.nslice = (int)off.size() - 1;
+    S.maxw = w;
+    S.off = dupload(off.data(), off.size(), st);
+    S.col = dupload(col.data(), col.size(), st);
+    const size_t m = std::max<size_t>(col.size(), 1);
+    if (M.mixed) {
+      S.v32 = dalloc<float>(m);
+      HIPCHK(hipMemsetAsync(S.v32, 0, m * sizeof(float), st));
+    } else {
+      S.v64 = dalloc<double>(m);
+      HIPCHK(hipMemsetAsync(S.v64, 0, m * sizeof(double), st));
+    }
+  };
+  up(F.ls, loff, lcol, lw);
+  up(F.us, uoff, ucol, uw);
+  if (M.mixed)
+    F.d32 = dalloc<float>((size_t)n);
+  else
+    F.d64 = dalloc<double>((size_t)n);
+  F.l2ls = dupload(l2ls.data(), l2ls.size(), st);
+  F.l2us = dupload(l2us.data(), l2us.size(), st);
+  F.sell = true;
+  launch_scatter_sell(F, M.mixed, st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  dfree(F.l2ls);
+  dfree(F.l2us);
+  F.l2ls = F.l2us = nullptr;
+}
+
 static void build_ic0(iccg_precond_s &M, const iccg_sched_s *s) {
   const HostCsr &A = M.A->h;
   std::vector<int> lrp, lci, urp, uci, l2u;
@@ -443,6 +548,7 @@ static void build_ic0(iccg_precond_s &M, const iccg_sched_s *s) {
   }
   launch_scatter_u(F, M.mixed, st);
   HIPCHK(hipGetLastError());
+  build_sliced_factor(M, lrp, lci, urp, uci, l2u);
   HIPCHK(hipStreamSynchronize(st));
   dfree(F.l2u);
   F.l2u = nullptr;
@@ -502,8 +608,8 @@ static void precond_init(iccg_solver_s &S) {
   const int n = S.n;
   if (M.kind == ICCG_PRECOND_IC0) {
     launch_fwd(M.sc, M.F, M.mixed, S.r, S.y, S.S, S.st);
-    launch_bwd(M.sc, M.F, M.mixed, S.y, S.z, S.S, S.st);
-    launch_rz(n, S.r, S.z, M.mixed, S.S, S.part, 1, S.st);
+    launch_bwd(M.sc, M.F, M.mixed, S.y, S.z, S.S, S.st, S.r, S.S, S.part, 2);
+    if (!bwd_fuses_rz(M.sc, M.F)) launch_rz(n, S.r, S.z, M.mixed, S.S, S.part, 1, S.st);
     launch_update_p(n, S.z, M.mixed, S.p, S.S, 1, S.st);
   } else {
     launch_precond_diag_init(M.kind == ICCG_PRECOND_JACOBI ? PK_JAC : PK_IDENT, n, S.r,
@@ -530,10 +636,10 @@ static void enqueue_iteration(iccg_solver_s &S, int it) {
     }
     {
       Timer t(S, 2, it);
-      launch_bwd(M.sc, M.F, M.mixed, S.y, S.z, S.S, S.st);
+      launch_bwd(M.sc, M.F, M.mixed, S.y, S.z, S.S, S.st, S.r, S.S, S.part, 1);
     }
     Timer t(S, 3, it);
-    launch_rz(n, S.r, S.z, M.mixed, S.S, S.part, 0, S.st);
+    if (!bwd_fuses_rz(M.sc, M.F)) launch_rz(n, S.r, S.z, M.mixed, S.S, S.part, 0, S.st);
     launch_update_p(n, S.z, M.mixed, S.p, S.S, 0, S.st);
   } else {
     const int pk = M.kind == ICCG_PRECOND_JACOBI ? PK_JAC : PK_IDENT;
@@ -554,6 +660,8 @@ static void run_solver(iccg_solver_s &S, int max_iter, double tol, int flags,
   const DevCsr &A = S.A->d;
   const bool true_resid = (flags & ICCG_SOLVE_TRUE_RESID) != 0;
   auto t0 = std::chrono::steady_clock::now();
+  if (flags & ICCG_SOLVE_ZERO_X0)
+    HIPCHK(hipMemsetAsync(S.x, 0, sizeof(double) * (size_t)S.n, S.st));
   launch_resid(A, S.x, S.b, S.r, S.S, S.part, 1, tol, max_iter, S.st);
   precond_init(S);
   HIPCHK(hipGetLastError());
@@ -651,6 +759,26 @@ int iccg_csr_create(int n, long nnz, const int *rowptr, const int *colind, const
     A->d.rp = dupload(rowptr, (size_t)n + 1, A->st);
     A->d.ci = dupload(colind, (size_t)nnz, A->st);
     A->d.v = dupload(val, (size_t)nnz, A->st);
+    {
+      std::vector<int> off, col, pos((size_t)nnz);
+      int w = 0;
+      sell_layout(
+          n, [&](int i, int &b, int &e) { b = rowptr[i]; e = rowptr[i + 1]; }, colind, off, col,
+          w, &pos);
+      const long padded = off.empty() ? 0 : off.back();
+      if (w <= 16 && padded <= 2 * nnz + 64L * ((n + 63) / 64)) {
+        std::vector<double> v((size_t)std::max<long>(padded, 1), 0.0);
+        for (long k = 0; k < nnz; ++k) v[(size_t)pos[k]] = val[k];
+        Sell &S = A->d.sell;
+        S.n = n;
+        S.nslice = (int)off.size() - 1;
+        S.maxw = w;
+        S.off = dupload(off.data(), off.size(), A->st);
+        S.col = dupload(col.data(), col.size(), A->st);
+        S.v64 = dupload(v.data(), v.size(), A->st);
+        HIPCHK(hipStreamSynchronize(A->st));
+      }
+    }
     HIPCHK(hipStreamSynchronize(A->st));
     *out = A.release();
   });

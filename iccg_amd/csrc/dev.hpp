@@ -46,11 +46,27 @@ enum PrecondKind { PK_IDENT = 0, PK_JAC = 1, PK_IC = 2 };
 // arrival-counter slots
 enum { SLOT_SPMV = 0, SLOT_XR = 1, SLOT_RESID = 2, SLOT_RZ = 3, SLOT_PINIT = 4, SLOT_TRUE = 5 };
 
+// Sliced ELL with 64-row slices (one wavefront per slice): entry k of row
+// i = 64 s + l sits at off[s] + 64 k + l, so the k-th load of a wavefront is
+// one contiguous 512-byte (f64) run.  A row keeps its stored (ascending
+// column) order; padding carries column -1 and is skipped, so a row sum is
+// the same fma chain as in CSR, bit for bit.
+struct Sell {
+  int n = 0;
+  int nslice = 0;
+  int maxw = 0;          // widest slice
+  int *off = nullptr;    // nslice + 1 (entry offsets, multiples of 64)
+  int *col = nullptr;    // off[nslice], -1 = padding
+  double *v64 = nullptr;
+  float *v32 = nullptr;
+};
+
 struct DevCsr {
   int n = 0;
   long nnz = 0;
   int *rp = nullptr, *ci = nullptr;
   double *v = nullptr;
+  Sell sell;             // used by the SpMV kernels when sell.off != nullptr
 };
 
 // Triangular-solve / factorization schedule on the device.
@@ -69,6 +85,8 @@ struct DevSched {
   uint16_t *d_lev_b = nullptr;     // per-row backward level
   int *d_nlev_f = nullptr, *d_nlev_b = nullptr;  // ABMC: per block
   int max_block = 0;
+  int nb = 0;
+  std::vector<int> h_bstart;       // ABMC: host copy of bstart
   // natural levels: level sets and launch segments of the forward and
   // backward plans.  A segment {0, l0, l1} runs levels [l0, l1) inside one
   // workgroup (narrow levels); {1, l, 0} runs level l over many workgroups.
@@ -90,6 +108,14 @@ struct Factor {
   double *uv = nullptr;
   float *uv32 = nullptr;
   int *l2u = nullptr;
+  // sliced-ELL copies of the strictly lower (L) and strictly upper (U = L^T)
+  // parts and the diagonal, filled from lv after the factorization
+  Sell ls, us;
+  double *d64 = nullptr;
+  float *d32 = nullptr;
+  int *l2ls = nullptr;   // L CSR position -> ls entry, or -1 - row (diagonal)
+  int *l2us = nullptr;   // L CSR position -> us entry, or -1 (diagonal)
+  bool sell = false;
 };
 
 // ---- launchers (kernels.hip) ----------------------------------------------
@@ -115,11 +141,17 @@ void launch_jacobi_apply(int n, const double *inv_diag, const double *r, double 
                          hipStream_t st);
 void launch_jacobi_setup(const DevCsr &A, double *inv_diag, hipStream_t st);
 
-// sweeps: skip when S != nullptr && S->done
+// sweeps: skip when S != nullptr && S->done.  With rz != 0 the backward
+// sweep also forms r.z and, in its last workgroup, beta (rz == 2: rz_old at
+// the start of a solve), like launch_rz.
 void launch_fwd(const DevSched &sc, const Factor &F, bool mixed, const double *r, void *y,
                 const Scal *S, hipStream_t st);
 void launch_bwd(const DevSched &sc, const Factor &F, bool mixed, const void *y, void *z,
-                const Scal *S, hipStream_t st);
+                const Scal *S, hipStream_t st, const double *r = nullptr, Scal *Sw = nullptr,
+                double *part = nullptr, int rz = 0);
+// true when launch_bwd fuses r.z for this schedule (sliced MC / ABMC plans)
+bool bwd_fuses_rz(const DevSched &sc, const Factor &F);
+void launch_scatter_sell(const Factor &F, bool mixed, hipStream_t st);
 void launch_ic0_factor(const DevSched &sc, const Factor &F, double shift,
                        unsigned long long *err, hipStream_t st);
 void launch_scatter_u(const Factor &F, bool mixed, hipStream_t st);

@@ -12,7 +12,11 @@
 //     workgroup to finish a reducing kernel folds all partials in workgroup
 //     order and updates Scal, so a CG iteration is a fixed launch sequence
 //     with no host round trip.
+// Memory layout for HBM bandwidth: matrices in 64-row sliced ELL (one
+// wavefront per slice, every load a contiguous 512-byte run); streaming
+// vector kernels move 32 bytes per lane per access (two 16-byte loads).
 // Compiled with -ffp-contract=off: only the fma() calls written here fuse.
+#include <algorithm>
 #include <cfloat>
 
 #include "dev.hpp"
@@ -43,8 +47,6 @@ __device__ __forceinline__ void dd_fma(double &h, double &l, double a, double b)
   l += e + pe;
 }
 
-__device__ __forceinline__ double dd_round(double h, double l) { return h + l; }
-
 template <int NC>
 __device__ __forceinline__ void wave_dd_reduce(double (&h)[NC], double (&l)[NC]) {
 #pragma unroll
@@ -58,13 +60,15 @@ __device__ __forceinline__ void wave_dd_reduce(double (&h)[NC], double (&l)[NC])
   }
 }
 
-// Block reduction of NC double-double sums; the workgroup that arrives last
-// folds every workgroup's partial in workgroup order.  Returns true on
-// thread 0 of that workgroup, with out[] = the rounded totals.
+// Block reduction of NC double-double sums.  Workgroup gidx of `total`
+// workgroups (over one or several launches on one stream) stores its
+// partial; the one that arrives last folds all partials in index order.
+// Returns true on thread 0 of that workgroup, with out[] = rounded totals.
+// blockDim.x must be a multiple of 64, at most 1024.
 template <int NC>
 __device__ bool reduce_last(double (&h)[NC], double (&l)[NC], double *part,
-                            unsigned int *arrive, double (&out)[NC]) {
-  __shared__ double sh[NC][2][kWG / 64];
+                            unsigned int *arrive, int gidx, int total, double (&out)[NC]) {
+  __shared__ double sh[NC][2][16];
   __shared__ int s_last;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nw = blockDim.x >> 6;
@@ -82,12 +86,12 @@ __device__ bool reduce_last(double (&h)[NC], double (&l)[NC], double *part,
     for (int c = 0; c < NC; ++c) {
       double ah = sh[c][0][0], al = sh[c][1][0];
       for (int k = 1; k < nw; ++k) dd_add(ah, al, sh[c][0][k], sh[c][1][k]);
-      part[(blockIdx.x * NC + c) * 2 + 0] = ah;
-      part[(blockIdx.x * NC + c) * 2 + 1] = al;
+      part[(gidx * NC + c) * 2 + 0] = ah;
+      part[(gidx * NC + c) * 2 + 1] = al;
     }
     __threadfence();
     unsigned int t = atomicAdd(arrive, 1u);
-    s_last = (t == gridDim.x - 1);
+    s_last = (t == (unsigned)total - 1u);
   }
   __syncthreads();
   if (!s_last) return false;
@@ -95,7 +99,7 @@ __device__ bool reduce_last(double (&h)[NC], double (&l)[NC], double *part,
   double h2[NC], l2[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) h2[c] = l2[c] = 0.0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+  for (int b = threadIdx.x; b < total; b += blockDim.x) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const volatile double *q = part + (b * NC + c) * 2;
@@ -117,14 +121,28 @@ __device__ bool reduce_last(double (&h)[NC], double (&l)[NC], double *part,
   for (int c = 0; c < NC; ++c) {
     double ah = sh[c][0][0], al = sh[c][1][0];
     for (int k = 1; k < nw; ++k) dd_add(ah, al, sh[c][0][k], sh[c][1][k]);
-    out[c] = dd_round(ah, al);
+    out[c] = ah + al;
   }
   *arrive = 0u;
   return true;
 }
 
-static inline int red_grid(long n) {
-  long g = (n + kWG - 1) / kWG;
+template <int NC>
+__device__ __forceinline__ bool reduce_last(double (&h)[NC], double (&l)[NC], double *part,
+                                            unsigned int *arrive, double (&out)[NC]) {
+  return reduce_last<NC>(h, l, part, arrive, blockIdx.x, gridDim.x, out);
+}
+
+// Makes this wavefront's LDS writes visible to its own lanes (rocPRIM's
+// wave_barrier: no s_barrier, only ordering).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+static inline int red_grid(long items_per_thread_block_unit) {
+  long g = items_per_thread_block_unit;
   if (g < 1) g = 1;
   if (g > kMaxRedBlocks) g = kMaxRedBlocks;
   return (int)g;
@@ -135,6 +153,71 @@ static inline int stream_grid(long n) {
   if (g < 1) g = 1;
   if (g > 65536) g = 65536;
   return (int)g;
+}
+
+// grid of a streaming kernel that takes 4 elements per lane per step
+static inline int quad_grid(long n) { return red_grid((n + 4L * kWG - 1) / (4L * kWG)); }
+// grid of a kernel that takes one 64-row slice per wavefront per step
+static inline int slice_grid(long nslices) { return red_grid((nslices + 3) / 4); }
+
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return fma(a, b, c); }
+
+// 4-element vector loads/stores (i multiple of 4; HBM buffers 256-B aligned)
+__device__ __forceinline__ void ld4(const double *p, int i, double (&o)[4]) {
+  const double2 u = *reinterpret_cast<const double2 *>(p + i);
+  const double2 w = *reinterpret_cast<const double2 *>(p + i + 2);
+  o[0] = u.x; o[1] = u.y; o[2] = w.x; o[3] = w.y;
+}
+__device__ __forceinline__ void ld4(const float *p, int i, double (&o)[4]) {
+  const float4 u = *reinterpret_cast<const float4 *>(p + i);
+  o[0] = u.x; o[1] = u.y; o[2] = u.z; o[3] = u.w;
+}
+__device__ __forceinline__ void st4(double *p, int i, const double (&o)[4]) {
+  *reinterpret_cast<double2 *>(p + i) = make_double2(o[0], o[1]);
+  *reinterpret_cast<double2 *>(p + i + 2) = make_double2(o[2], o[3]);
+}
+
+// ---------------------------------------------------------------------------
+// Sliced-ELL row chains
+// ---------------------------------------------------------------------------
+// acc op= sum_k v_k * x[c_k] over the entries of the lane's row in slice s,
+// ascending k; NEG selects acc - v*x (triangular solves) over acc + v*x.
+template <class VT, class AT, class XT, int WM, bool NEG>
+__device__ __forceinline__ AT sell_chain(const int *__restrict__ off, const int *__restrict__ col,
+                                         const VT *__restrict__ v, const XT *__restrict__ x,
+                                         int s, int lane, AT acc) {
+  const int o0 = off[s];
+  const int w = (off[s + 1] - o0) >> 6;
+  const int base = o0 + lane;
+  if (WM > 0) {
+    int cc[WM > 0 ? WM : 1];
+    VT vv[WM > 0 ? WM : 1];
+#pragma unroll
+    for (int k = 0; k < WM; ++k) {
+      cc[k] = -1;
+      vv[k] = (VT)0;
+      if (k < w) {
+        cc[k] = col[base + 64 * k];
+        vv[k] = v[base + 64 * k];
+      }
+    }
+    AT xv[WM > 0 ? WM : 1];
+#pragma unroll
+    for (int k = 0; k < WM; ++k) xv[k] = cc[k] >= 0 ? (AT)x[cc[k]] : (AT)0;
+#pragma unroll
+    for (int k = 0; k < WM; ++k)
+      if (cc[k] >= 0) acc = NEG ? fma_t(-(AT)vv[k], xv[k], acc) : fma_t((AT)vv[k], xv[k], acc);
+  } else {
+    for (int k = 0; k < w; ++k) {
+      const int c = col[base + 64 * k];
+      if (c >= 0) {
+        const AT a = (AT)v[base + 64 * k];
+        acc = NEG ? fma_t(-a, (AT)x[c], acc) : fma_t(a, (AT)x[c], acc);
+      }
+    }
+  }
+  return acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -149,33 +232,47 @@ __device__ __forceinline__ double row_dot(const int *__restrict__ rp, const int 
   return sum;
 }
 
-__global__ __launch_bounds__(kWG) void k_spmv_plain(int n, const int *__restrict__ rp,
-                                                    const int *__restrict__ ci,
-                                                    const double *__restrict__ v,
-                                                    const double *__restrict__ x,
+// Visits the rows of A: sliced ELL (one wavefront per 64-row slice) or CSR.
+// f(i, Ax_i) for every row i < n; lanes past n get no call.
+template <int WM, class F>
+__device__ __forceinline__ void for_rows_Ax(const DevCsr &A, const double *__restrict__ x, F f) {
+  if (A.sell.off) {
+    const int lane = threadIdx.x & 63;
+    const int wv = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int nwv = gridDim.x * (blockDim.x >> 6);
+    for (int s = wv; s < A.sell.nslice; s += nwv) {
+      const double y = sell_chain<double, double, double, WM, false>(A.sell.off, A.sell.col,
+                                                                    A.sell.v64, x, s, lane, 0.0);
+      const int i = (s << 6) + lane;
+      if (i < A.n) f(i, y);
+    }
+  } else {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x)
+      f(i, row_dot(A.rp, A.ci, A.v, x, i));
+  }
+}
+
+template <int WM>
+__global__ __launch_bounds__(kWG) void k_spmv_plain(DevCsr A, const double *__restrict__ x,
                                                     double *__restrict__ y) {
-  for (int i = blockIdx.x * kWG + threadIdx.x; i < n; i += gridDim.x * kWG)
-    y[i] = row_dot(rp, ci, v, x, i);
+  for_rows_Ax<WM>(A, x, [&](int i, double v) { y[i] = v; });
 }
 
 // Ap = A p with p.Ap, ||p||^2, ||Ap||^2; the last workgroup applies the
 // breakdown test (cg_crs.cpp:127-133) and computes alpha (135).
-__global__ __launch_bounds__(kWG) void k_spmv_cg(int n, const int *__restrict__ rp,
-                                                 const int *__restrict__ ci,
-                                                 const double *__restrict__ v,
-                                                 const double *__restrict__ p,
+template <int WM>
+__global__ __launch_bounds__(kWG) void k_spmv_cg(DevCsr A, const double *__restrict__ p,
                                                  double *__restrict__ Ap, Scal *S,
                                                  double *part) {
   if (S->done) return;
   double h[3] = {0, 0, 0}, l[3] = {0, 0, 0};
-  for (int i = blockIdx.x * kWG + threadIdx.x; i < n; i += gridDim.x * kWG) {
-    double y = row_dot(rp, ci, v, p, i);
+  for_rows_Ax<WM>(A, p, [&](int i, double y) {
     Ap[i] = y;
-    double pi = p[i];
+    const double pi = p[i];
     dd_fma(h[0], l[0], pi, y);
     dd_fma(h[1], l[1], pi, pi);
     dd_fma(h[2], l[2], y, y);
-  }
+  });
   double out[3];
   if (reduce_last<3>(h, l, part, &S->arrive[SLOT_SPMV], out)) {
     const double pAp = out[0];
@@ -191,22 +288,19 @@ __global__ __launch_bounds__(kWG) void k_spmv_cg(int n, const int *__restrict__ 
 }
 
 // r = b - A x (cg_crs.cpp:102-110, 118-123; true residual 517-528)
-__global__ __launch_bounds__(kWG) void k_resid(int n, const int *__restrict__ rp,
-                                               const int *__restrict__ ci,
-                                               const double *__restrict__ v,
-                                               const double *__restrict__ x,
+template <int WM>
+__global__ __launch_bounds__(kWG) void k_resid(DevCsr A, const double *__restrict__ x,
                                                const double *__restrict__ b,
                                                double *__restrict__ r, Scal *S, double *part,
                                                int mode, double tol, int max_iter) {
   double h[2] = {0, 0}, l[2] = {0, 0};
-  for (int i = blockIdx.x * kWG + threadIdx.x; i < n; i += gridDim.x * kWG) {
-    double ax = row_dot(rp, ci, v, x, i);
-    double bi = b[i];
-    double ri = bi - ax;
+  for_rows_Ax<WM>(A, x, [&](int i, double ax) {
+    const double bi = b[i];
+    const double ri = bi - ax;
     if (mode != 0) r[i] = ri;
     dd_fma(h[0], l[0], bi, bi);
     dd_fma(h[1], l[1], ri, ri);
-  }
+  });
   double out[2];
   if (reduce_last<2>(h, l, part, &S->arrive[SLOT_RESID], out)) {
     double normb = sqrt(out[0]);
@@ -236,6 +330,15 @@ __global__ __launch_bounds__(kWG) void k_resid(int n, const int *__restrict__ rp
   }
 }
 
+// ---------------------------------------------------------------------------
+// Streaming vector kernels (4 elements per lane per step)
+// ---------------------------------------------------------------------------
+#define QUAD_LOOP(n)                                                                  \
+  for (int q_ = blockIdx.x * kWG + threadIdx.x, nq_ = (n) >> 2; q_ < nq_;            \
+       q_ += gridDim.x * kWG)
+#define TAIL_LOOP(n) \
+  for (int i = (((n) >> 2) << 2) + blockIdx.x * kWG + threadIdx.x; i < (n); i += gridDim.x * kWG)
+
 // x += alpha p; r -= alpha Ap (cg_crs.cpp:137-141) with ||r|| and the stop
 // test (143-149).  For Identity / Jacobi the preconditioner (z = r or
 // z = D^-1 r, precond.hpp:12-14, 38-42) and r.z / beta (152-153) fuse here.
@@ -249,14 +352,41 @@ __global__ __launch_bounds__(kWG) void k_xr(int n, double *__restrict__ x,
   if (S->done) return;
   const double alpha = S->alpha;
   double h[2] = {0, 0}, l[2] = {0, 0};
-  for (int i = blockIdx.x * kWG + threadIdx.x; i < n; i += gridDim.x * kWG) {
-    double xi = fma(alpha, p[i], x[i]);
-    double ri = fma(-alpha, Ap[i], r[i]);
+  QUAD_LOOP(n) {
+    const int i = q_ << 2;
+    double xv[4], pv[4], rv[4], av[4];
+    ld4(x, i, xv);
+    ld4(p, i, pv);
+    ld4(r, i, rv);
+    ld4(Ap, i, av);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      xv[e] = fma(alpha, pv[e], xv[e]);
+      rv[e] = fma(-alpha, av[e], rv[e]);
+    }
+    st4(x, i, xv);
+    st4(r, i, rv);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) dd_fma(h[0], l[0], rv[e], rv[e]);
+    if (PK == PK_JAC) {
+      double dv[4], zv[4];
+      ld4(inv_diag, i, dv);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        zv[e] = dv[e] * rv[e];
+        dd_fma(h[1], l[1], rv[e], zv[e]);
+      }
+      st4(z, i, zv);
+    }
+  }
+  TAIL_LOOP(n) {
+    const double xi = fma(alpha, p[i], x[i]);
+    const double ri = fma(-alpha, Ap[i], r[i]);
     x[i] = xi;
     r[i] = ri;
     dd_fma(h[0], l[0], ri, ri);
     if (PK == PK_JAC) {
-      double zi = inv_diag[i] * ri;
+      const double zi = inv_diag[i] * ri;
       z[i] = zi;
       dd_fma(h[1], l[1], ri, zi);
     }
@@ -290,14 +420,23 @@ __global__ __launch_bounds__(kWG) void k_precond_diag_init(int n, const double *
   if (S->done) return;
   double h[1] = {0}, l[1] = {0};
   for (int i = blockIdx.x * kWG + threadIdx.x; i < n; i += gridDim.x * kWG) {
-    double ri = r[i];
-    double zi = (PK == PK_JAC) ? inv_diag[i] * ri : ri;
+    const double ri = r[i];
+    const double zi = (PK == PK_JAC) ? inv_diag[i] * ri : ri;
     if (PK == PK_JAC) z[i] = zi;
     p[i] = zi;
     dd_fma(h[0], l[0], ri, zi);
   }
   double out[1];
   if (reduce_last<1>(h, l, part, &S->arrive[SLOT_PINIT], out)) S->rz_old = out[0];
+}
+
+__device__ __forceinline__ void rz_finish(Scal *S, double rz, int init) {
+  if (init) {
+    S->rz_old = rz;
+  } else {
+    S->beta = rz / S->rz_old;
+    S->rz_old = rz;
+  }
 }
 
 // rz_new = r.z; beta = rz_new / rz_old (cg_crs.cpp:152-153), or rz_old at
@@ -308,17 +447,17 @@ __global__ __launch_bounds__(kWG) void k_rz(int n, const double *__restrict__ r,
                                             int init) {
   if (S->done) return;
   double h[1] = {0}, l[1] = {0};
-  for (int i = blockIdx.x * kWG + threadIdx.x; i < n; i += gridDim.x * kWG)
-    dd_fma(h[0], l[0], r[i], (double)z[i]);
-  double out[1];
-  if (reduce_last<1>(h, l, part, &S->arrive[SLOT_RZ], out)) {
-    if (init) {
-      S->rz_old = out[0];
-    } else {
-      S->beta = out[0] / S->rz_old;
-      S->rz_old = out[0];
-    }
+  QUAD_LOOP(n) {
+    const int i = q_ << 2;
+    double rv[4], zv[4];
+    ld4(r, i, rv);
+    ld4(z, i, zv);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) dd_fma(h[0], l[0], rv[e], zv[e]);
   }
+  TAIL_LOOP(n) { dd_fma(h[0], l[0], r[i], (double)z[i]); }
+  double out[1];
+  if (reduce_last<1>(h, l, part, &S->arrive[SLOT_RZ], out)) rz_finish(S, out[0], init);
 }
 
 // p = z + beta p (cg_crs.cpp:155-157), or p = z at start (114)
@@ -328,8 +467,21 @@ __global__ __launch_bounds__(kWG) void k_update_p(int n, const ZT *__restrict__ 
                                                   int init) {
   if (S->done) return;
   const double beta = S->beta;
-  for (int i = blockIdx.x * kWG + threadIdx.x; i < n; i += gridDim.x * kWG) {
-    double zi = (double)z[i];
+  QUAD_LOOP(n) {
+    const int i = q_ << 2;
+    double zv[4], pv[4];
+    ld4(z, i, zv);
+    if (init) {
+      st4(p, i, zv);
+    } else {
+      ld4(p, i, pv);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) pv[e] = fma(beta, pv[e], zv[e]);
+      st4(p, i, pv);
+    }
+  }
+  TAIL_LOOP(n) {
+    const double zi = (double)z[i];
     p[i] = init ? zi : fma(beta, p[i], zi);
   }
 }
@@ -355,10 +507,9 @@ __global__ void k_jacobi_apply(int n, const double *__restrict__ inv,
 }
 
 // ---------------------------------------------------------------------------
-// Row operators of the triangular solves and the factorization
+// CSR row operators (natural-order level schedule, factorization, and the
+// general fallbacks)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
-__device__ __forceinline__ double fma_t(double a, double b, double c) { return fma(a, b, c); }
 
 // L y = r, one row (ic0.cpp:153-167 / 201-214): j<i terms in stored order,
 // diagonal stored last in the row.
@@ -440,9 +591,6 @@ struct IC0Row {
   }
 };
 
-// ---------------------------------------------------------------------------
-// Schedule drivers
-// ---------------------------------------------------------------------------
 template <class F>
 __global__ __launch_bounds__(kWG) void k_range(int r0, int r1, F f, const Scal *S) {
   if (S && S->done) return;
@@ -470,8 +618,8 @@ __global__ __launch_bounds__(1024) void k_levels_1wg(const int *__restrict__ lvl
   }
 }
 
-// ABMC: one workgroup per block; blocks of one color run concurrently; rows
-// of a block follow their in-block dependency levels.
+// ABMC, general blocks: one workgroup per block; rows of a block follow
+// their in-block dependency levels.
 template <class F>
 __global__ void k_abmc(const int *__restrict__ bstart, const int *__restrict__ brows,
                        const uint16_t *__restrict__ lev, const int *__restrict__ nlev, int kb0,
@@ -489,9 +637,345 @@ __global__ void k_abmc(const int *__restrict__ bstart, const int *__restrict__ b
   }
 }
 
+// ---------------------------------------------------------------------------
+// Sliced-ELL multicolor sweeps (ic0.cpp:194-249): rows [c0, c1) of one
+// color, one wavefront per 64-row slice.  Every dependency of a row lies in
+// an earlier (forward) / later (backward) color, finished by a previous
+// launch.
+// ---------------------------------------------------------------------------
+template <class VT, class YT, int WM>
+__global__ __launch_bounds__(kWG) void k_mc_fwd(int c0, int c1, Sell L, const VT *__restrict__ lv,
+                                                const VT *__restrict__ diag,
+                                                const double *__restrict__ r,
+                                                YT *__restrict__ y, const Scal *S) {
+  if (S && S->done) return;
+  const int lane = threadIdx.x & 63;
+  const int wv = blockIdx.x * (kWG >> 6) + (threadIdx.x >> 6), nwv = gridDim.x * (kWG >> 6);
+  const int s0 = c0 >> 6, s1 = (c1 - 1) >> 6;
+  for (int s = s0 + wv; s <= s1; s += nwv) {
+    const int i = (s << 6) + lane;
+    if (i < c0 || i >= c1) continue;
+    YT acc = (YT)r[i];
+    acc = sell_chain<VT, YT, YT, WM, true>(L.off, L.col, lv, y, s, lane, acc);
+    y[i] = acc / (YT)diag[i];
+  }
+}
+
+template <class VT, class YT, int WM, bool RZ>
+__global__ __launch_bounds__(kWG) void k_mc_bwd(int c0, int c1, Sell U, const VT *__restrict__ uv,
+                                                const VT *__restrict__ diag,
+                                                const YT *__restrict__ y, YT *__restrict__ z,
+                                                const double *__restrict__ r, Scal *S,
+                                                double *part, int gidx0, int total, int init) {
+  if (S && S->done) return;
+  const int lane = threadIdx.x & 63;
+  const int wv = blockIdx.x * (kWG >> 6) + (threadIdx.x >> 6), nwv = gridDim.x * (kWG >> 6);
+  const int s0 = c0 >> 6, s1 = (c1 - 1) >> 6;
+  double h[1] = {0}, l[1] = {0};
+  for (int s = s0 + wv; s <= s1; s += nwv) {
+    const int i = (s << 6) + lane;
+    if (i < c0 || i >= c1) continue;
+    YT acc = y[i];
+    acc = sell_chain<VT, YT, YT, WM, true>(U.off, U.col, uv, z, s, lane, acc);
+    const YT zi = acc / (YT)diag[i];
+    z[i] = zi;
+    if (RZ) dd_fma(h[0], l[0], r[i], (double)zi);
+  }
+  if (RZ) {
+    double out[1];
+    if (reduce_last<1>(h, l, part, &S->arrive[SLOT_RZ], gidx0 + blockIdx.x, total, out))
+      rz_finish(S, out[0], init);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ABMC sweeps for blocks of at most 64 rows (ic0.cpp:263-339): one wavefront
+// per block, blocks of a color spread over the grid.  A block's rows are the
+// contiguous new ids [bs, be) and lane t owns row bs + t.
+//   forward : entries with column < bs belong to earlier colors (final) and
+//             come first in the ascending row; they are summed before the
+//             level loop.  In-block entries [bs, i) are read from LDS as the
+//             in-block levels complete.
+//   backward: in-block entries (i, be) come first, then columns >= be of
+//             later colors, whose values are prefetched before the levels.
+// The per-row operation order equals the sequential reference, bit for bit.
+// ---------------------------------------------------------------------------
+template <class VT, class YT, int WM>
+__global__ __launch_bounds__(kWG) void k_abmc_fwd_wave(int kb0, int kb1,
+                                                       const int *__restrict__ bstart,
+                                                       const uint16_t *__restrict__ lev,
+                                                       const int *__restrict__ nlev, Sell L,
+                                                       const VT *__restrict__ lv,
+                                                       const VT *__restrict__ diag,
+                                                       const double *__restrict__ r,
+                                                       YT *__restrict__ y, const Scal *S) {
+  __shared__ YT ys[kWG];
+  if (S && S->done) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  YT *my = ys + (w << 6);
+  for (int kb = kb0 + blockIdx.x * (kWG >> 6) + w; kb < kb1; kb += gridDim.x * (kWG >> 6)) {
+    const int bs = bstart[kb], be = bstart[kb + 1];
+    const int i = bs + lane;
+    const bool act = i < be;
+    int cc[WM];
+    VT vv[WM];
+    YT acc = (YT)0, d = (YT)1;
+    int mylev = -1;
+    if (act) {
+      const int s = i >> 6, ln = i & 63;
+      const int o0 = L.off[s];
+      const int wd = (L.off[s + 1] - o0) >> 6;
+      const int base = o0 + ln;
+#pragma unroll
+      for (int k = 0; k < WM; ++k) {
+        cc[k] = -1;
+        vv[k] = (VT)0;
+        if (k < wd) {
+          cc[k] = L.col[base + 64 * k];
+          vv[k] = lv[base + 64 * k];
+        }
+      }
+      mylev = lev[i];
+      d = (YT)diag[i];
+      acc = (YT)r[i];
+      YT xo[WM];
+#pragma unroll
+      for (int k = 0; k < WM; ++k) xo[k] = (cc[k] >= 0 && cc[k] < bs) ? y[cc[k]] : (YT)0;
+#pragma unroll
+      for (int k = 0; k < WM; ++k)
+        if (cc[k] >= 0 && cc[k] < bs) acc = fma_t(-(YT)vv[k], xo[k], acc);
+    }
+    const int nl = nlev[kb];
+    for (int l = 0; l < nl; ++l) {
+      if (mylev == l) {
+#pragma unroll
+        for (int k = 0; k < WM; ++k)
+          if (cc[k] >= bs) acc = fma_t(-(YT)vv[k], my[cc[k] - bs], acc);
+        const YT yi = acc / d;
+        my[lane] = yi;
+        y[i] = yi;
+      }
+      wave_sync();
+    }
+  }
+}
+
+template <class VT, class YT, int WM, bool RZ>
+__global__ __launch_bounds__(kWG) void k_abmc_bwd_wave(
+    int kb0, int kb1, const int *__restrict__ bstart, const uint16_t *__restrict__ lev,
+    const int *__restrict__ nlev, Sell U, const VT *__restrict__ uv, const VT *__restrict__ diag,
+    const YT *__restrict__ y, YT *__restrict__ z, const double *__restrict__ r, Scal *S,
+    double *part, int gidx0, int total, int init) {
+  __shared__ YT zs[kWG];
+  if (S && S->done) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  YT *my = zs + (w << 6);
+  double h[1] = {0}, l[1] = {0};
+  for (int kb = kb0 + blockIdx.x * (kWG >> 6) + w; kb < kb1; kb += gridDim.x * (kWG >> 6)) {
+    const int bs = bstart[kb], be = bstart[kb + 1];
+    const int i = bs + lane;
+    const bool act = i < be;
+    int cc[WM];
+    VT vv[WM];
+    YT xo[WM];
+    YT acc = (YT)0, d = (YT)1, zi = (YT)0;
+    double ri = 0.0;
+    int mylev = -1;
+    if (act) {
+      const int s = i >> 6, ln = i & 63;
+      const int o0 = U.off[s];
+      const int wd = (U.off[s + 1] - o0) >> 6;
+      const int base = o0 + ln;
+#pragma unroll
+      for (int k = 0; k < WM; ++k) {
+        cc[k] = -1;
+        vv[k] = (VT)0;
+        if (k < wd) {
+          cc[k] = U.col[base + 64 * k];
+          vv[k] = uv[base + 64 * k];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < WM; ++k) xo[k] = cc[k] >= be ? z[cc[k]] : (YT)0;
+      mylev = lev[i];
+      d = (YT)diag[i];
+      acc = y[i];
+      if (RZ) ri = r[i];
+    }
+    const int nl = nlev[kb];
+    for (int lv = 0; lv < nl; ++lv) {
+      if (mylev == lv) {
+#pragma unroll
+        for (int k = 0; k < WM; ++k)
+          if (cc[k] >= 0 && cc[k] < be) acc = fma_t(-(YT)vv[k], my[cc[k] - bs], acc);
+#pragma unroll
+        for (int k = 0; k < WM; ++k)
+          if (cc[k] >= be) acc = fma_t(-(YT)vv[k], xo[k], acc);
+        zi = acc / d;
+        my[lane] = zi;
+        z[i] = zi;
+      }
+      wave_sync();
+    }
+    if (RZ && act) dd_fma(h[0], l[0], ri, (double)zi);
+  }
+  if (RZ) {
+    double out[1];
+    if (reduce_last<1>(h, l, part, &S->arrive[SLOT_RZ], gidx0 + blockIdx.x, total, out))
+      rz_finish(S, out[0], init);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// small elementwise kernels
+// ---------------------------------------------------------------------------
+__global__ void k_scatter_u(long nnz, const int *__restrict__ l2u, const double *__restrict__ lv,
+                            double *__restrict__ uv, float *__restrict__ lv32,
+                            float *__restrict__ uv32) {
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < nnz;
+       k += (long)gridDim.x * blockDim.x) {
+    const double a = lv[k];
+    const int u = l2u[k];
+    uv[u] = a;
+    if (lv32) {
+      lv32[k] = (float)a;
+      uv32[u] = (float)a;
+    }
+  }
+}
+
+// factor values -> sliced-ELL copies of L, U and the diagonal
+__global__ void k_scatter_sell(long nnz, const double *__restrict__ lv,
+                               const int *__restrict__ l2ls, const int *__restrict__ l2us,
+                               double *__restrict__ ls64, float *__restrict__ ls32,
+                               double *__restrict__ us64, float *__restrict__ us32,
+                               double *__restrict__ d64, float *__restrict__ d32) {
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < nnz;
+       k += (long)gridDim.x * blockDim.x) {
+    const double a = lv[k];
+    const int m = l2ls[k];
+    if (m < 0) {
+      const int row = -1 - m;
+      if (d64) d64[row] = a;
+      if (d32) d32[row] = (float)a;
+    } else {
+      if (ls64) ls64[m] = a;
+      if (ls32) ls32[m] = (float)a;
+    }
+    const int u = l2us[k];
+    if (u >= 0) {
+      if (us64) us64[u] = a;
+      if (us32) us32[u] = (float)a;
+    }
+  }
+}
+
+__global__ void k_cast_f32(long n, const double *__restrict__ a, float *__restrict__ b) {
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
+       k += (long)gridDim.x * blockDim.x)
+    b[k] = (float)a[k];
+}
+__global__ void k_cast_f64(long n, const float *__restrict__ a, double *__restrict__ b) {
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
+       k += (long)gridDim.x * blockDim.x)
+    b[k] = (double)a[k];
+}
+__global__ void k_copy(long n, const double *__restrict__ a, double *__restrict__ b) {
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
+       k += (long)gridDim.x * blockDim.x)
+    b[k] = a[k];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static int wm_of(int maxw) { return maxw <= 4 ? 4 : maxw <= 8 ? 8 : maxw <= 16 ? 16 : 0; }
+
+static int ax_grid(const DevCsr &A) {
+  return A.sell.off ? slice_grid(A.sell.nslice) : red_grid((A.n + kWG - 1) / kWG);
+}
+
+#define DISPATCH_WM(wm, KERNEL, ...)                                  \
+  do {                                                                \
+    switch (wm) {                                                     \
+      case 4: hipLaunchKernelGGL(KERNEL<4>, __VA_ARGS__); break;      \
+      case 8: hipLaunchKernelGGL(KERNEL<8>, __VA_ARGS__); break;      \
+      case 16: hipLaunchKernelGGL(KERNEL<16>, __VA_ARGS__); break;    \
+      default: hipLaunchKernelGGL(KERNEL<0>, __VA_ARGS__); break;     \
+    }                                                                 \
+  } while (0)
+
+void launch_spmv_plain(const DevCsr &A, const double *x, double *y, hipStream_t st) {
+  if (A.n == 0) return;
+  DISPATCH_WM(wm_of(A.sell.maxw), k_spmv_plain, dim3(ax_grid(A)), dim3(kWG), 0, st, A, x, y);
+}
+
+void launch_spmv_cg(const DevCsr &A, const double *p, double *Ap, Scal *S, double *part,
+                    hipStream_t st) {
+  DISPATCH_WM(wm_of(A.sell.maxw), k_spmv_cg, dim3(ax_grid(A)), dim3(kWG), 0, st, A, p, Ap, S,
+              part);
+}
+
+void launch_resid(const DevCsr &A, const double *x, const double *b, double *r, Scal *S,
+                  double *part, int mode, double tol, int max_iter, hipStream_t st) {
+  DISPATCH_WM(wm_of(A.sell.maxw), k_resid, dim3(ax_grid(A)), dim3(kWG), 0, st, A, x, b, r, S,
+              part, mode, tol, max_iter);
+}
+
+void launch_xr(int pk, int n, double *x, const double *p, double *r, const double *Ap,
+               const double *inv_diag, double *z, Scal *S, double *part, hipStream_t st) {
+  dim3 g(quad_grid(n)), b(kWG);
+  if (pk == PK_IDENT)
+    hipLaunchKernelGGL(k_xr<PK_IDENT>, g, b, 0, st, n, x, p, r, Ap, inv_diag, z, S, part);
+  else if (pk == PK_JAC)
+    hipLaunchKernelGGL(k_xr<PK_JAC>, g, b, 0, st, n, x, p, r, Ap, inv_diag, z, S, part);
+  else
+    hipLaunchKernelGGL(k_xr<PK_IC>, g, b, 0, st, n, x, p, r, Ap, inv_diag, z, S, part);
+}
+
+void launch_precond_diag_init(int pk, int n, const double *r, const double *inv_diag, double *z,
+                              double *p, Scal *S, double *part, hipStream_t st) {
+  dim3 g(red_grid((n + kWG - 1) / kWG)), b(kWG);
+  if (pk == PK_JAC)
+    hipLaunchKernelGGL(k_precond_diag_init<PK_JAC>, g, b, 0, st, n, r, inv_diag, z, p, S, part);
+  else
+    hipLaunchKernelGGL(k_precond_diag_init<PK_IDENT>, g, b, 0, st, n, r, inv_diag, z, p, S,
+                       part);
+}
+
+void launch_rz(int n, const double *r, const void *z, bool z32, Scal *S, double *part, int init,
+               hipStream_t st) {
+  dim3 g(quad_grid(n)), b(kWG);
+  if (z32)
+    hipLaunchKernelGGL(k_rz<float>, g, b, 0, st, n, r, (const float *)z, S, part, init);
+  else
+    hipLaunchKernelGGL(k_rz<double>, g, b, 0, st, n, r, (const double *)z, S, part, init);
+}
+
+void launch_update_p(int n, const void *z, bool z32, double *p, const Scal *S, int init,
+                     hipStream_t st) {
+  dim3 g(quad_grid(n)), b(kWG);
+  if (z32)
+    hipLaunchKernelGGL(k_update_p<float>, g, b, 0, st, n, (const float *)z, p, S, init);
+  else
+    hipLaunchKernelGGL(k_update_p<double>, g, b, 0, st, n, (const double *)z, p, S, init);
+}
+
+void launch_jacobi_setup(const DevCsr &A, double *inv_diag, hipStream_t st) {
+  if (A.n == 0) return;
+  hipLaunchKernelGGL(k_jacobi_setup, dim3(stream_grid(A.n)), dim3(kWG), 0, st, A.n, A.rp, A.ci,
+                     A.v, inv_diag);
+}
+
+void launch_jacobi_apply(int n, const double *inv_diag, const double *r, double *z,
+                         hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_jacobi_apply, dim3(stream_grid(n)), dim3(kWG), 0, st, n, inv_diag, r, z);
+}
+
+// CSR-functor plans: natural-order levels, non-contiguous MC, large ABMC blocks
 template <class F>
-static void run_forward_plan(const DevSched &sc, F f, const Scal *S, hipStream_t st,
-                             bool backward) {
+static void run_csr_plan(const DevSched &sc, F f, const Scal *S, hipStream_t st, bool backward) {
   if (sc.kind == 1) {  // MC
     const std::vector<int> &cp = sc.h_color_ptr_v;
     for (int t = 0; t < sc.nc; ++t) {
@@ -536,144 +1020,158 @@ static void run_forward_plan(const DevSched &sc, F f, const Scal *S, hipStream_t
   }
 }
 
-// ---------------------------------------------------------------------------
-// small elementwise kernels
-// ---------------------------------------------------------------------------
-__global__ void k_scatter_u(long nnz, const int *__restrict__ l2u, const double *__restrict__ lv,
-                            double *__restrict__ uv, float *__restrict__ lv32,
-                            float *__restrict__ uv32) {
-  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < nnz;
-       k += (long)gridDim.x * blockDim.x) {
-    const double a = lv[k];
-    const int u = l2u[k];
-    uv[u] = a;
-    if (lv32) {
-      lv32[k] = (float)a;
-      uv32[u] = (float)a;
+// sliced plans are used for MC with contiguous colors and ABMC with
+// contiguous blocks of <= 64 rows, when the factor has sliced copies
+static bool sliced_plan(const DevSched &sc, const Factor &F) {
+  if (!F.sell || std::max(F.ls.maxw, F.us.maxw) > 16) return false;
+  if (sc.kind == 1) return sc.d_rows == nullptr;
+  if (sc.kind == 2) return sc.d_brows == nullptr && sc.max_block <= 64;
+  return false;
+}
+
+bool bwd_fuses_rz(const DevSched &sc, const Factor &F) { return sliced_plan(sc, F); }
+
+static int sweep_wm(const Factor &F) { return wm_of(std::max(F.ls.maxw, F.us.maxw)); }
+
+template <class VT, class YT, int WM>
+static void fwd_sliced(const DevSched &sc, const Factor &F, const VT *lv, const VT *dg,
+                       const double *r, YT *y, const Scal *S, hipStream_t st) {
+  if (sc.kind == 1) {
+    for (int c = 0; c < sc.nc; ++c) {
+      const int c0 = sc.h_color_ptr_v[c], c1 = sc.h_color_ptr_v[c + 1];
+      if (c1 <= c0) continue;
+      const int ns = ((c1 - 1) >> 6) - (c0 >> 6) + 1;
+      hipLaunchKernelGGL((k_mc_fwd<VT, YT, WM>), dim3(slice_grid(ns)), dim3(kWG), 0, st, c0, c1,
+                         F.ls, lv, dg, r, y, S);
+    }
+  } else {
+    for (int c = 0; c < sc.nc; ++c) {
+      const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
+      if (b1 <= b0) continue;
+      hipLaunchKernelGGL((k_abmc_fwd_wave<VT, YT, WM>), dim3(slice_grid(b1 - b0)), dim3(kWG), 0,
+                         st, b0, b1, sc.d_bstart, sc.d_lev_f, sc.d_nlev_f, F.ls, lv, dg, r, y, S);
     }
   }
 }
 
-__global__ void k_cast_f32(long n, const double *__restrict__ a, float *__restrict__ b) {
-  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
-       k += (long)gridDim.x * blockDim.x)
-    b[k] = (float)a[k];
-}
-__global__ void k_cast_f64(long n, const float *__restrict__ a, double *__restrict__ b) {
-  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
-       k += (long)gridDim.x * blockDim.x)
-    b[k] = (double)a[k];
-}
-__global__ void k_copy(long n, const double *__restrict__ a, double *__restrict__ b) {
-  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
-       k += (long)gridDim.x * blockDim.x)
-    b[k] = a[k];
-}
-
-// ---------------------------------------------------------------------------
-// launchers
-// ---------------------------------------------------------------------------
-void launch_spmv_plain(const DevCsr &A, const double *x, double *y, hipStream_t st) {
-  if (A.n == 0) return;
-  hipLaunchKernelGGL(k_spmv_plain, dim3(stream_grid(A.n)), dim3(kWG), 0, st, A.n, A.rp, A.ci,
-                     A.v, x, y);
-}
-
-void launch_spmv_cg(const DevCsr &A, const double *p, double *Ap, Scal *S, double *part,
-                    hipStream_t st) {
-  hipLaunchKernelGGL(k_spmv_cg, dim3(red_grid(A.n)), dim3(kWG), 0, st, A.n, A.rp, A.ci, A.v, p,
-                     Ap, S, part);
-}
-
-void launch_resid(const DevCsr &A, const double *x, const double *b, double *r, Scal *S,
-                  double *part, int mode, double tol, int max_iter, hipStream_t st) {
-  hipLaunchKernelGGL(k_resid, dim3(red_grid(A.n)), dim3(kWG), 0, st, A.n, A.rp, A.ci, A.v, x, b,
-                     r, S, part, mode, tol, max_iter);
-}
-
-void launch_xr(int pk, int n, double *x, const double *p, double *r, const double *Ap,
-               const double *inv_diag, double *z, Scal *S, double *part, hipStream_t st) {
-  dim3 g(red_grid(n)), b(kWG);
-  if (pk == PK_IDENT)
-    hipLaunchKernelGGL(k_xr<PK_IDENT>, g, b, 0, st, n, x, p, r, Ap, inv_diag, z, S, part);
-  else if (pk == PK_JAC)
-    hipLaunchKernelGGL(k_xr<PK_JAC>, g, b, 0, st, n, x, p, r, Ap, inv_diag, z, S, part);
-  else
-    hipLaunchKernelGGL(k_xr<PK_IC>, g, b, 0, st, n, x, p, r, Ap, inv_diag, z, S, part);
-}
-
-void launch_precond_diag_init(int pk, int n, const double *r, const double *inv_diag, double *z,
-                              double *p, Scal *S, double *part, hipStream_t st) {
-  dim3 g(red_grid(n)), b(kWG);
-  if (pk == PK_JAC)
-    hipLaunchKernelGGL(k_precond_diag_init<PK_JAC>, g, b, 0, st, n, r, inv_diag, z, p, S, part);
-  else
-    hipLaunchKernelGGL(k_precond_diag_init<PK_IDENT>, g, b, 0, st, n, r, inv_diag, z, p, S,
-                       part);
+template <class VT, class YT, int WM>
+static void bwd_sliced(const DevSched &sc, const Factor &F, const VT *uv, const VT *dg,
+                       const YT *y, YT *z, const double *r, Scal *S, double *part, int rz,
+                       hipStream_t st) {
+  // workgroup counts of every color launch, for the cross-launch r.z sum
+  std::vector<int> grids;
+  for (int t = 0; t < sc.nc; ++t) {
+    const int c = sc.nc - 1 - t;
+    int g = 0;
+    if (sc.kind == 1) {
+      const int c0 = sc.h_color_ptr_v[c], c1 = sc.h_color_ptr_v[c + 1];
+      if (c1 > c0) g = slice_grid(((c1 - 1) >> 6) - (c0 >> 6) + 1);
+    } else {
+      const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
+      if (b1 > b0) g = slice_grid(b1 - b0);
+    }
+    grids.push_back(g);
+  }
+  int total = 0;
+  for (int g : grids) total += g;
+  const bool fuse = rz != 0 && r != nullptr;
+  int gidx0 = 0;
+  for (int t = 0; t < sc.nc; ++t) {
+    const int c = sc.nc - 1 - t;
+    const int g = grids[t];
+    if (g == 0) continue;
+    if (sc.kind == 1) {
+      const int c0 = sc.h_color_ptr_v[c], c1 = sc.h_color_ptr_v[c + 1];
+      if (fuse)
+        hipLaunchKernelGGL((k_mc_bwd<VT, YT, WM, true>), dim3(g), dim3(kWG), 0, st, c0, c1, F.us,
+                           uv, dg, y, z, r, S, part, gidx0, total, rz == 2);
+      else
+        hipLaunchKernelGGL((k_mc_bwd<VT, YT, WM, false>), dim3(g), dim3(kWG), 0, st, c0, c1,
+                           F.us, uv, dg, y, z, r, S, part, gidx0, total, 0);
+    } else {
+      const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
+      if (fuse)
+        hipLaunchKernelGGL((k_abmc_bwd_wave<VT, YT, WM, true>), dim3(g), dim3(kWG), 0, st, b0,
+                           b1, sc.d_bstart, sc.d_lev_b, sc.d_nlev_b, F.us, uv, dg, y, z, r, S,
+                           part, gidx0, total, rz == 2);
+      else
+        hipLaunchKernelGGL((k_abmc_bwd_wave<VT, YT, WM, false>), dim3(g), dim3(kWG), 0, st, b0,
+                           b1, sc.d_bstart, sc.d_lev_b, sc.d_nlev_b, F.us, uv, dg, y, z, r, S,
+                           part, gidx0, total, 0);
+    }
+    gidx0 += g;
+  }
 }
 
-void launch_rz(int n, const double *r, const void *z, bool z32, Scal *S, double *part, int init,
-               hipStream_t st) {
-  dim3 g(red_grid(n)), b(kWG);
-  if (z32)
-    hipLaunchKernelGGL(k_rz<float>, g, b, 0, st, n, r, (const float *)z, S, part, init);
-  else
-    hipLaunchKernelGGL(k_rz<double>, g, b, 0, st, n, r, (const double *)z, S, part, init);
-}
-
-void launch_update_p(int n, const void *z, bool z32, double *p, const Scal *S, int init,
-                     hipStream_t st) {
-  dim3 g(stream_grid(n)), b(kWG);
-  if (z32)
-    hipLaunchKernelGGL(k_update_p<float>, g, b, 0, st, n, (const float *)z, p, S, init);
-  else
-    hipLaunchKernelGGL(k_update_p<double>, g, b, 0, st, n, (const double *)z, p, S, init);
-}
-
-void launch_jacobi_setup(const DevCsr &A, double *inv_diag, hipStream_t st) {
-  if (A.n == 0) return;
-  hipLaunchKernelGGL(k_jacobi_setup, dim3(stream_grid(A.n)), dim3(kWG), 0, st, A.n, A.rp, A.ci,
-                     A.v, inv_diag);
-}
-
-void launch_jacobi_apply(int n, const double *inv_diag, const double *r, double *z,
-                         hipStream_t st) {
-  if (n == 0) return;
-  hipLaunchKernelGGL(k_jacobi_apply, dim3(stream_grid(n)), dim3(kWG), 0, st, n, inv_diag, r, z);
-}
+#define DISPATCH_SWEEP_WM(wm, FN, VT, YT, ...)   \
+  do {                                           \
+    switch (wm) {                                \
+      case 4: FN<VT, YT, 4>(__VA_ARGS__); break; \
+      case 8: FN<VT, YT, 8>(__VA_ARGS__); break; \
+      default: FN<VT, YT, 16>(__VA_ARGS__); break; \
+    }                                            \
+  } while (0)
 
 void launch_fwd(const DevSched &sc, const Factor &F, bool mixed, const double *r, void *y,
                 const Scal *S, hipStream_t st) {
+  if (sliced_plan(sc, F)) {
+    const int wm = sweep_wm(F);
+    if (mixed)
+      DISPATCH_SWEEP_WM(wm, fwd_sliced, float, float, sc, F, F.ls.v32, F.d32, r, (float *)y, S,
+                        st);
+    else
+      DISPATCH_SWEEP_WM(wm, fwd_sliced, double, double, sc, F, F.ls.v64, F.d64, r, (double *)y,
+                        S, st);
+    return;
+  }
   if (mixed) {
     FwdRow<float, float> f{F.lrp, F.lci, F.lv32, r, (float *)y};
-    run_forward_plan(sc, f, S, st, false);
+    run_csr_plan(sc, f, S, st, false);
   } else {
     FwdRow<double, double> f{F.lrp, F.lci, F.lv, r, (double *)y};
-    run_forward_plan(sc, f, S, st, false);
+    run_csr_plan(sc, f, S, st, false);
   }
 }
 
 void launch_bwd(const DevSched &sc, const Factor &F, bool mixed, const void *y, void *z,
-                const Scal *S, hipStream_t st) {
+                const Scal *S, hipStream_t st, const double *r, Scal *Sw, double *part, int rz) {
+  if (sliced_plan(sc, F)) {
+    const int wm = sweep_wm(F);
+    if (mixed)
+      DISPATCH_SWEEP_WM(wm, bwd_sliced, float, float, sc, F, F.us.v32, F.d32, (const float *)y,
+                        (float *)z, r, Sw, part, rz, st);
+    else
+      DISPATCH_SWEEP_WM(wm, bwd_sliced, double, double, sc, F, F.us.v64, F.d64,
+                        (const double *)y, (double *)z, r, Sw, part, rz, st);
+    return;
+  }
   if (mixed) {
     BwdRow<float, float> f{F.urp, F.uci, F.uv32, (const float *)y, (float *)z};
-    run_forward_plan(sc, f, S, st, true);
+    run_csr_plan(sc, f, S, st, true);
   } else {
     BwdRow<double, double> f{F.urp, F.uci, F.uv, (const double *)y, (double *)z};
-    run_forward_plan(sc, f, S, st, true);
+    run_csr_plan(sc, f, S, st, true);
   }
 }
 
 void launch_ic0_factor(const DevSched &sc, const Factor &F, double shift,
                        unsigned long long *err, hipStream_t st) {
   IC0Row f{F.lrp, F.lci, F.lv, shift, err};
-  run_forward_plan(sc, f, (const Scal *)nullptr, st, false);
+  run_csr_plan(sc, f, (const Scal *)nullptr, st, false);
 }
 
 void launch_scatter_u(const Factor &F, bool mixed, hipStream_t st) {
   if (F.nnz == 0) return;
   hipLaunchKernelGGL(k_scatter_u, dim3(stream_grid(F.nnz)), dim3(kWG), 0, st, F.nnz, F.l2u, F.lv,
                      F.uv, mixed ? F.lv32 : nullptr, mixed ? F.uv32 : nullptr);
+}
+
+void launch_scatter_sell(const Factor &F, bool mixed, hipStream_t st) {
+  if (F.nnz == 0 || !F.sell) return;
+  hipLaunchKernelGGL(k_scatter_sell, dim3(stream_grid(F.nnz)), dim3(kWG), 0, st, F.nnz, F.lv,
+                     F.l2ls, F.l2us, mixed ? nullptr : F.ls.v64, mixed ? F.ls.v32 : nullptr,
+                     mixed ? nullptr : F.us.v64, mixed ? F.us.v32 : nullptr,
+                     mixed ? nullptr : F.d64, mixed ? F.d32 : nullptr);
 }
 
 void launch_cast_f32(const double *a, float *b, long n, hipStream_t st) {

@@ -45,9 +45,11 @@ enum { ICCG_SCHED_MC = 0, ICCG_SCHED_ABMC = 1 };
 
 /* solve flags */
 enum {
-  ICCG_SOLVE_TRUE_RESID = 1 /* stop only when the FP64 true residual
-                               ||b - A x|| / ||b|| < tol (restart CG from the
-                               true residual otherwise).  Default in MIXED.  */
+  ICCG_SOLVE_TRUE_RESID = 1, /* stop only when the FP64 true residual
+                                ||b - A x|| / ||b|| < tol (restart CG from the
+                                true residual otherwise).  Default in MIXED. */
+  ICCG_SOLVE_ZERO_X0 = 2      /* start from x0 = 0 (cg_crs.cpp:439), set on the
+                                device instead of uploading zeros            */
 };
 
 typedef struct iccg_csr_s *iccg_csr_t;

@@ -46,7 +46,14 @@ def lib():
                              _vp, _vp]
         L.ref_ic0_apply.argtypes = [C.c_int, _vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_int,
                                     C.c_int, _vp, _vp, _vp, _vp]
+        L.ref_set_threads.argtypes = [C.c_int]
+        L.ref_set_threads(1)  # sequential reductions unless set_threads() says otherwise
     return _lib
+
+
+def set_threads(n):
+    """OpenMP threads of the reference's loops (1 = sequential reductions)."""
+    lib().ref_set_threads(int(n))
 
 
 def _err():
@@ -184,6 +191,30 @@ def cg(A: Csr, b, kind, *, sched: Schedule | None = None, x0=None, max_iter=1000
                       max_iter, tol, *_sched_args(sched if k == 2 else None),
                       C.byref(rel), C.byref(conv))
     return it, rel.value, bool(conv.value), x
+
+
+def cg_timed_ic0(A: Csr, b, sched: Schedule | None, max_iter, tol=1e-8, shift=0.0,
+                 build_factor=True):
+    """IC(0)-CG of the reference, timing only the CG phase as its main()
+    does.  Returns (iters, rel_resid, converged, x, seconds)."""
+    if build_factor:
+        ic0_factor_only(A, shift)
+    b = f64(b)
+    x = np.zeros(A.n)
+    rel = C.c_double(0.0)
+    conv = C.c_int(0)
+    sec = C.c_double(0.0)
+    f = lib().ref_cg_timed_ic0
+    f.argtypes = [C.c_int] + [_vp] * 5 + [C.c_int, C.c_double, C.c_int, C.c_int, _vp, _vp,
+                                          C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    it = f(A.n, _ptr(A.rowptr), _ptr(A.colind), _ptr(A.val), _ptr(b), _ptr(x), max_iter, tol,
+           *_sched_args(sched), C.byref(rel), C.byref(conv), C.byref(sec))
+    return it, rel.value, bool(conv.value), x, sec.value
+
+
+def ic0_factor_only(A: Csr, shift=0.0):
+    if lib().ref_ic0_create(A.n, _ptr(A.rowptr), _ptr(A.colind), _ptr(A.val), shift) != 0:
+        raise RuntimeError(_err())
 
 
 def spmv(A: Csr, x):

@@ -7,6 +7,8 @@
 // reference code; it includes the reference's own translation unit
 // cg_crs.cpp (with its main renamed) and calls the functions declared there
 // and in crs_io.hpp / color.hpp / block.hpp / ic0.hpp.
+#include <omp.h>
+
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -63,6 +65,10 @@ ColorSchedule make_sched(int nc, const int *color_ptr, const int *rows_of_color,
 extern "C" {
 
 const char *ref_last_error() { return g_err.c_str(); }
+
+// OpenMP threads of the reference's parallel loops.  One thread gives the
+// sequential summation order of its long double reductions (the oracle's).
+void ref_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 
 // ---- CRS holder ------------------------------------------------------------
 void ref_crs_dims(int *n, long *nnz) {
@@ -242,6 +248,27 @@ int ref_cg(int kind, int n, const int *rp, const int *ci, const double *v,
   } else {
     res = conjugate_gradient(A, bv, xv, *g_ic0, max_iter, tol, sp);
   }
+  std::memcpy(x, xv.data(), sizeof(double) * n);
+  *rel_out = res.rel_resid;
+  *conv_out = res.converged ? 1 : 0;
+  return res.iters;
+}
+
+// The reference's CG phase alone, timed like main() (cg_crs.cpp:507-510) but
+// in seconds at steady_clock resolution; uses the factor of ref_ic0_create.
+int ref_cg_timed_ic0(int n, const int *rp, const int *ci, const double *v,
+                     const double *b, double *x, int max_iter, double tol,
+                     int use_sched, int nc, const int *color_ptr,
+                     const int *rows_of_color, int mode, int nb, const int *cpb,
+                     const int *boc, const int *bp, const int *rob,
+                     double *rel_out, int *conv_out, double *seconds) {
+  CRS A = make_crs(n, rp, ci, v);
+  std::vector<double> bv(b, b + n), xv(x, x + n);
+  ColorSchedule s = make_sched(nc, color_ptr, rows_of_color, mode, nb, cpb, boc, bp, rob);
+  auto t0 = std::chrono::steady_clock::now();
+  CGResult res = conjugate_gradient(A, bv, xv, *g_ic0, max_iter, tol, use_sched ? &s : nullptr);
+  auto t1 = std::chrono::steady_clock::now();
+  *seconds = std::chrono::duration<double>(t1 - t0).count();
   std::memcpy(x, xv.data(), sizeof(double) * n);
   *rel_out = res.rel_resid;
   *conv_out = res.converged ? 1 : 0;
