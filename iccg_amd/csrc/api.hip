@@ -180,6 +180,7 @@ struct iccg_solver_s {
   double *x = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr, *b = nullptr;
   void *y = nullptr, *z = nullptr;  // f64 or f32 (mixed IC0)
   double *part = nullptr;
+  int part_slots = 0;  // (hi, lo) pairs the partial buffer holds
   Scal *S = nullptr;
   Scal *hS = nullptr;  // pinned mirror
   bool timing = false;
@@ -659,6 +660,15 @@ static void run_solver(iccg_solver_s &S, int max_iter, double tol, int flags,
   HIPCHK(hipSetDevice(S.device));
   const DevCsr &A = S.A->d;
   const bool true_resid = (flags & ICCG_SOLVE_TRUE_RESID) != 0;
+  // the preconditioner may have been re-planned since the solver was made
+  if (S.M->kind == ICCG_PRECOND_IC0 && bwd_partial_slots(S.M->sc, S.M->F) > S.part_slots) {
+    const int slots = bwd_partial_slots(S.M->sc, S.M->F);
+    double *np = dalloc<double>((size_t)slots * 2);
+    HIPCHK(hipStreamSynchronize(S.st));
+    dfree(S.part);
+    S.part = np;
+    S.part_slots = slots;
+  }
   auto t0 = std::chrono::steady_clock::now();
   if (flags & ICCG_SOLVE_ZERO_X0)
     HIPCHK(hipMemsetAsync(S.x, 0, sizeof(double) * (size_t)S.n, S.st));
@@ -996,7 +1006,11 @@ int iccg_solver_create(iccg_csr_t A, iccg_precond_t M, iccg_solver_t *out) {
     const size_t vb = M->mixed ? sizeof(float) : sizeof(double);
     if (M->kind == ICCG_PRECOND_IC0) S->y = dalloc<char>(vb * (size_t)n);
     if (M->kind != ICCG_PRECOND_IDENTITY) S->z = dalloc<char>(vb * (size_t)n);
-    S->part = dalloc<double>((size_t)kMaxRedBlocks * 8);
+    const int slots = std::max(kMaxRedBlocks * 3, M->kind == ICCG_PRECOND_IC0
+                                                      ? bwd_partial_slots(M->sc, M->F)
+                                                      : 0);
+    S->part = dalloc<double>((size_t)slots * 2);
+    S->part_slots = slots;
     S->S = dalloc<Scal>(1);
     HIPCHK(hipMemsetAsync(S->S, 0, sizeof(Scal), S->st));
     HIPCHK(hipMemsetAsync(S->x, 0, sizeof(double) * (size_t)n, S->st));

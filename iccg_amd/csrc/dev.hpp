@@ -21,8 +21,9 @@ namespace iccg {
 constexpr int kWG = 256;          // threads per workgroup of streaming kernels
 constexpr int kMaxRedBlocks = 2048;  // workgroups of reducing kernels (8 per CU)
 
-// Device-resident solver scalars.  Written only by the last-arriving
-// workgroup of a reducing kernel; read by the others at kernel start.
+// Device-resident solver scalars.  Written only by the one-workgroup finish
+// kernels that fold the reducing kernels' partials; read by the others at
+// kernel start.
 struct alignas(128) Scal {
   double normb;    // ||b|| (1.0 when ||b|| == 0, cg_crs.cpp:108-110)
   double rz_old;   // r.z of the previous iteration
@@ -37,14 +38,11 @@ struct alignas(128) Scal {
   int breakdown;
   int max_iter;
   int pad0;
-  unsigned int arrive[8];  // last-workgroup counters, one per kernel kind
   unsigned long long err;  // IC(0) failure key (row << 32 | row_j), ~0 = none
 };
 
 enum PrecondKind { PK_IDENT = 0, PK_JAC = 1, PK_IC = 2 };
 
-// arrival-counter slots
-enum { SLOT_SPMV = 0, SLOT_XR = 1, SLOT_RESID = 2, SLOT_RZ = 3, SLOT_PINIT = 4, SLOT_TRUE = 5 };
 
 // Sliced ELL with 64-row slices (one wavefront per slice): entry k of row
 // i = 64 s + l sits at off[s] + 64 k + l, so the k-th load of a wavefront is
@@ -151,6 +149,7 @@ void launch_bwd(const DevSched &sc, const Factor &F, bool mixed, const void *y, 
                 double *part = nullptr, int rz = 0);
 // true when launch_bwd fuses r.z for this schedule (sliced MC / ABMC plans)
 bool bwd_fuses_rz(const DevSched &sc, const Factor &F);
+int bwd_partial_slots(const DevSched &sc, const Factor &F);
 void launch_scatter_sell(const Factor &F, bool mixed, hipStream_t st);
 void launch_ic0_factor(const DevSched &sc, const Factor &F, double shift,
                        unsigned long long *err, hipStream_t st);

@@ -60,16 +60,14 @@ __device__ __forceinline__ void wave_dd_reduce(double (&h)[NC], double (&l)[NC])
   }
 }
 
-// Block reduction of NC double-double sums.  Workgroup gidx of `total`
-// workgroups (over one or several launches on one stream) stores its
-// partial; the one that arrives last folds all partials in index order.
-// Returns true on thread 0 of that workgroup, with out[] = rounded totals.
-// blockDim.x must be a multiple of 64, at most 1024.
+// Workgroup partial of NC double-double sums: the workgroup's lanes are
+// folded in a fixed tree and thread 0 stores (hi, lo) pairs at slot gidx.
+// No fence: a separate one-workgroup finish kernel reads the partials after
+// the kernel boundary (an agent-scope fence per workgroup would write the
+// L2 back once per workgroup).  blockDim.x: multiple of 64, at most 1024.
 template <int NC>
-__device__ bool reduce_last(double (&h)[NC], double (&l)[NC], double *part,
-                            unsigned int *arrive, int gidx, int total, double (&out)[NC]) {
+__device__ void block_partial(double (&h)[NC], double (&l)[NC], double *part, int gidx) {
   __shared__ double sh[NC][2][16];
-  __shared__ int s_last;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nw = blockDim.x >> 6;
   wave_dd_reduce<NC>(h, l);
@@ -89,30 +87,29 @@ __device__ bool reduce_last(double (&h)[NC], double (&l)[NC], double *part,
       part[(gidx * NC + c) * 2 + 0] = ah;
       part[(gidx * NC + c) * 2 + 1] = al;
     }
-    __threadfence();
-    unsigned int t = atomicAdd(arrive, 1u);
-    s_last = (t == (unsigned)total - 1u);
   }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
-  double h2[NC], l2[NC];
+}
+
+// One workgroup folds `count` partials in slot order; true on thread 0,
+// with out[] = the totals rounded once to double.
+template <int NC>
+__device__ bool finish_reduce(const double *__restrict__ part, int count, double (&out)[NC]) {
+  __shared__ double sh[NC][2][16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  double h[NC], l[NC];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) h2[c] = l2[c] = 0.0;
-  for (int b = threadIdx.x; b < total; b += blockDim.x) {
+  for (int c = 0; c < NC; ++c) h[c] = l[c] = 0.0;
+  for (int b = threadIdx.x; b < count; b += blockDim.x) {
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const volatile double *q = part + (b * NC + c) * 2;
-      dd_add(h2[c], l2[c], q[0], q[1]);
-    }
+    for (int c = 0; c < NC; ++c) dd_add(h[c], l[c], part[(b * NC + c) * 2], part[(b * NC + c) * 2 + 1]);
   }
-  wave_dd_reduce<NC>(h2, l2);
-  __syncthreads();
+  wave_dd_reduce<NC>(h, l);
   if (lane == 0) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      sh[c][0][w] = h2[c];
-      sh[c][1][w] = l2[c];
+      sh[c][0][w] = h[c];
+      sh[c][1][w] = l[c];
     }
   }
   __syncthreads();
@@ -123,14 +120,7 @@ __device__ bool reduce_last(double (&h)[NC], double (&l)[NC], double *part,
     for (int k = 1; k < nw; ++k) dd_add(ah, al, sh[c][0][k], sh[c][1][k]);
     out[c] = ah + al;
   }
-  *arrive = 0u;
   return true;
-}
-
-template <int NC>
-__device__ __forceinline__ bool reduce_last(double (&h)[NC], double (&l)[NC], double *part,
-                                            unsigned int *arrive, double (&out)[NC]) {
-  return reduce_last<NC>(h, l, part, arrive, blockIdx.x, gridDim.x, out);
 }
 
 // Makes this wavefront's LDS writes visible to its own lanes (rocPRIM's
@@ -273,8 +263,15 @@ __global__ __launch_bounds__(kWG) void k_spmv_cg(DevCsr A, const double *__restr
     dd_fma(h[1], l[1], pi, pi);
     dd_fma(h[2], l[2], y, y);
   });
+  block_partial<3>(h, l, part, blockIdx.x);
+}
+
+// breakdown test (cg_crs.cpp:127-133) and alpha (135)
+__global__ __launch_bounds__(kWG) void k_fin_alpha(const double *__restrict__ part, int count,
+                                                   Scal *S) {
+  if (S->done) return;
   double out[3];
-  if (reduce_last<3>(h, l, part, &S->arrive[SLOT_SPMV], out)) {
+  if (finish_reduce<3>(part, count, out)) {
     const double pAp = out[0];
     const double thr = DBL_EPSILON * sqrt(out[1]) * sqrt(out[2]);
     S->pAp = pAp;
@@ -301,8 +298,13 @@ __global__ __launch_bounds__(kWG) void k_resid(DevCsr A, const double *__restric
     dd_fma(h[0], l[0], bi, bi);
     dd_fma(h[1], l[1], ri, ri);
   });
+  block_partial<2>(h, l, part, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kWG) void k_fin_resid(const double *__restrict__ part, int count,
+                                                   Scal *S, int mode, double tol, int max_iter) {
   double out[2];
-  if (reduce_last<2>(h, l, part, &S->arrive[SLOT_RESID], out)) {
+  if (finish_reduce<2>(part, count, out)) {
     double normb = sqrt(out[0]);
     if (normb == 0.0) normb = 1.0;
     if (mode == 0) {
@@ -391,8 +393,16 @@ __global__ __launch_bounds__(kWG) void k_xr(int n, double *__restrict__ x,
       dd_fma(h[1], l[1], ri, zi);
     }
   }
+  block_partial<2>(h, l, part, blockIdx.x);
+}
+
+// stop test (cg_crs.cpp:143-149); Identity / Jacobi also beta (152-153)
+template <int PK>
+__global__ __launch_bounds__(kWG) void k_fin_xr(const double *__restrict__ part, int count,
+                                                Scal *S) {
+  if (S->done) return;
   double out[2];
-  if (reduce_last<2>(h, l, part, &S->arrive[SLOT_XR], out)) {
+  if (finish_reduce<2>(part, count, out)) {
     const double rel = sqrt(out[0]) / S->normb;
     const int it = S->iters + 1;
     S->iters = it;
@@ -426,16 +436,22 @@ __global__ __launch_bounds__(kWG) void k_precond_diag_init(int n, const double *
     p[i] = zi;
     dd_fma(h[0], l[0], ri, zi);
   }
-  double out[1];
-  if (reduce_last<1>(h, l, part, &S->arrive[SLOT_PINIT], out)) S->rz_old = out[0];
+  block_partial<1>(h, l, part, blockIdx.x);
 }
 
-__device__ __forceinline__ void rz_finish(Scal *S, double rz, int init) {
-  if (init) {
-    S->rz_old = rz;
-  } else {
-    S->beta = rz / S->rz_old;
-    S->rz_old = rz;
+// rz_new = r.z; beta = rz_new / rz_old (cg_crs.cpp:152-153), or rz_old at
+// the start of a solve (115)
+__global__ __launch_bounds__(kWG) void k_fin_rz(const double *__restrict__ part, int count,
+                                                Scal *S, int init) {
+  if (S->done) return;
+  double out[1];
+  if (finish_reduce<1>(part, count, out)) {
+    if (init) {
+      S->rz_old = out[0];
+    } else {
+      S->beta = out[0] / S->rz_old;
+      S->rz_old = out[0];
+    }
   }
 }
 
@@ -456,8 +472,7 @@ __global__ __launch_bounds__(kWG) void k_rz(int n, const double *__restrict__ r,
     for (int e = 0; e < 4; ++e) dd_fma(h[0], l[0], rv[e], zv[e]);
   }
   TAIL_LOOP(n) { dd_fma(h[0], l[0], r[i], (double)z[i]); }
-  double out[1];
-  if (reduce_last<1>(h, l, part, &S->arrive[SLOT_RZ], out)) rz_finish(S, out[0], init);
+  block_partial<1>(h, l, part, blockIdx.x);
 }
 
 // p = z + beta p (cg_crs.cpp:155-157), or p = z at start (114)
@@ -681,11 +696,7 @@ __global__ __launch_bounds__(kWG) void k_mc_bwd(int c0, int c1, Sell U, const VT
     z[i] = zi;
     if (RZ) dd_fma(h[0], l[0], r[i], (double)zi);
   }
-  if (RZ) {
-    double out[1];
-    if (reduce_last<1>(h, l, part, &S->arrive[SLOT_RZ], gidx0 + blockIdx.x, total, out))
-      rz_finish(S, out[0], init);
-  }
+  if (RZ) block_partial<1>(h, l, part, gidx0 + blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -819,11 +830,7 @@ __global__ __launch_bounds__(kWG) void k_abmc_bwd_wave(
     }
     if (RZ && act) dd_fma(h[0], l[0], ri, (double)zi);
   }
-  if (RZ) {
-    double out[1];
-    if (reduce_last<1>(h, l, part, &S->arrive[SLOT_RZ], gidx0 + blockIdx.x, total, out))
-      rz_finish(S, out[0], init);
-  }
+  if (RZ) block_partial<1>(h, l, part, gidx0 + blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -912,25 +919,32 @@ void launch_spmv_plain(const DevCsr &A, const double *x, double *y, hipStream_t 
 
 void launch_spmv_cg(const DevCsr &A, const double *p, double *Ap, Scal *S, double *part,
                     hipStream_t st) {
-  DISPATCH_WM(wm_of(A.sell.maxw), k_spmv_cg, dim3(ax_grid(A)), dim3(kWG), 0, st, A, p, Ap, S,
-              part);
+  const int g = ax_grid(A);
+  DISPATCH_WM(wm_of(A.sell.maxw), k_spmv_cg, dim3(g), dim3(kWG), 0, st, A, p, Ap, S, part);
+  hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kWG), 0, st, part, g, S);
 }
 
 void launch_resid(const DevCsr &A, const double *x, const double *b, double *r, Scal *S,
                   double *part, int mode, double tol, int max_iter, hipStream_t st) {
-  DISPATCH_WM(wm_of(A.sell.maxw), k_resid, dim3(ax_grid(A)), dim3(kWG), 0, st, A, x, b, r, S,
-              part, mode, tol, max_iter);
+  const int g = ax_grid(A);
+  DISPATCH_WM(wm_of(A.sell.maxw), k_resid, dim3(g), dim3(kWG), 0, st, A, x, b, r, S, part, mode,
+              tol, max_iter);
+  hipLaunchKernelGGL(k_fin_resid, dim3(1), dim3(kWG), 0, st, part, g, S, mode, tol, max_iter);
 }
 
 void launch_xr(int pk, int n, double *x, const double *p, double *r, const double *Ap,
                const double *inv_diag, double *z, Scal *S, double *part, hipStream_t st) {
   dim3 g(quad_grid(n)), b(kWG);
-  if (pk == PK_IDENT)
+  if (pk == PK_IDENT) {
     hipLaunchKernelGGL(k_xr<PK_IDENT>, g, b, 0, st, n, x, p, r, Ap, inv_diag, z, S, part);
-  else if (pk == PK_JAC)
+    hipLaunchKernelGGL(k_fin_xr<PK_IDENT>, dim3(1), b, 0, st, part, (int)g.x, S);
+  } else if (pk == PK_JAC) {
     hipLaunchKernelGGL(k_xr<PK_JAC>, g, b, 0, st, n, x, p, r, Ap, inv_diag, z, S, part);
-  else
+    hipLaunchKernelGGL(k_fin_xr<PK_JAC>, dim3(1), b, 0, st, part, (int)g.x, S);
+  } else {
     hipLaunchKernelGGL(k_xr<PK_IC>, g, b, 0, st, n, x, p, r, Ap, inv_diag, z, S, part);
+    hipLaunchKernelGGL(k_fin_xr<PK_IC>, dim3(1), b, 0, st, part, (int)g.x, S);
+  }
 }
 
 void launch_precond_diag_init(int pk, int n, const double *r, const double *inv_diag, double *z,
@@ -941,6 +955,7 @@ void launch_precond_diag_init(int pk, int n, const double *r, const double *inv_
   else
     hipLaunchKernelGGL(k_precond_diag_init<PK_IDENT>, g, b, 0, st, n, r, inv_diag, z, p, S,
                        part);
+  hipLaunchKernelGGL(k_fin_rz, dim3(1), b, 0, st, part, (int)g.x, S, 1);
 }
 
 void launch_rz(int n, const double *r, const void *z, bool z32, Scal *S, double *part, int init,
@@ -950,6 +965,7 @@ void launch_rz(int n, const double *r, const void *z, bool z32, Scal *S, double 
     hipLaunchKernelGGL(k_rz<float>, g, b, 0, st, n, r, (const float *)z, S, part, init);
   else
     hipLaunchKernelGGL(k_rz<double>, g, b, 0, st, n, r, (const double *)z, S, part, init);
+  hipLaunchKernelGGL(k_fin_rz, dim3(1), b, 0, st, part, (int)g.x, S, init);
 }
 
 void launch_update_p(int n, const void *z, bool z32, double *p, const Scal *S, int init,
@@ -1031,6 +1047,23 @@ static bool sliced_plan(const DevSched &sc, const Factor &F) {
 
 bool bwd_fuses_rz(const DevSched &sc, const Factor &F) { return sliced_plan(sc, F); }
 
+// partial slots of the fused backward sweep (one per workgroup of every
+// color launch)
+int bwd_partial_slots(const DevSched &sc, const Factor &F) {
+  if (!sliced_plan(sc, F)) return 0;
+  int total = 0;
+  for (int c = 0; c < sc.nc; ++c) {
+    if (sc.kind == 1) {
+      const int c0 = sc.h_color_ptr_v[c], c1 = sc.h_color_ptr_v[c + 1];
+      if (c1 > c0) total += slice_grid(((c1 - 1) >> 6) - (c0 >> 6) + 1);
+    } else {
+      const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
+      if (b1 > b0) total += slice_grid(b1 - b0);
+    }
+  }
+  return total;
+}
+
 static int sweep_wm(const Factor &F) { return wm_of(std::max(F.ls.maxw, F.us.maxw)); }
 
 template <class VT, class YT, int WM>
@@ -1101,6 +1134,7 @@ static void bwd_sliced(const DevSched &sc, const Factor &F, const VT *uv, const 
     }
     gidx0 += g;
   }
+  if (fuse) hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(kWG), 0, st, part, total, S, rz == 2);
 }
 
 #define DISPATCH_SWEEP_WM(wm, FN, VT, YT, ...)   \
