@@ -73,34 +73,45 @@ def build_problem(N, block):
 
 
 def time_solves(S, steps, warmup, max_iter, tol, true_res):
+    """Timed solves (iterations replayed as hipGraphs), then one more solve
+    launched kernel by kernel with HIP events on the solver's stream around
+    every launch group, for the per-kernel durations."""
+    S.set_timing(False)
     for _ in range(warmup):
         S.run(max_iter, tol, true_residual=true_res, zero_x0=True)
-    S.reset_timing()
     res = []
     t0 = time.perf_counter()
     for _ in range(steps):
         res.append(S.run(max_iter, tol, true_residual=true_res, zero_x0=True))
     wall = time.perf_counter() - t0
-    return res, wall
+    S.reset_timing()
+    S.set_timing(True)
+    prof = S.run(max_iter, tol, true_residual=true_res, zero_x0=True)
+    S.set_timing(False)
+    return res, wall, prof
 
 
-def kernel_table(S, n, nnz, nnz_l, mixed, nc):
+def kernel_table(S, n, nnz, nnz_l, mixed, nc, iters):
+    """Per kernel class, from HIP events recorded on the solver's stream
+    around every launch group during the timed solves."""
     names = ["spmv_dots", "fwd_sweep", "bwd_sweep", "vector_ops"]
     fwd = trsv_bytes(n, nnz_l, 4, 8, 4) if mixed else trsv_bytes(n, nnz_l, 8, 8, 8)
     bwd = trsv_bytes(n, nnz_l, 4, 4, 4) if mixed else trsv_bytes(n, nnz_l, 8, 8, 8)
-    per_launch = [spmv_bytes(n, nnz), fwd, bwd, 80 * n]
+    per_unit = [spmv_bytes(n, nnz), fwd, bwd, 80 * n]
+    units = ["one SpMV launch + its dot-product finish kernel",
+             f"one forward sweep ({nc} color launches)",
+             f"one backward sweep ({nc} color launches + r.z finish kernel)",
+             "one iteration's vector work (x/r update + finish, p update)"]
     out = {}
     for c, nm in enumerate(names):
         cnt, ms = S.timing(c)
         if cnt == 0:
             continue
-        avg = ms / cnt
-        gbs = per_launch[c] / (avg * 1e-3) / 1e9
-        out[nm] = {"launches": cnt, "avg_ms": round(avg, 5), "bytes": int(per_launch[c]),
-                   "gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
-                   "unit": "one launch" if c == 0 else
-                   (f"one sweep ({nc} color launches)" if c in (1, 2) else
-                    "x/r update + r.z + p update (3 launches)")}
+        per = cnt if c < 3 else max(iters, 1)
+        avg = ms / per
+        gbs = per_unit[c] / (avg * 1e-3) / 1e9
+        out[nm] = {"count": per, "avg_ms": round(avg, 5), "bytes": int(per_unit[c]),
+                   "gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "unit": units[c]}
     return out
 
 
@@ -144,7 +155,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=192)
-    ap.add_argument("--block", default="64", help="ABMC block size, or 'box' (4x4x4 boxes)")
+    ap.add_argument("--block", default="box",
+                    help="'box': 4x4x4 grid boxes of 64 rows; or an integer B for breadth-first "
+                         "blocks of B rows (both colored first-fit)")
     ap.add_argument("--max-iter", type=int, default=20000)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
@@ -171,12 +184,12 @@ def main():
     log(f"device setup + IC(0) (mixed): {time.time() - t0:.1f}s")
     S32 = I.Solver(dA, M32)
     S32.set(b)
-    S32.set_timing(True)
-    res32, wall32 = time_solves(S32, args.steps, args.warmup, args.max_iter, args.tol, True)
+    res32, wall32, prof32 = time_solves(S32, args.steps, args.warmup, args.max_iter, args.tol,
+                                        True)
     ms32 = wall32 / args.steps * 1e3
     r32 = res32[-1]
     x32 = S32.x()
-    k32 = kernel_table(S32, n, nnz, nnz_l, True, info["nc"])
+    k32 = kernel_table(S32, n, nnz, nnz_l, True, info["nc"], prof32.iters)
     log(f"mixed: {r32.iters} iters, true rel {r32.true_rel_resid:.3e}, {ms32:.1f} ms/solve, "
         f"restarts {r32.restarts}")
     del S32
@@ -188,12 +201,12 @@ def main():
         log(f"IC(0) (fp64): {time.time() - t0:.1f}s")
         S64 = I.Solver(dA, M64)
         S64.set(b)
-        S64.set_timing(True)
-        res64, wall64 = time_solves(S64, args.steps, args.warmup, args.max_iter, args.tol, False)
+        res64, wall64, prof64 = time_solves(S64, args.steps, args.warmup, args.max_iter,
+                                            args.tol, False)
         ms64 = wall64 / args.steps * 1e3
         r64 = res64[-1]
         x64 = S64.x()
-        k64 = kernel_table(S64, n, nnz, nnz_l, False, info["nc"])
+        k64 = kernel_table(S64, n, nnz, nnz_l, False, info["nc"], prof64.iters)
         err = float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))
         log(f"fp64: {r64.iters} iters, rel {r64.rel_resid:.3e}, true {r64.true_rel_resid:.3e}, "
             f"{ms64:.1f} ms/solve; mixed vs fp64 rel err {err:.2e}")
@@ -203,10 +216,15 @@ def main():
                 "kernels": k64}
         del S64, M64
 
+    # practical bandwidth of a plain device copy (1 GiB read + 1 GiB write)
+    copy_ms = I.bench_copy(1 << 30, 20)
+    copy_gbs = round(2 * (1 << 30) / (copy_ms * 1e-3) / 1e9, 1)
+    log(f"copy kernel: {copy_gbs} GB/s")
+
     bytes_it = iter_bytes(n, nnz, nnz_l, True)
     gbs32 = bytes_it * r32.iters / (ms32 * 1e-3) / 1e9
     # dominant kernel class of the headline solve
-    dom = max(k32.items(), key=lambda kv: kv[1]["avg_ms"] * kv[1]["launches"])
+    dom = max(k32.items(), key=lambda kv: kv[1]["avg_ms"] * kv[1]["count"])
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{dom[0]}.json")
     if os.path.exists(tpath):
@@ -215,6 +233,7 @@ def main():
     roofline = {"bound": "hbm", "kernel": dom[0], "unit_of_launch": dom[1]["unit"],
                 "achieved": dom[1]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": dom[1]["frac"], "traffic": traffic,
+                "copy_kernel_gbs": copy_gbs,
                 "algorithmic_bytes_per_launch": dom[1]["bytes"], "avg_ms": dom[1]["avg_ms"]}
 
     cpu = None

@@ -85,6 +85,7 @@ _SIGS = {
     "iccg_solver_get_x": (_i, [_vp, _vp]),
     "iccg_solver_set_timing": (_i, [_vp, _i]),
     "iccg_solver_get_timing": (_i, [_vp, _i, _lp, _dp]),
+    "iccg_solver_set_graphs": (_i, [_vp, _i]),
     "iccg_solver_reset_timing": (_i, [_vp]),
     "iccg_bench_copy": (_i, [_l, _i, _dp]),
 }

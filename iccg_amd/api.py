@@ -509,6 +509,9 @@ class Solver:
     def set_timing(self, on=True):
         check(lib().iccg_solver_set_timing(self._h.h, int(on)))
 
+    def set_graphs(self, on=True):
+        check(lib().iccg_solver_set_graphs(self._h.h, int(on)))
+
     def reset_timing(self):
         check(lib().iccg_solver_reset_timing(self._h.h))
 

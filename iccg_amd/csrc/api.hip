@@ -133,6 +133,7 @@ struct iccg_precond_s {
   double *inv_diag = nullptr;  // Jacobi
   Factor F;                    // IC0
   DevSched sc;
+  int plan_version = 0;  // bumped whenever the device plan (sc) is replaced
   ~iccg_precond_s() {
     (void)hipSetDevice(device);
     dfree(inv_diag);
@@ -184,12 +185,22 @@ struct iccg_solver_s {
   Scal *S = nullptr;
   Scal *hS = nullptr;  // pinned mirror
   bool timing = false;
+  // hipGraphs of k consecutive iterations, replayed while the plan is the
+  // one they were captured from
+  bool graphs = true;
+  int graph_version = -1;
+  std::vector<std::pair<int, hipGraphExec_t>> execs;
   std::vector<TimedPair> pending;
   std::vector<hipEvent_t> pool;
   long t_launch[4] = {0, 0, 0, 0};
   double t_ms[4] = {0, 0, 0, 0};
+  void drop_graphs() {
+    for (auto &e : execs) (void)hipGraphExecDestroy(e.second);
+    execs.clear();
+  }
   ~iccg_solver_s() {
     (void)hipSetDevice(device);
+    drop_graphs();
     dfree(x);
     dfree(r);
     dfree(p);
@@ -655,6 +666,33 @@ static void sync_scal(iccg_solver_s &S) {
   HIPCHK(hipStreamSynchronize(S.st));
 }
 
+// Enqueues k iterations: replays a captured hipGraph of k iterations, or
+// launches them one by one while event timing is on.
+static void enqueue_chunk(iccg_solver_s &S, int first, int k) {
+  if (S.timing || !S.graphs) {
+    for (int j = 0; j < k; ++j) enqueue_iteration(S, first + j);
+    return;
+  }
+  if (S.graph_version != S.M->plan_version) {
+    S.drop_graphs();
+    S.graph_version = S.M->plan_version;
+  }
+  hipGraphExec_t exec = nullptr;
+  for (auto &e : S.execs)
+    if (e.first == k) exec = e.second;
+  if (!exec) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(S.st, hipStreamCaptureModeThreadLocal));
+    for (int j = 0; j < k; ++j) enqueue_iteration(S, -1);
+    HIPCHK(hipStreamEndCapture(S.st, &g));
+    hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(e);
+    S.execs.push_back({k, exec});
+  }
+  HIPCHK(hipGraphLaunch(exec, S.st));
+}
+
 static void run_solver(iccg_solver_s &S, int max_iter, double tol, int flags,
                        iccg_cg_result &res) {
   HIPCHK(hipSetDevice(S.device));
@@ -668,6 +706,7 @@ static void run_solver(iccg_solver_s &S, int max_iter, double tol, int flags,
     dfree(S.part);
     S.part = np;
     S.part_slots = slots;
+    S.drop_graphs();
   }
   auto t0 = std::chrono::steady_clock::now();
   if (flags & ICCG_SOLVE_ZERO_X0)
@@ -678,7 +717,9 @@ static void run_solver(iccg_solver_s &S, int max_iter, double tol, int flags,
   int issued = 0, restarts = 0;
   int chunk = S.n < 100000 ? 8 : 16;
   for (;;) {
-    for (int j = 0; j < chunk && issued < max_iter; ++j) enqueue_iteration(S, issued++);
+    const int k = std::min(chunk, max_iter - issued);
+    enqueue_chunk(S, issued, k);
+    issued += k;
     HIPCHK(hipGetLastError());
     sync_scal(S);
     if (S.timing) harvest(S, S.hS->iters);
@@ -918,6 +959,7 @@ int iccg_precond_set_schedule(iccg_precond_t M, iccg_sched_t sched) {
     tmp.st = M->st;
     build_schedule(tmp, sched, lrp, lci, urp, uci);
     std::swap(M->sc, tmp.sc);
+    M->plan_version++;
   });
 }
 
@@ -1062,6 +1104,13 @@ int iccg_solver_set_timing(iccg_solver_t S, int on) {
   return guard([&] {
     require(S, ICCG_ERR_ARG, "null argument");
     S->timing = on != 0;
+  });
+}
+
+int iccg_solver_set_graphs(iccg_solver_t S, int on) {
+  return guard([&] {
+    require(S, ICCG_ERR_ARG, "null argument");
+    S->graphs = on != 0;
   });
 }
 

@@ -131,6 +131,31 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// XCD-aware distribution of [lo, hi) over the wavefronts of the grid.
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b -> XCD b % 8)
+// and each XCD has its own L2, so each XCD gets one contiguous eighth of the
+// items: neighbouring slices / blocks, whose x-vector gathers overlap, then
+// share an L2 instead of fetching the same lines on several XCDs.
+constexpr int kXcd = 8;
+struct WaveRange {
+  int first, step, end;
+};
+__device__ __forceinline__ WaveRange xcd_waves(int lo, int hi) {
+  const int g = gridDim.x;
+  const int nch = g < kXcd ? g : kXcd;  // chunks: every chunk has a workgroup
+  const int x = blockIdx.x % nch, k = blockIdx.x / nch;
+  const int wgs = (g - x + nch - 1) / nch;  // workgroups on this chunk
+  const int wpg = blockDim.x >> 6;
+  const int n = hi - lo;
+  const int chunk = (n + nch - 1) / nch;
+  const int a = lo + min(n, x * chunk), b = lo + min(n, (x + 1) * chunk);
+  WaveRange r;
+  r.first = a + k * wpg + (threadIdx.x >> 6);
+  r.step = wgs * wpg;
+  r.end = b;
+  return r;
+}
+
 static inline int red_grid(long items_per_thread_block_unit) {
   long g = items_per_thread_block_unit;
   if (g < 1) g = 1;
@@ -228,9 +253,8 @@ template <int WM, class F>
 __device__ __forceinline__ void for_rows_Ax(const DevCsr &A, const double *__restrict__ x, F f) {
   if (A.sell.off) {
     const int lane = threadIdx.x & 63;
-    const int wv = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int nwv = gridDim.x * (blockDim.x >> 6);
-    for (int s = wv; s < A.sell.nslice; s += nwv) {
+    const WaveRange wr = xcd_waves(0, A.sell.nslice);
+    for (int s = wr.first; s < wr.end; s += wr.step) {
       const double y = sell_chain<double, double, double, WM, false>(A.sell.off, A.sell.col,
                                                                     A.sell.v64, x, s, lane, 0.0);
       const int i = (s << 6) + lane;
@@ -665,9 +689,8 @@ __global__ __launch_bounds__(kWG) void k_mc_fwd(int c0, int c1, Sell L, const VT
                                                 YT *__restrict__ y, const Scal *S) {
   if (S && S->done) return;
   const int lane = threadIdx.x & 63;
-  const int wv = blockIdx.x * (kWG >> 6) + (threadIdx.x >> 6), nwv = gridDim.x * (kWG >> 6);
-  const int s0 = c0 >> 6, s1 = (c1 - 1) >> 6;
-  for (int s = s0 + wv; s <= s1; s += nwv) {
+  const WaveRange wr = xcd_waves(c0 >> 6, ((c1 - 1) >> 6) + 1);
+  for (int s = wr.first; s < wr.end; s += wr.step) {
     const int i = (s << 6) + lane;
     if (i < c0 || i >= c1) continue;
     YT acc = (YT)r[i];
@@ -684,10 +707,9 @@ __global__ __launch_bounds__(kWG) void k_mc_bwd(int c0, int c1, Sell U, const VT
                                                 double *part, int gidx0, int total, int init) {
   if (S && S->done) return;
   const int lane = threadIdx.x & 63;
-  const int wv = blockIdx.x * (kWG >> 6) + (threadIdx.x >> 6), nwv = gridDim.x * (kWG >> 6);
-  const int s0 = c0 >> 6, s1 = (c1 - 1) >> 6;
+  const WaveRange wr = xcd_waves(c0 >> 6, ((c1 - 1) >> 6) + 1);
   double h[1] = {0}, l[1] = {0};
-  for (int s = s0 + wv; s <= s1; s += nwv) {
+  for (int s = wr.first; s < wr.end; s += wr.step) {
     const int i = (s << 6) + lane;
     if (i < c0 || i >= c1) continue;
     YT acc = y[i];
@@ -724,7 +746,8 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_wave(int kb0, int kb1,
   if (S && S->done) return;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   YT *my = ys + (w << 6);
-  for (int kb = kb0 + blockIdx.x * (kWG >> 6) + w; kb < kb1; kb += gridDim.x * (kWG >> 6)) {
+  const WaveRange wr = xcd_waves(kb0, kb1);
+  for (int kb = wr.first; kb < wr.end; kb += wr.step) {
     const int bs = bstart[kb], be = bstart[kb + 1];
     const int i = bs + lane;
     const bool act = i < be;
@@ -782,7 +805,8 @@ __global__ __launch_bounds__(kWG) void k_abmc_bwd_wave(
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   YT *my = zs + (w << 6);
   double h[1] = {0}, l[1] = {0};
-  for (int kb = kb0 + blockIdx.x * (kWG >> 6) + w; kb < kb1; kb += gridDim.x * (kWG >> 6)) {
+  const WaveRange wr = xcd_waves(kb0, kb1);
+  for (int kb = wr.first; kb < wr.end; kb += wr.step) {
     const int bs = bstart[kb], be = bstart[kb + 1];
     const int i = bs + lane;
     const bool act = i < be;

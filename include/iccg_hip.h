@@ -238,8 +238,11 @@ int iccg_solver_set(iccg_solver_t S, const double *b, const double *x0);
 int iccg_solver_run(iccg_solver_t S, int max_iter, double tol, int flags,
                     iccg_cg_result *res);
 int iccg_solver_get_x(iccg_solver_t S, double *x);
-/* Per-kernel-class event timing inside iccg_solver_run (0 = off). */
+/* Per-kernel-class event timing inside iccg_solver_run (0 = off).  While
+ * timing is on, iterations are launched one kernel at a time. */
 int iccg_solver_set_timing(iccg_solver_t S, int on);
+/* Replay captured hipGraphs of whole iterations (default on). */
+int iccg_solver_set_graphs(iccg_solver_t S, int on);
 /* Kernel classes: 0 SpMV(+dots), 1 forward sweep, 2 backward sweep,
  * 3 vector updates.  Returns launches and summed milliseconds since the last
  * reset. */

@@ -242,3 +242,34 @@ def test_invalid_schedule_rejected():
     with pytest.raises(I.IccgError, match="not a valid MC schedule"):
         I.IC0(o.A, 0.0, sched=s)
     del bad
+
+
+def test_graph_replay_matches_eager():
+    """Iterations replayed from captured hipGraphs give the eager launches'
+    result bit for bit, on every replay, with eager work in between, and after
+    the preconditioner is re-planned for another schedule."""
+    A = P.grid(24, 20, 16, coef=(-2.0, 2.0), shift=1e-6)
+    o = P.abmc_order(A, grid_dims=(24, 20, 16))
+    b = I.gen_normal(A.n, 12)
+    M = I.IC0(o.A, 0.0, sched=o.sched)
+    S = I.Solver(o.A, M)
+    S.set(b)
+    S.set_graphs(False)
+    ref = S.run(10000, 1e-8, zero_x0=True)
+    x_ref = S.x()
+    S.set_graphs(True)
+    for _ in range(3):
+        r = S.run(10000, 1e-8, zero_x0=True)
+        assert (r.iters, r.rel_resid) == (ref.iters, ref.rel_resid)
+        assert np.array_equal(S.x(), x_ref)
+        S.set_graphs(False)
+        assert S.run(10000, 1e-8, zero_x0=True).iters == ref.iters  # eager in between
+        S.set_graphs(True)
+    # re-plan M for the natural order: same factor, same bits (valid schedules)
+    M.use_schedule(None)
+    r = S.run(10000, 1e-8, zero_x0=True)
+    assert (r.iters, r.rel_resid) == (ref.iters, ref.rel_resid)
+    assert np.array_equal(S.x(), x_ref)
+    # max_iter not a multiple of the chunk size
+    r = S.run(7, 1e-8, zero_x0=True)
+    assert r.iters == 7 and not r.converged
