@@ -18,6 +18,7 @@
 // Compiled with -ffp-contract=off: only the fma() calls written here fuse.
 #include <algorithm>
 #include <cfloat>
+#include <cstdlib>
 
 #include "dev.hpp"
 
@@ -137,6 +138,7 @@ __device__ __forceinline__ void wave_sync() {
 // items: neighbouring slices / blocks, whose x-vector gathers overlap, then
 // share an L2 instead of fetching the same lines on several XCDs.
 constexpr int kXcd = 8;
+constexpr int kFinWG = 1024;  // threads of the one-workgroup finish kernels
 struct WaveRange {
   int first, step, end;
 };
@@ -291,7 +293,7 @@ __global__ __launch_bounds__(kWG) void k_spmv_cg(DevCsr A, const double *__restr
 }
 
 // breakdown test (cg_crs.cpp:127-133) and alpha (135)
-__global__ __launch_bounds__(kWG) void k_fin_alpha(const double *__restrict__ part, int count,
+__global__ __launch_bounds__(kFinWG) void k_fin_alpha(const double *__restrict__ part, int count,
                                                    Scal *S) {
   if (S->done) return;
   double out[3];
@@ -325,7 +327,7 @@ __global__ __launch_bounds__(kWG) void k_resid(DevCsr A, const double *__restric
   block_partial<2>(h, l, part, blockIdx.x);
 }
 
-__global__ __launch_bounds__(kWG) void k_fin_resid(const double *__restrict__ part, int count,
+__global__ __launch_bounds__(kFinWG) void k_fin_resid(const double *__restrict__ part, int count,
                                                    Scal *S, int mode, double tol, int max_iter) {
   double out[2];
   if (finish_reduce<2>(part, count, out)) {
@@ -422,7 +424,7 @@ __global__ __launch_bounds__(kWG) void k_xr(int n, double *__restrict__ x,
 
 // stop test (cg_crs.cpp:143-149); Identity / Jacobi also beta (152-153)
 template <int PK>
-__global__ __launch_bounds__(kWG) void k_fin_xr(const double *__restrict__ part, int count,
+__global__ __launch_bounds__(kFinWG) void k_fin_xr(const double *__restrict__ part, int count,
                                                 Scal *S) {
   if (S->done) return;
   double out[2];
@@ -465,7 +467,7 @@ __global__ __launch_bounds__(kWG) void k_precond_diag_init(int n, const double *
 
 // rz_new = r.z; beta = rz_new / rz_old (cg_crs.cpp:152-153), or rz_old at
 // the start of a solve (115)
-__global__ __launch_bounds__(kWG) void k_fin_rz(const double *__restrict__ part, int count,
+__global__ __launch_bounds__(kFinWG) void k_fin_rz(const double *__restrict__ part, int count,
                                                 Scal *S, int init) {
   if (S->done) return;
   double out[1];
@@ -780,17 +782,18 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_wave(int kb0, int kb1,
         if (cc[k] >= 0 && cc[k] < bs) acc = fma_t(-(YT)vv[k], xo[k], acc);
     }
     const int nl = nlev[kb];
+    YT yi = (YT)0;
     for (int l = 0; l < nl; ++l) {
       if (mylev == l) {
 #pragma unroll
         for (int k = 0; k < WM; ++k)
           if (cc[k] >= bs) acc = fma_t(-(YT)vv[k], my[cc[k] - bs], acc);
-        const YT yi = acc / d;
+        yi = acc / d;
         my[lane] = yi;
-        y[i] = yi;
       }
       wave_sync();
     }
+    if (act) y[i] = yi;  // one coalesced store per block
   }
 }
 
@@ -848,10 +851,10 @@ __global__ __launch_bounds__(kWG) void k_abmc_bwd_wave(
           if (cc[k] >= be) acc = fma_t(-(YT)vv[k], xo[k], acc);
         zi = acc / d;
         my[lane] = zi;
-        z[i] = zi;
       }
       wave_sync();
     }
+    if (act) z[i] = zi;  // one coalesced store per block
     if (RZ && act) dd_fma(h[0], l[0], ri, (double)zi);
   }
   if (RZ) block_partial<1>(h, l, part, gidx0 + blockIdx.x);
@@ -945,7 +948,7 @@ void launch_spmv_cg(const DevCsr &A, const double *p, double *Ap, Scal *S, doubl
                     hipStream_t st) {
   const int g = ax_grid(A);
   DISPATCH_WM(wm_of(A.sell.maxw), k_spmv_cg, dim3(g), dim3(kWG), 0, st, A, p, Ap, S, part);
-  hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kWG), 0, st, part, g, S);
+  hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinWG), 0, st, part, g, S);
 }
 
 void launch_resid(const DevCsr &A, const double *x, const double *b, double *r, Scal *S,
@@ -953,7 +956,7 @@ void launch_resid(const DevCsr &A, const double *x, const double *b, double *r, 
   const int g = ax_grid(A);
   DISPATCH_WM(wm_of(A.sell.maxw), k_resid, dim3(g), dim3(kWG), 0, st, A, x, b, r, S, part, mode,
               tol, max_iter);
-  hipLaunchKernelGGL(k_fin_resid, dim3(1), dim3(kWG), 0, st, part, g, S, mode, tol, max_iter);
+  hipLaunchKernelGGL(k_fin_resid, dim3(1), dim3(kFinWG), 0, st, part, g, S, mode, tol, max_iter);
 }
 
 void launch_xr(int pk, int n, double *x, const double *p, double *r, const double *Ap,
@@ -961,13 +964,13 @@ void launch_xr(int pk, int n, double *x, const double *p, double *r, const doubl
   dim3 g(quad_grid(n)), b(kWG);
   if (pk == PK_IDENT) {
     hipLaunchKernelGGL(k_xr<PK_IDENT>, g, b, 0, st, n, x, p, r, Ap, inv_diag, z, S, part);
-    hipLaunchKernelGGL(k_fin_xr<PK_IDENT>, dim3(1), b, 0, st, part, (int)g.x, S);
+    hipLaunchKernelGGL(k_fin_xr<PK_IDENT>, dim3(1), dim3(kFinWG), 0, st, part, (int)g.x, S);
   } else if (pk == PK_JAC) {
     hipLaunchKernelGGL(k_xr<PK_JAC>, g, b, 0, st, n, x, p, r, Ap, inv_diag, z, S, part);
-    hipLaunchKernelGGL(k_fin_xr<PK_JAC>, dim3(1), b, 0, st, part, (int)g.x, S);
+    hipLaunchKernelGGL(k_fin_xr<PK_JAC>, dim3(1), dim3(kFinWG), 0, st, part, (int)g.x, S);
   } else {
     hipLaunchKernelGGL(k_xr<PK_IC>, g, b, 0, st, n, x, p, r, Ap, inv_diag, z, S, part);
-    hipLaunchKernelGGL(k_fin_xr<PK_IC>, dim3(1), b, 0, st, part, (int)g.x, S);
+    hipLaunchKernelGGL(k_fin_xr<PK_IC>, dim3(1), dim3(kFinWG), 0, st, part, (int)g.x, S);
   }
 }
 
@@ -979,7 +982,7 @@ void launch_precond_diag_init(int pk, int n, const double *r, const double *inv_
   else
     hipLaunchKernelGGL(k_precond_diag_init<PK_IDENT>, g, b, 0, st, n, r, inv_diag, z, p, S,
                        part);
-  hipLaunchKernelGGL(k_fin_rz, dim3(1), b, 0, st, part, (int)g.x, S, 1);
+  hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(kFinWG), 0, st, part, (int)g.x, S, 1);
 }
 
 void launch_rz(int n, const double *r, const void *z, bool z32, Scal *S, double *part, int init,
@@ -989,7 +992,7 @@ void launch_rz(int n, const double *r, const void *z, bool z32, Scal *S, double 
     hipLaunchKernelGGL(k_rz<float>, g, b, 0, st, n, r, (const float *)z, S, part, init);
   else
     hipLaunchKernelGGL(k_rz<double>, g, b, 0, st, n, r, (const double *)z, S, part, init);
-  hipLaunchKernelGGL(k_fin_rz, dim3(1), b, 0, st, part, (int)g.x, S, init);
+  hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(kFinWG), 0, st, part, (int)g.x, S, init);
 }
 
 void launch_update_p(int n, const void *z, bool z32, double *p, const Scal *S, int init,
@@ -1069,7 +1072,21 @@ static bool sliced_plan(const DevSched &sc, const Factor &F) {
   return false;
 }
 
-bool bwd_fuses_rz(const DevSched &sc, const Factor &F) { return sliced_plan(sc, F); }
+// r.z comes from a separate streaming kernel after the backward sweep
+// (measured faster on gfx950 than forming it inside the sweep: the fused
+// sweep lost ~23 us per color launch against 15 us for the extra pass);
+// ICCG_FUSE_RZ=1 selects the fused sweep, which gives the same bits.
+static bool fuse_rz_enabled() {
+  static const bool on = [] {
+    const char *e = std::getenv("ICCG_FUSE_RZ");
+    return e && e[0] == '1';
+  }();
+  return on;
+}
+
+bool bwd_fuses_rz(const DevSched &sc, const Factor &F) {
+  return sliced_plan(sc, F) && fuse_rz_enabled();
+}
 
 // partial slots of the fused backward sweep (one per workgroup of every
 // color launch)
@@ -1131,7 +1148,7 @@ static void bwd_sliced(const DevSched &sc, const Factor &F, const VT *uv, const 
   }
   int total = 0;
   for (int g : grids) total += g;
-  const bool fuse = rz != 0 && r != nullptr;
+  const bool fuse = rz != 0 && r != nullptr && fuse_rz_enabled();
   int gidx0 = 0;
   for (int t = 0; t < sc.nc; ++t) {
     const int c = sc.nc - 1 - t;
@@ -1158,7 +1175,7 @@ static void bwd_sliced(const DevSched &sc, const Factor &F, const VT *uv, const 
     }
     gidx0 += g;
   }
-  if (fuse) hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(kWG), 0, st, part, total, S, rz == 2);
+  if (fuse) hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(kFinWG), 0, st, part, total, S, rz == 2);
 }
 
 #define DISPATCH_SWEEP_WM(wm, FN, VT, YT, ...)   \
