@@ -98,10 +98,11 @@ def kernel_table(S, n, nnz, nnz_l, mixed, nc, iters):
     fwd = trsv_bytes(n, nnz_l, 4, 8, 4) if mixed else trsv_bytes(n, nnz_l, 8, 8, 8)
     bwd = trsv_bytes(n, nnz_l, 4, 4, 4) if mixed else trsv_bytes(n, nnz_l, 8, 8, 8)
     per_unit = [spmv_bytes(n, nnz), fwd, bwd, 80 * n]
-    units = ["one SpMV launch + its dot-product finish kernel",
+    units = ["one SpMV launch (k_spmv_cg)",
              f"one forward sweep ({nc} color launches)",
-             f"one backward sweep ({nc} color launches + r.z finish kernel)",
-             "one iteration's vector work (x/r update + finish, p update)"]
+             f"one backward sweep ({nc} color launches)",
+             "one iteration's vector and scalar work (alpha finish, x/r update + finish, "
+             "r.z + finish, p update)"]
     out = {}
     for c, nm in enumerate(names):
         cnt, ms = S.timing(c)

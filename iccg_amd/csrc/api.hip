@@ -635,11 +635,12 @@ static void enqueue_iteration(iccg_solver_s &S, int it) {
   const DevCsr &A = S.A->d;
   {
     Timer t(S, 0, it);
-    launch_spmv_cg(A, S.p, S.Ap, S.S, S.part, S.st);
+    launch_spmv_cg(A, S.p, S.Ap, S.S, S.part, S.st, false);
   }
   if (M.kind == ICCG_PRECOND_IC0) {
     {
       Timer t(S, 3, it);
+      launch_fin_alpha(A, S.S, S.part, S.st);
       launch_xr(PK_IC, n, S.x, S.p, S.r, S.Ap, nullptr, nullptr, S.S, S.part, S.st);
     }
     {
@@ -656,6 +657,7 @@ static void enqueue_iteration(iccg_solver_s &S, int it) {
   } else {
     const int pk = M.kind == ICCG_PRECOND_JACOBI ? PK_JAC : PK_IDENT;
     Timer t(S, 3, it);
+    launch_fin_alpha(A, S.S, S.part, S.st);
     launch_xr(pk, n, S.x, S.p, S.r, S.Ap, M.inv_diag, (double *)S.z, S.S, S.part, S.st);
     launch_update_p(n, pk == PK_JAC ? S.z : (const void *)S.r, false, S.p, S.S, 0, S.st);
   }

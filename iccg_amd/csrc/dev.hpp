@@ -118,8 +118,11 @@ struct Factor {
 
 // ---- launchers (kernels.hip) ----------------------------------------------
 void launch_spmv_plain(const DevCsr &A, const double *x, double *y, hipStream_t st);
+// Ap = A p with the partials of p.Ap, ||p||^2, ||Ap||^2; finish = also
+// launch the alpha / breakdown kernel (launch_fin_alpha)
 void launch_spmv_cg(const DevCsr &A, const double *p, double *Ap, Scal *S, double *part,
-                    hipStream_t st);
+                    hipStream_t st, bool finish = true);
+void launch_fin_alpha(const DevCsr &A, Scal *S, double *part, hipStream_t st);
 // r = b - A x with ||b|| and ||r||.
 //   mode 1: start of a solve -- resets S (iters 0), stores tol/max_iter,
 //           sets normb, rel and done (rel < tol or max_iter <= 0)

@@ -945,10 +945,14 @@ void launch_spmv_plain(const DevCsr &A, const double *x, double *y, hipStream_t 
 }
 
 void launch_spmv_cg(const DevCsr &A, const double *p, double *Ap, Scal *S, double *part,
-                    hipStream_t st) {
+                    hipStream_t st, bool finish) {
   const int g = ax_grid(A);
   DISPATCH_WM(wm_of(A.sell.maxw), k_spmv_cg, dim3(g), dim3(kWG), 0, st, A, p, Ap, S, part);
-  hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinWG), 0, st, part, g, S);
+  if (finish) launch_fin_alpha(A, S, part, st);
+}
+
+void launch_fin_alpha(const DevCsr &A, Scal *S, double *part, hipStream_t st) {
+  hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinWG), 0, st, part, ax_grid(A), S);
 }
 
 void launch_resid(const DevCsr &A, const double *x, const double *b, double *r, Scal *S,
