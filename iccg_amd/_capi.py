@@ -88,6 +88,14 @@ _SIGS = {
     "iccg_solver_set_graphs": (_i, [_vp, _i]),
     "iccg_solver_reset_timing": (_i, [_vp]),
     "iccg_bench_copy": (_i, [_l, _i, _dp]),
+    "iccg_batch_create": (_i, [_i, C.POINTER(_vp)]),
+    "iccg_batch_destroy": (_i, [_vp]),
+    "iccg_batch_add": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _d, _i, _ip]),
+    "iccg_batch_run": (_i, [_vp, _i, _ip]),
+    "iccg_batch_result": (_i, [_vp, _i, _ip, C.POINTER(CGResultC)]),
+    "iccg_batch_get_x": (_i, [_vp, _i, _vp]),
+    "iccg_batch_remove": (_i, [_vp, _i]),
+    "iccg_batch_set_graphs": (_i, [_vp, _i]),
 }
 
 _lib = None

@@ -532,3 +532,53 @@ def bench_copy(nbytes, reps):
     ms = C.c_double(0)
     check(lib().iccg_bench_copy(nbytes, reps, C.byref(ms)))
     return ms.value
+
+
+class Batch:
+    """Many independent systems advanced together (iccg_batch_*): every CG
+    phase is one launch over all running systems; systems may be added while
+    others run.  Each system needs an IC0 preconditioner built with a
+    contiguous multicolor schedule (e.g. from greedy_color)."""
+
+    def __init__(self, capacity):
+        h = _vp()
+        check(lib().iccg_batch_create(int(capacity), C.byref(h)))
+        self._h = _Handle(h, lib().iccg_batch_destroy)
+        self._keep = {}
+
+    def add(self, A, M, b, x0=None, max_iter=10000, tol=1e-8, true_residual=None):
+        dA = _device_csr(A)
+        if true_residual is None:
+            true_residual = M.precision == _capi.MIXED
+        b = _f64(b)
+        x0 = None if x0 is None else _f64(x0)
+        slot = C.c_int(-1)
+        check(lib().iccg_batch_add(self._h.h, dA.handle, M.handle, _p(b), _p(x0), max_iter, tol,
+                                   _capi.SOLVE_TRUE_RESID if true_residual else 0,
+                                   C.byref(slot)))
+        self._keep[slot.value] = (dA, M)
+        return slot.value
+
+    def run(self, max_steps=1 << 30):
+        n = C.c_int(0)
+        check(lib().iccg_batch_run(self._h.h, int(max_steps), C.byref(n)))
+        return n.value
+
+    def result(self, slot):
+        fin = C.c_int(0)
+        res = _capi.CGResultC()
+        check(lib().iccg_batch_result(self._h.h, slot, C.byref(fin), C.byref(res)))
+        return bool(fin.value), _result(res)
+
+    def x(self, slot):
+        n = self._keep[slot][0].n
+        out = np.zeros(n)
+        check(lib().iccg_batch_get_x(self._h.h, slot, _p(out)))
+        return out
+
+    def remove(self, slot):
+        check(lib().iccg_batch_remove(self._h.h, slot))
+        self._keep.pop(slot, None)
+
+    def set_graphs(self, on=True):
+        check(lib().iccg_batch_set_graphs(self._h.h, int(on)))

@@ -13,159 +13,19 @@
 
 #include "dev.hpp"
 #include "internal.hpp"
+#include "objects.hpp"
 
 namespace iccg {
 
 static thread_local std::string g_last_error;
 void set_last_error(const std::string &msg) { g_last_error = msg; }
 
-#define HIPCHK(expr)                                                                   \
-  do {                                                                                 \
-    hipError_t _e = (expr);                                                            \
-    if (_e != hipSuccess)                                                              \
-      throw Error(ICCG_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
-  } while (0)
-
 static thread_local int g_device = 0;
-
-template <class T>
-static T *dalloc(size_t count) {
-  if (count == 0) count = 1;
-  void *p = nullptr;
-  hipError_t e = hipMalloc(&p, count * sizeof(T));
-  if (e != hipSuccess)
-    throw Error(ICCG_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
-                                    " bytes failed: " + hipGetErrorString(e));
-  return (T *)p;
-}
-
-template <class T>
-static T *dupload(const T *h, size_t count, hipStream_t st) {
-  T *d = dalloc<T>(count);
-  if (count) HIPCHK(hipMemcpyAsync(d, h, count * sizeof(T), hipMemcpyHostToDevice, st));
-  return d;
-}
-
-static void dfree(void *p) {
-  if (p) (void)hipFree(p);
-}
-
-static void free_sell(Sell &S) {
-  dfree(S.off);
-  dfree(S.col);
-  dfree(S.v64);
-  dfree(S.v32);
-  S = Sell();
-}
-
-// Host layout of a 64-row sliced ELL over row ranges [beg(i), end(i)) of a
-// CSR-like index array ci: off[nslice+1], col (padding -1), and for every
-// included CSR position k the ELL position (pos[k]).
-template <class Range>
-static void sell_layout(int n, Range range, const int *ci, std::vector<int> &off,
-                        std::vector<int> &col, int &maxw, std::vector<int> *pos) {
-  const int ns = (n + 63) / 64;
-  off.assign((size_t)ns + 1, 0);
-  maxw = 0;
-  long total = 0;
-  for (int s = 0; s < ns; ++s) {
-    int w = 0;
-    for (int i = s * 64; i < std::min(n, s * 64 + 64); ++i) {
-      int b, e;
-      range(i, b, e);
-      w = std::max(w, e - b);
-    }
-    maxw = std::max(maxw, w);
-    total += 64L * w;
-    require(total < 0x7fffffffL, ICCG_ERR_ARG, "matrix too large for the sliced layout");
-    off[s + 1] = (int)total;
-  }
-  col.assign((size_t)total, -1);
-#pragma omp parallel for schedule(static)
-  for (int i = 0; i < n; ++i) {
-    int b, e;
-    range(i, b, e);
-    const int base = off[i >> 6] + (i & 63);
-    for (int k = b; k < e; ++k) {
-      const int q = base + 64 * (k - b);
-      col[q] = ci[k];
-      if (pos) (*pos)[k] = q;
-    }
-  }
-}
+int current_device() { return g_device; }
 
 }  // namespace iccg
 
 using namespace iccg;
-
-// ---------------------------------------------------------------------------
-// objects
-// ---------------------------------------------------------------------------
-struct iccg_csr_s {
-  int device = 0;
-  HostCsr h;  // host copy of the pattern and values (setup of preconditioners)
-  DevCsr d;
-  hipStream_t st = nullptr;
-  ~iccg_csr_s() {
-    (void)hipSetDevice(device);
-    dfree(d.rp);
-    dfree(d.ci);
-    dfree(d.v);
-    free_sell(d.sell);
-    if (st) (void)hipStreamDestroy(st);
-  }
-};
-
-struct iccg_sched_s {
-  int n = 0, mode = ICCG_SCHED_MC, nc = 0, nb = 0;
-  std::vector<int> color_ptr, rows_of_color;                              // MC
-  std::vector<int> color_ptr_blk, blocks_of_color, block_ptr, rows_of_block;  // ABMC
-};
-
-struct iccg_precond_s {
-  int device = 0;
-  int kind = ICCG_PRECOND_IDENTITY;
-  bool mixed = false;
-  double shift = 0.0;
-  int n = 0;
-  iccg_csr_t A = nullptr;
-  hipStream_t st = nullptr;
-  double *inv_diag = nullptr;  // Jacobi
-  Factor F;                    // IC0
-  DevSched sc;
-  int plan_version = 0;  // bumped whenever the device plan (sc) is replaced
-  ~iccg_precond_s() {
-    (void)hipSetDevice(device);
-    dfree(inv_diag);
-    dfree(F.lrp);
-    dfree(F.lci);
-    dfree(F.lv);
-    dfree(F.lv32);
-    dfree(F.urp);
-    dfree(F.uci);
-    dfree(F.uv);
-    dfree(F.uv32);
-    dfree(F.l2u);
-    free_sell(F.ls);
-    free_sell(F.us);
-    dfree(F.d64);
-    dfree(F.d32);
-    dfree(F.l2ls);
-    dfree(F.l2us);
-    dfree(sc.d_rows);
-    dfree(sc.d_bstart);
-    dfree(sc.d_brows);
-    dfree(sc.d_lev_f);
-    dfree(sc.d_lev_b);
-    dfree(sc.d_nlev_f);
-    dfree(sc.d_nlev_b);
-    dfree(sc.d_lvl_ptr_f);
-    dfree(sc.d_lvl_rows_f);
-    dfree(sc.d_lvl_ptr_b);
-    dfree(sc.d_lvl_rows_b);
-    if (st) (void)hipStreamDestroy(st);
-  }
-};
 
 struct TimedPair {
   int cls, iter;
@@ -829,6 +689,14 @@ int iccg_csr_create(int n, long nnz, const int *rowptr, const int *colind, const
         S.off = dupload(off.data(), off.size(), A->st);
         S.col = dupload(col.data(), col.size(), A->st);
         S.v64 = dupload(v.data(), v.size(), A->st);
+        // values that are all exact floats also get an f32 copy (half the
+        // value bytes for the multi-system SpMV, same products bit for bit)
+        bool exact = true;
+        for (long k = 0; k < nnz && exact; ++k) exact = (double)(float)val[k] == val[k];
+        if (exact) {
+          std::vector<float> v32(v.begin(), v.end());
+          A->d.sell_v32 = dupload(v32.data(), v32.size(), A->st);
+        }
         HIPCHK(hipStreamSynchronize(A->st));
       }
     }

@@ -65,6 +65,7 @@ struct DevCsr {
   int *rp = nullptr, *ci = nullptr;
   double *v = nullptr;
   Sell sell;             // used by the SpMV kernels when sell.off != nullptr
+  float *sell_v32 = nullptr;  // f32 copy of sell values when all are exact floats
 };
 
 // Triangular-solve / factorization schedule on the device.
@@ -114,6 +115,37 @@ struct Factor {
   int *l2ls = nullptr;   // L CSR position -> ls entry, or -1 - row (diagonal)
   int *l2us = nullptr;   // L CSR position -> us entry, or -1 (diagonal)
   bool sell = false;
+};
+
+// ---- multi-system engine (batch.hip) --------------------------------------
+// One entry per system slot: the system's own HBM arrays.  Kernels take
+// lists of work items that name a slot, so one launch per CG phase covers
+// every active system whatever its size, and the set can change between
+// launches (continuous batching).
+struct SysDesc {
+  int n = 0, nslice = 0;
+  int mixed = 0;                     // factor, y, z in f32
+  const int *a_off = nullptr, *a_col = nullptr;  // A, sliced ELL
+  const double *a_v64 = nullptr;
+  const float *a_v32 = nullptr;      // set when every value of A is an exact float
+  const int *l_off = nullptr, *l_col = nullptr, *u_off = nullptr, *u_col = nullptr;
+  const void *l_v = nullptr, *u_v = nullptr, *d = nullptr;
+  double *x = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr;
+  const double *b = nullptr;
+  void *y = nullptr, *z = nullptr;
+};
+
+// per-system solver scalars (the multi-system Scal)
+struct alignas(64) SysScal {
+  double normb, rz_old, alpha, beta, rel, pAp, rr_true, tol;
+  int iters, done, converged, breakdown, max_iter, pad[3];
+};
+
+struct ChunkItem {  // rows [64*s0, min(64*s1, n)) of slot sys: one workgroup
+  int sys, s0, s1, pad;
+};
+struct ColorItem {  // rows [lo, hi) of slot sys inside 64-row slice `slice`
+  int sys, slice, lo, hi;
 };
 
 // ---- launchers (kernels.hip) ----------------------------------------------

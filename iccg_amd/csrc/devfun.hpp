@@ -1,0 +1,228 @@
+// devfun.hpp -- device helpers shared by the solver kernels (kernels.hip)
+// and the multi-system kernels (batch.hip): double-double dot products,
+// workgroup partials and their one-workgroup fold, XCD-aware work ranges,
+// vector loads, and the sliced-ELL row chain.
+#pragma once
+
+#include <algorithm>
+#include <cfloat>
+
+#include "dev.hpp"
+
+namespace iccg {
+
+// ---------------------------------------------------------------------------
+// double-double helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void dd_add(double &h, double &l, double oh, double ol) {
+  double s = h + oh;
+  double bb = s - h;
+  double e = (h - (s - bb)) + (oh - bb);
+  e += l + ol;
+  double hi = s + e;
+  l = e - (hi - s);
+  h = hi;
+}
+
+// (h,l) += a*b
+__device__ __forceinline__ void dd_fma(double &h, double &l, double a, double b) {
+  double p = a * b;
+  double pe = fma(a, b, -p);
+  double s = h + p;
+  double bb = s - h;
+  double e = (h - (s - bb)) + (p - bb);
+  h = s;
+  l += e + pe;
+}
+
+template <int NC>
+__device__ __forceinline__ void wave_dd_reduce(double (&h)[NC], double (&l)[NC]) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      double oh = __shfl_xor(h[c], off, 64);
+      double ol = __shfl_xor(l[c], off, 64);
+      dd_add(h[c], l[c], oh, ol);
+    }
+  }
+}
+
+// Workgroup partial of NC double-double sums: the workgroup's lanes are
+// folded in a fixed tree and thread 0 stores (hi, lo) pairs at slot gidx.
+// No fence: a separate one-workgroup finish kernel reads the partials after
+// the kernel boundary (an agent-scope fence per workgroup would write the
+// L2 back once per workgroup).  blockDim.x: multiple of 64, at most 1024.
+template <int NC>
+__device__ void block_partial(double (&h)[NC], double (&l)[NC], double *part, int gidx) {
+  __shared__ double sh[NC][2][16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  wave_dd_reduce<NC>(h, l);
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      sh[c][0][w] = h[c];
+      sh[c][1][w] = l[c];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      double ah = sh[c][0][0], al = sh[c][1][0];
+      for (int k = 1; k < nw; ++k) dd_add(ah, al, sh[c][0][k], sh[c][1][k]);
+      part[(gidx * NC + c) * 2 + 0] = ah;
+      part[(gidx * NC + c) * 2 + 1] = al;
+    }
+  }
+}
+
+// One workgroup folds `count` partials in slot order; true on thread 0,
+// with out[] = the totals rounded once to double.
+template <int NC>
+__device__ bool finish_reduce(const double *__restrict__ part, int count, double (&out)[NC]) {
+  __shared__ double sh[NC][2][16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  double h[NC], l[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) h[c] = l[c] = 0.0;
+  for (int b = threadIdx.x; b < count; b += blockDim.x) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) dd_add(h[c], l[c], part[(b * NC + c) * 2], part[(b * NC + c) * 2 + 1]);
+  }
+  wave_dd_reduce<NC>(h, l);
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      sh[c][0][w] = h[c];
+      sh[c][1][w] = l[c];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return false;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    double ah = sh[c][0][0], al = sh[c][1][0];
+    for (int k = 1; k < nw; ++k) dd_add(ah, al, sh[c][0][k], sh[c][1][k]);
+    out[c] = ah + al;
+  }
+  return true;
+}
+
+// Makes this wavefront's LDS writes visible to its own lanes (rocPRIM's
+// wave_barrier: no s_barrier, only ordering).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// XCD-aware distribution of [lo, hi) over the wavefronts of the grid.
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b -> XCD b % 8)
+// and each XCD has its own L2, so each XCD gets one contiguous eighth of the
+// items: neighbouring slices / blocks, whose x-vector gathers overlap, then
+// share an L2 instead of fetching the same lines on several XCDs.
+constexpr int kXcd = 8;
+constexpr int kFinWG = 1024;  // threads of the one-workgroup finish kernels
+struct WaveRange {
+  int first, step, end;
+};
+__device__ __forceinline__ WaveRange xcd_waves(int lo, int hi) {
+  const int g = gridDim.x;
+  const int nch = g < kXcd ? g : kXcd;  // chunks: every chunk has a workgroup
+  const int x = blockIdx.x % nch, k = blockIdx.x / nch;
+  const int wgs = (g - x + nch - 1) / nch;  // workgroups on this chunk
+  const int wpg = blockDim.x >> 6;
+  const int n = hi - lo;
+  const int chunk = (n + nch - 1) / nch;
+  const int a = lo + min(n, x * chunk), b = lo + min(n, (x + 1) * chunk);
+  WaveRange r;
+  r.first = a + k * wpg + (threadIdx.x >> 6);
+  r.step = wgs * wpg;
+  r.end = b;
+  return r;
+}
+
+static inline int red_grid(long items_per_thread_block_unit) {
+  long g = items_per_thread_block_unit;
+  if (g < 1) g = 1;
+  if (g > kMaxRedBlocks) g = kMaxRedBlocks;
+  return (int)g;
+}
+
+static inline int stream_grid(long n) {
+  long g = (n + kWG - 1) / kWG;
+  if (g < 1) g = 1;
+  if (g > 65536) g = 65536;
+  return (int)g;
+}
+
+// grid of a streaming kernel that takes 4 elements per lane per step
+static inline int quad_grid(long n) { return red_grid((n + 4L * kWG - 1) / (4L * kWG)); }
+// grid of a kernel that takes one 64-row slice per wavefront per step
+static inline int slice_grid(long nslices) { return red_grid((nslices + 3) / 4); }
+
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return fma(a, b, c); }
+
+// 4-element vector loads/stores (i multiple of 4; HBM buffers 256-B aligned)
+__device__ __forceinline__ void ld4(const double *p, int i, double (&o)[4]) {
+  const double2 u = *reinterpret_cast<const double2 *>(p + i);
+  const double2 w = *reinterpret_cast<const double2 *>(p + i + 2);
+  o[0] = u.x; o[1] = u.y; o[2] = w.x; o[3] = w.y;
+}
+__device__ __forceinline__ void ld4(const float *p, int i, double (&o)[4]) {
+  const float4 u = *reinterpret_cast<const float4 *>(p + i);
+  o[0] = u.x; o[1] = u.y; o[2] = u.z; o[3] = u.w;
+}
+__device__ __forceinline__ void st4(double *p, int i, const double (&o)[4]) {
+  *reinterpret_cast<double2 *>(p + i) = make_double2(o[0], o[1]);
+  *reinterpret_cast<double2 *>(p + i + 2) = make_double2(o[2], o[3]);
+}
+
+// ---------------------------------------------------------------------------
+// Sliced-ELL row chains
+// ---------------------------------------------------------------------------
+// acc op= sum_k v_k * x[c_k] over the entries of the lane's row in slice s,
+// ascending k; NEG selects acc - v*x (triangular solves) over acc + v*x.
+template <class VT, class AT, class XT, int WM, bool NEG>
+__device__ __forceinline__ AT sell_chain(const int *__restrict__ off, const int *__restrict__ col,
+                                         const VT *__restrict__ v, const XT *__restrict__ x,
+                                         int s, int lane, AT acc) {
+  const int o0 = off[s];
+  const int w = (off[s + 1] - o0) >> 6;
+  const int base = o0 + lane;
+  if (WM > 0) {
+    int cc[WM > 0 ? WM : 1];
+    VT vv[WM > 0 ? WM : 1];
+#pragma unroll
+    for (int k = 0; k < WM; ++k) {
+      cc[k] = -1;
+      vv[k] = (VT)0;
+      if (k < w) {
+        cc[k] = col[base + 64 * k];
+        vv[k] = v[base + 64 * k];
+      }
+    }
+    AT xv[WM > 0 ? WM : 1];
+#pragma unroll
+    for (int k = 0; k < WM; ++k) xv[k] = cc[k] >= 0 ? (AT)x[cc[k]] : (AT)0;
+#pragma unroll
+    for (int k = 0; k < WM; ++k)
+      if (cc[k] >= 0) acc = NEG ? fma_t(-(AT)vv[k], xv[k], acc) : fma_t((AT)vv[k], xv[k], acc);
+  } else {
+    for (int k = 0; k < w; ++k) {
+      const int c = col[base + 64 * k];
+      if (c >= 0) {
+        const AT a = (AT)v[base + 64 * k];
+        acc = NEG ? fma_t(-a, (AT)x[c], acc) : fma_t(a, (AT)x[c], acc);
+      }
+    }
+  }
+  return acc;
+}
+
+
+}  // namespace iccg

@@ -1,0 +1,165 @@
+// objects.hpp -- the C ABI's object types and device-memory helpers,
+// shared by api.hip (single-system solver) and batch.hip (multi-system).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "dev.hpp"
+#include "internal.hpp"
+
+namespace iccg {
+
+// HIP device selected by iccg_device_set on this thread
+int current_device();
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess)                                                              \
+      throw Error(ICCG_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+template <class T>
+inline T *dalloc(size_t count) {
+  if (count == 0) count = 1;
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, count * sizeof(T));
+  if (e != hipSuccess)
+    throw Error(ICCG_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+                                    " bytes failed: " + hipGetErrorString(e));
+  return (T *)p;
+}
+
+template <class T>
+inline T *dupload(const T *h, size_t count, hipStream_t st) {
+  T *d = dalloc<T>(count);
+  if (count) HIPCHK(hipMemcpyAsync(d, h, count * sizeof(T), hipMemcpyHostToDevice, st));
+  return d;
+}
+
+inline void dfree(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+inline void free_sell(Sell &S) {
+  dfree(S.off);
+  dfree(S.col);
+  dfree(S.v64);
+  dfree(S.v32);
+  S = Sell();
+}
+
+// Host layout of a 64-row sliced ELL over row ranges [beg(i), end(i)) of a
+// CSR-like index array ci: off[nslice+1], col (padding -1), and for every
+// included CSR position k the ELL position (pos[k]).
+template <class Range>
+inline void sell_layout(int n, Range range, const int *ci, std::vector<int> &off,
+                        std::vector<int> &col, int &maxw, std::vector<int> *pos) {
+  const int ns = (n + 63) / 64;
+  off.assign((size_t)ns + 1, 0);
+  maxw = 0;
+  long total = 0;
+  for (int s = 0; s < ns; ++s) {
+    int w = 0;
+    for (int i = s * 64; i < std::min(n, s * 64 + 64); ++i) {
+      int b, e;
+      range(i, b, e);
+      w = std::max(w, e - b);
+    }
+    maxw = std::max(maxw, w);
+    total += 64L * w;
+    require(total < 0x7fffffffL, ICCG_ERR_ARG, "matrix too large for the sliced layout");
+    off[s + 1] = (int)total;
+  }
+  col.assign((size_t)total, -1);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    int b, e;
+    range(i, b, e);
+    const int base = off[i >> 6] + (i & 63);
+    for (int k = b; k < e; ++k) {
+      const int q = base + 64 * (k - b);
+      col[q] = ci[k];
+      if (pos) (*pos)[k] = q;
+    }
+  }
+}
+
+}  // namespace iccg
+
+using namespace iccg;
+
+// ---------------------------------------------------------------------------
+// objects
+// ---------------------------------------------------------------------------
+struct iccg_csr_s {
+  int device = 0;
+  HostCsr h;  // host copy of the pattern and values (setup of preconditioners)
+  DevCsr d;
+  hipStream_t st = nullptr;
+  ~iccg_csr_s() {
+    (void)hipSetDevice(device);
+    dfree(d.rp);
+    dfree(d.ci);
+    dfree(d.v);
+    free_sell(d.sell);
+    dfree(d.sell_v32);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+struct iccg_sched_s {
+  int n = 0, mode = ICCG_SCHED_MC, nc = 0, nb = 0;
+  std::vector<int> color_ptr, rows_of_color;                              // MC
+  std::vector<int> color_ptr_blk, blocks_of_color, block_ptr, rows_of_block;  // ABMC
+};
+
+struct iccg_precond_s {
+  int device = 0;
+  int kind = ICCG_PRECOND_IDENTITY;
+  bool mixed = false;
+  double shift = 0.0;
+  int n = 0;
+  iccg_csr_t A = nullptr;
+  hipStream_t st = nullptr;
+  double *inv_diag = nullptr;  // Jacobi
+  Factor F;                    // IC0
+  DevSched sc;
+  int plan_version = 0;  // bumped whenever the device plan (sc) is replaced
+  ~iccg_precond_s() {
+    (void)hipSetDevice(device);
+    dfree(inv_diag);
+    dfree(F.lrp);
+    dfree(F.lci);
+    dfree(F.lv);
+    dfree(F.lv32);
+    dfree(F.urp);
+    dfree(F.uci);
+    dfree(F.uv);
+    dfree(F.uv32);
+    dfree(F.l2u);
+    free_sell(F.ls);
+    free_sell(F.us);
+    dfree(F.d64);
+    dfree(F.d32);
+    dfree(F.l2ls);
+    dfree(F.l2us);
+    dfree(sc.d_rows);
+    dfree(sc.d_bstart);
+    dfree(sc.d_brows);
+    dfree(sc.d_lev_f);
+    dfree(sc.d_lev_b);
+    dfree(sc.d_nlev_f);
+    dfree(sc.d_nlev_b);
+    dfree(sc.d_lvl_ptr_f);
+    dfree(sc.d_lvl_rows_f);
+    dfree(sc.d_lvl_ptr_b);
+    dfree(sc.d_lvl_rows_b);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+

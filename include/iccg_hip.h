@@ -250,6 +250,31 @@ int iccg_solver_get_timing(iccg_solver_t S, int cls, long *launches,
                            double *total_ms);
 int iccg_solver_reset_timing(iccg_solver_t S);
 
+/* ---- many systems at once (no reference counterpart) -----------------------
+ * Independent systems, each with its own matrix, IC0 preconditioner (a
+ * contiguous multicolor schedule, e.g. from iccg_greedy_color) and vectors
+ * resident in HBM, advanced together: every CG phase is one launch over all
+ * running systems, each with its own alpha/beta/stop test.  Systems can be
+ * added while others run (continuous batching); a system's iterates equal
+ * those of iccg_cg_solve on it alone.  All systems of one batch share the
+ * preconditioner precision.  The matrix and preconditioner must outlive the
+ * slot. */
+typedef struct iccg_batch_s *iccg_batch_t;
+int iccg_batch_create(int capacity, iccg_batch_t *out);
+int iccg_batch_destroy(iccg_batch_t B);
+/* Admits a system into a free slot (copies b and x0 to HBM; NULL x0 = 0).
+ * flags: ICCG_SOLVE_TRUE_RESID. */
+int iccg_batch_add(iccg_batch_t B, iccg_csr_t A, iccg_precond_t M, const double *b,
+                   const double *x0, int max_iter, double tol, int flags, int *slot);
+/* Starts admitted systems, then runs up to max_steps iterations of every
+ * running system; *n_running = systems not yet finished. */
+int iccg_batch_run(iccg_batch_t B, int max_steps, int *n_running);
+int iccg_batch_result(iccg_batch_t B, int slot, int *finished, iccg_cg_result *res);
+int iccg_batch_get_x(iccg_batch_t B, int slot, double *x);
+/* Frees a slot (finished or not). */
+int iccg_batch_remove(iccg_batch_t B, int slot);
+int iccg_batch_set_graphs(iccg_batch_t B, int on);
+
 /* ---- measurement helpers -------------------------------------------------- */
 /* Device copy of nbytes (HBM -> HBM) repeated reps times; average ms per
  * launch through *avg_ms.  Calibrates the hardware byte counters. */
