@@ -150,8 +150,174 @@ def cpu_baseline(Ap, sched, b, iters_to_tol, budget_s):
             "ms_per_iter": round(per_iter * 1e3, 3)}
 
 
+def mc_system(A, seed_b):
+    """Greedy (first-fit) multicolor ordering of A, the permuted system and
+    its schedule, and b ~ N(0,1) in the permuted numbering, as picg_crs
+    solves it (cg_crs.cpp:413-447)."""
+    import iccg_amd as I
+    color, nc = I.greedy_color(A)
+    p = I.BuildPermutationByColor(color)
+    Ap = I.expand_lower_to_full(I.Permute_PtAP_LowerCRS_to_LowerCRS(A, p))
+    cp, rc = I.BuildColorPtrRows_FromPermutationAndColor(color, nc, p)
+    sched = I.ColorSchedule.mc(nc, cp, rc)
+    b_old = I.gen_normal(A.n, seed_b)
+    b = np.empty_like(b_old)
+    b[p] = b_old
+    return Ap, sched, b
+
+
+def batch_problem(count):
+    """`count` 2D 5-point variable-coefficient systems: grid side uniform in
+    [32, 256], face coefficients 10^U[-1,1] stored as FP32 values, b ~ N(0,1),
+    greedy (red-black) multicolor ordering."""
+    import iccg_amd as I
+    rng = np.random.default_rng(SEED)
+    sides = rng.integers(32, 257, size=count)
+    out = []
+    for s, N in enumerate(sides):
+        A = I.gen_grid(int(N), int(N), 1, coef_exp=(-1.0, 1.0), seed=SEED + s, fp32_values=True)
+        out.append(mc_system(A, SEED + 1_000_000 + s))
+    return sides, out
+
+
+def cpu_baseline_systems(systems, tol, budget_s, label):
+    """The reference's picg_crs IC(0)-CG (oracle/_ref, OpenMP) on the first
+    systems of the workload, one after another, until the budget is spent:
+    solves per second of CG time (factorization untimed, as main() times
+    it, cg_crs.cpp:507-510)."""
+    from oracle import ref as R
+    from oracle.oracle import Csr, Schedule
+    if not R.available():
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    R.set_threads(threads)
+    done, cg_s, t0 = 0, 0.0, time.time()
+    for Ap, sched, b in systems:
+        A = Csr(Ap.n, Ap.rowptr, Ap.colind, Ap.val)
+        s = Schedule(nc=sched.nc, color_ptr=sched.color_ptr, rows_of_color=sched.rows_of_color,
+                     mode=0)
+        R.ic0_factor_only(A, 0.0)
+        it, rel, conv, _, sec = R.cg_timed_ic0(A, b, s, 10000, tol=tol, build_factor=False)
+        cg_s += sec
+        done += 1
+        if time.time() - t0 > budget_s:
+            break
+    log(f"reference picg_crs: {done} solves, {cg_s:.2f}s of CG on {threads} threads")
+    return {"value": round(done / cg_s, 3), "unit": "solves/s", "cores": threads,
+            "kind": "reference",
+            "sample": (f"the first {done} systems of the {label}, solved one after another by "
+                       f"the reference's picg_crs IC(0)-CG (oracle/_ref, OpenMP {threads} "
+                       f"threads), FP64, tol {tol:g}; CG phase timed")}
+
+
+def main_batch(args):
+    """Many-system workload: one Batch solving every system (mixed precision,
+    stop on the FP64 true residual) against the plain FP64 single-system
+    solver run on the systems one at a time."""
+    import iccg_amd as I
+    count, tol, max_iter = args.systems, args.tol, args.max_iter
+    t0 = time.time()
+    sides, systems = batch_problem(count)
+    n_tot = sum(A.n for A, _, _ in systems)
+    nnz_tot = sum(A.nnz for A, _, _ in systems)
+    log(f"{count} systems, sum n = {n_tot}, sum nnz = {nnz_tot} ({time.time() - t0:.1f}s)")
+    t0 = time.time()
+    dA = [I.DeviceCSR(A) for A, _, _ in systems]
+    M32 = [I.IC0(d, 0.0, sched=s, precision="mixed") for d, (_, s, _) in zip(dA, systems)]
+    log(f"device setup + IC(0) mixed: {time.time() - t0:.1f}s")
+
+    B = I.Batch(count)
+
+    def batch_step():
+        slots = [B.add(dA[k], M32[k], systems[k][2], max_iter=max_iter, tol=tol)
+                 for k in range(count)]
+        while B.run():
+            pass
+        res = [B.result(s)[1] for s in slots]
+        return slots, res
+
+    for _ in range(args.warmup):
+        slots, _ = batch_step()
+        for s in slots:
+            B.remove(s)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        slots, res = batch_step()
+        if k < args.steps - 1:
+            for s in slots:
+                B.remove(s)
+    wall = time.perf_counter() - t0
+    xs = [B.x(s) for s in slots]
+    chunk, cw = B.tuning
+    sps = count * args.steps / wall
+    iters = np.array([r.iters for r in res])
+    trel = np.array([r.true_rel_resid for r in res])
+    log(f"batched mixed: {sps:.1f} solves/s ({wall / args.steps * 1e3:.1f} ms per {count}); "
+        f"iters {iters.min()}..{iters.max()} (mean {iters.mean():.1f}); max true rel "
+        f"{trel.max():.2e}; all converged {all(r.converged for r in res)}; "
+        f"tuning chunk={chunk} slices, sweep={cw} slices/wave")
+    del B
+
+    # one at a time: the plain FP64 single-system solver, then mixed
+    def one_at_a_time(precision):
+        Ms = [I.IC0(d, 0.0, sched=s, precision=precision) for d, (_, s, _) in zip(dA, systems)]
+        Ss = []
+        for d, M, (_, _, b) in zip(dA, Ms, systems):
+            S = I.Solver(d, M)
+            S.set(b)
+            Ss.append(S)
+        tr = precision == "mixed"
+        for S in Ss:  # warm: graphs captured
+            S.run(max_iter, tol, true_residual=tr, zero_x0=True)
+        t0 = time.perf_counter()
+        out = [S.run(max_iter, tol, true_residual=tr, zero_x0=True) for S in Ss]
+        w = time.perf_counter() - t0
+        xs1 = [S.x() for S in Ss]
+        del Ss, Ms
+        return count / w, out, xs1
+
+    sps64, res64, xs64 = one_at_a_time("fp64")
+    tr64 = np.array([r.true_rel_resid for r in res64])
+    log(f"one-at-a-time FP64: {sps64:.1f} solves/s; max true rel {tr64.max():.2e}")
+    sps32_1, res32_1, _ = one_at_a_time("mixed")
+    log(f"one-at-a-time mixed: {sps32_1:.1f} solves/s")
+    err = max(float(np.linalg.norm(a - c) / np.linalg.norm(c)) for a, c in zip(xs, xs64))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_systems(systems, tol, args.cpu_budget,
+                                   f"{count}-system workload")
+    line = {
+        "metric": METRIC, "value": round(sps, 2), "unit": "solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 CG + f32 values and factors",
+        "data": "synthetic",
+        "config": {"workload": (f"batched ICCG: {count} independent 2D 5-pt variable-coefficient "
+                                f"systems, grid side U[32,256] (sum n={n_tot}, sum nnz={nnz_tot}), "
+                                f"coefficients 10^U[-1,1] stored as FP32, b~N(0,1), greedy "
+                                f"multicolor, FP32 CRS values and factors + FP64 CG and dots, "
+                                f"tol {tol:g} on the FP64 true residual; a step solves all "
+                                f"{count}"),
+                   "systems": count, "sum_n": n_tot, "sum_nnz": nnz_tot, "tol": tol},
+        "one_at_a_time_fp64_solves_per_s": round(sps64, 2),
+        "speedup_vs_one_at_a_time_fp64": round(sps / sps64, 3),
+        "one_at_a_time_mixed_solves_per_s": round(sps32_1, 2),
+        "iters_min_mean_max": [int(iters.min()), round(float(iters.mean()), 1), int(iters.max())],
+        "max_true_rel_resid": float(trel.max()),
+        "fp64_max_true_rel_resid": float(tr64.max()),
+        "max_err_vs_fp64": err,
+        "tuning": {"chunk_slices": chunk, "sweep_slices_per_wave": cw},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["abmc192", "batch"], default="abmc192",
+                    help="abmc192: the single-system headline; batch: many systems at once")
+    ap.add_argument("--systems", type=int, default=512)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -174,6 +340,8 @@ def main():
     if I.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "batch":
+        return main_batch(args)
     N = args.size
     Ap, sched, b, info = build_problem(N, args.block)
     n, nnz = Ap.n, Ap.nnz

@@ -96,6 +96,9 @@ _SIGS = {
     "iccg_batch_get_x": (_i, [_vp, _i, _vp]),
     "iccg_batch_remove": (_i, [_vp, _i]),
     "iccg_batch_set_graphs": (_i, [_vp, _i]),
+    "iccg_batch_set_autotune": (_i, [_vp, _i]),
+    "iccg_batch_get_tuning": (_i, [_vp, _ip, _ip]),
+    "iccg_batch_set_tuning": (_i, [_vp, _i, _i]),
 }
 
 _lib = None

@@ -278,8 +278,12 @@ def abmc_blocking_grid(nx, ny, nz, bx, by, bz):
     return bo, nb.value, bc[: nb.value].copy(), nc.value
 
 
-def gen_grid(nx, ny=1, nz=1, coef_exp=(0.0, 0.0), diag_shift=0.0, seed=0x1CC6) -> CRS:
-    """2D 5-point / 3D 7-point finite-volume matrix (see iccg_gen_grid)."""
+def gen_grid(nx, ny=1, nz=1, coef_exp=(0.0, 0.0), diag_shift=0.0, seed=0x1CC6,
+             fp32_values=False) -> CRS:
+    """2D 5-point / 3D 7-point finite-volume matrix (see iccg_gen_grid).
+
+    fp32_values: round every value to the nearest float, the diagonal
+    upwards (so it still dominates), for systems stored with FP32 values."""
     n = C.c_int(0)
     nnz = C.c_long(0)
     lo, hi = coef_exp
@@ -290,6 +294,12 @@ def gen_grid(nx, ny=1, nz=1, coef_exp=(0.0, 0.0), diag_shift=0.0, seed=0x1CC6) -
     v = np.zeros(nnz.value)
     check(lib().iccg_gen_grid(nx, ny, nz, lo, hi, diag_shift, seed, C.byref(n), C.byref(nnz),
                               _p(rp), _p(ci), _p(v)))
+    if fp32_values:
+        v32 = v.astype(np.float32)
+        diag = ci == np.repeat(np.arange(n.value, dtype=np.int32), np.diff(rp))
+        low = diag & (v32.astype(np.float64) < v)
+        v32[low] = np.nextafter(v32[low], np.float32(np.inf))
+        v = v32.astype(np.float64)
     return CRS(n.value, rp, ci, v)
 
 
@@ -538,7 +548,8 @@ class Batch:
     """Many independent systems advanced together (iccg_batch_*): every CG
     phase is one launch over all running systems; systems may be added while
     others run.  Each system needs an IC0 preconditioner built with a
-    contiguous multicolor schedule (e.g. from greedy_color)."""
+    contiguous multicolor schedule (e.g. from greedy_color) or an ABMC
+    schedule of contiguous blocks of at most 64 rows."""
 
     def __init__(self, capacity):
         h = _vp()
@@ -582,3 +593,18 @@ class Batch:
 
     def set_graphs(self, on=True):
         check(lib().iccg_batch_set_graphs(self._h.h, int(on)))
+
+    def set_autotune(self, on=True):
+        check(lib().iccg_batch_set_autotune(self._h.h, int(on)))
+
+    @property
+    def tuning(self):
+        """(64-row slices per workgroup of the SpMV/vector kernels, slices or
+        ABMC blocks per wavefront of the sweeps)"""
+        a, b = C.c_int(0), C.c_int(0)
+        check(lib().iccg_batch_get_tuning(self._h.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    @tuning.setter
+    def tuning(self, v):
+        check(lib().iccg_batch_set_tuning(self._h.h, int(v[0]), int(v[1])))

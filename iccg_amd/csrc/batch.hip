@@ -3,21 +3,24 @@
 // together so that every CG phase is ONE launch over all active systems.
 //
 // Work is described by item lists that name a system slot:
-//   ChunkItem (slot, s0, s1): up to kChunk 64-row slices of one system, one
+//   ChunkItem (slot, s0, s1): 64-row slices [s0, s1) of one system, one
 //     workgroup -- SpMV, vector updates and dot-product partials;
-//   ColorItem (slot, slice, lo, hi): rows [lo, hi) of one color inside one
-//     slice, one wavefront -- the multicolor triangular sweeps.
+//   ColorItem (slot, a0, a1): one wavefront of a triangular sweep over one
+//     color of one system -- 64-row slices (multicolor) or blocks (ABMC).
 // Every system keeps its own scalars (SysScal); a finish kernel with one
 // workgroup per system folds that system's partials.  Converged systems
 // skip their items, and systems can be admitted or retired between launches:
-// the serving loop's continuous batching.  Per system, the arithmetic is
-// the single-system solver's: same row chains, same double-double dots, so
-// a system gives the same iterates in a batch as alone.
+// the serving loop's continuous batching.  Each system keeps its own
+// precision (FP64, or FP32 factor with FP64 CG) and its own ordering
+// (multicolor or ABMC); per system, the arithmetic is the single-system
+// solver's: same row chains, same double-double dots, so a system gives the
+// same iterates in a batch as alone.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <vector>
 
@@ -28,21 +31,23 @@
 
 namespace iccg {
 
-constexpr int kChunk = 16;  // slices per chunk item (1024 rows)
-
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-template <class AV>
-__device__ __forceinline__ const AV *a_vals(const SysDesc &D);
-template <>
-__device__ __forceinline__ const double *a_vals<double>(const SysDesc &D) { return D.a_v64; }
-template <>
-__device__ __forceinline__ const float *a_vals<float>(const SysDesc &D) { return D.a_v32; }
+// (A x)_i of the lane's row in slice s, from the f32 copy of A when every
+// value is an exact float (same products, same fma chain)
+template <int WM>
+__device__ __forceinline__ double a_row(const SysDesc &D, const double *x, int s, int lane) {
+  if (D.a_v32)
+    return sell_chain<float, double, double, WM, false>(D.a_off, D.a_col, D.a_v32, x, s, lane,
+                                                         0.0);
+  return sell_chain<double, double, double, WM, false>(D.a_off, D.a_col, D.a_v64, x, s, lane,
+                                                        0.0);
+}
 
 // r = b - A x with b.b and r.r partials (mode 1 / 2 write r; mode 0 only
 // measures the true residual)
-template <class AV, int WM>
+template <int WM>
 __global__ __launch_bounds__(kWG) void kb_resid(const SysDesc *__restrict__ desc,
                                                 const ChunkItem *__restrict__ items,
                                                 double *part, int mode) {
@@ -51,8 +56,7 @@ __global__ __launch_bounds__(kWG) void kb_resid(const SysDesc *__restrict__ desc
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   double h[2] = {0, 0}, l[2] = {0, 0};
   for (int s = c.s0 + w; s < c.s1; s += kWG >> 6) {
-    const double ax = sell_chain<AV, double, double, WM, false>(D.a_off, D.a_col, a_vals<AV>(D),
-                                                                D.x, s, lane, 0.0);
+    const double ax = a_row<WM>(D, D.x, s, lane);
     const int i = (s << 6) + lane;
     if (i < D.n) {
       const double bi = D.b[i], ri = bi - ax;
@@ -64,7 +68,8 @@ __global__ __launch_bounds__(kWG) void kb_resid(const SysDesc *__restrict__ desc
   block_partial<2>(h, l, part, blockIdx.x);
 }
 
-template <class AV, int WM>
+// Ap = A p with p.Ap, p.p, Ap.Ap partials
+template <int WM>
 __global__ __launch_bounds__(kWG) void kb_spmv(const SysDesc *__restrict__ desc,
                                                const ChunkItem *__restrict__ items,
                                                const SysScal *__restrict__ sc, double *part) {
@@ -74,8 +79,7 @@ __global__ __launch_bounds__(kWG) void kb_spmv(const SysDesc *__restrict__ desc,
     const SysDesc &D = desc[c.sys];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int s = c.s0 + w; s < c.s1; s += kWG >> 6) {
-      const double y = sell_chain<AV, double, double, WM, false>(D.a_off, D.a_col, a_vals<AV>(D),
-                                                                 D.p, s, lane, 0.0);
+      const double y = a_row<WM>(D, D.p, s, lane);
       const int i = (s << 6) + lane;
       if (i < D.n) {
         D.Ap[i] = y;
@@ -111,6 +115,12 @@ __global__ __launch_bounds__(kWG) void kb_xr(const SysDesc *__restrict__ desc,
 }
 
 template <class FT>
+__device__ __forceinline__ void rz_rows(const SysDesc &D, int beg, int end, double (&h)[1],
+                                        double (&l)[1]) {
+  const FT *z = (const FT *)D.z;
+  for (int i = beg; i < end; i += kWG) dd_fma(h[0], l[0], D.r[i], (double)z[i]);
+}
+
 __global__ __launch_bounds__(kWG) void kb_rz(const SysDesc *__restrict__ desc,
                                              const ChunkItem *__restrict__ items,
                                              const SysScal *__restrict__ sc, double *part) {
@@ -118,71 +128,120 @@ __global__ __launch_bounds__(kWG) void kb_rz(const SysDesc *__restrict__ desc,
   double h[1] = {0}, l[1] = {0};
   if (!sc[c.sys].done) {
     const SysDesc &D = desc[c.sys];
-    const FT *z = (const FT *)D.z;
-    const int end = min(D.n, c.s1 << 6);
-    for (int i = (c.s0 << 6) + threadIdx.x; i < end; i += kWG)
-      dd_fma(h[0], l[0], D.r[i], (double)z[i]);
+    const int beg = (c.s0 << 6) + threadIdx.x, end = min(D.n, c.s1 << 6);
+    if (D.mixed)
+      rz_rows<float>(D, beg, end, h, l);
+    else
+      rz_rows<double>(D, beg, end, h, l);
   }
   block_partial<1>(h, l, part, blockIdx.x);
 }
 
 template <class FT>
+__device__ __forceinline__ void p_rows(const SysDesc &D, int beg, int end, double beta,
+                                       int init) {
+  const FT *z = (const FT *)D.z;
+  for (int i = beg; i < end; i += kWG) {
+    const double zi = (double)z[i];
+    D.p[i] = init ? zi : fma(beta, D.p[i], zi);
+  }
+}
+
 __global__ __launch_bounds__(kWG) void kb_update_p(const SysDesc *__restrict__ desc,
                                                    const ChunkItem *__restrict__ items,
                                                    const SysScal *__restrict__ sc, int init) {
   const ChunkItem c = items[blockIdx.x];
   if (sc[c.sys].done) return;
   const SysDesc &D = desc[c.sys];
-  const FT *z = (const FT *)D.z;
   const double beta = sc[c.sys].beta;
-  const int end = min(D.n, c.s1 << 6);
-  for (int i = (c.s0 << 6) + threadIdx.x; i < end; i += kWG) {
-    const double zi = (double)z[i];
-    D.p[i] = init ? zi : fma(beta, D.p[i], zi);
+  const int beg = (c.s0 << 6) + threadIdx.x, end = min(D.n, c.s1 << 6);
+  if (D.mixed)
+    p_rows<float>(D, beg, end, beta, init);
+  else
+    p_rows<double>(D, beg, end, beta, init);
+}
+
+// ---- triangular sweeps (ic0.cpp:194-249 multicolor, 263-339 ABMC) --------
+template <class FT, int WM>
+__device__ __forceinline__ void mc_fwd_item(const SysDesc &D, const ColorItem &c, int lane) {
+  FT *y = (FT *)D.y;
+  const FT *lv = (const FT *)D.l_v, *dg = (const FT *)D.d;
+  for (int s = c.a0; s < c.a1; ++s) {
+    const int i = (s << 6) + lane;
+    if (i < c.lo || i >= c.hi) continue;
+    FT acc = (FT)D.r[i];
+    acc = sell_chain<FT, FT, FT, WM, true>(D.l_off, D.l_col, lv, y, s, lane, acc);
+    y[i] = acc / dg[i];
   }
 }
 
-// multicolor sweeps, one wavefront per color item (ic0.cpp:194-249)
 template <class FT, int WM>
-__global__ __launch_bounds__(kWG) void kb_mc_fwd(const SysDesc *__restrict__ desc,
-                                                 const ColorItem *__restrict__ items, int nitems,
-                                                 const SysScal *__restrict__ sc) {
-  const int wv = blockIdx.x * (kWG >> 6) + (threadIdx.x >> 6);
-  if (wv >= nitems) return;
-  const ColorItem c = items[wv];
-  if (sc[c.sys].done) return;
-  const SysDesc &D = desc[c.sys];
-  const int lane = threadIdx.x & 63;
-  const int i = (c.slice << 6) + lane;
-  if (i < c.lo || i >= c.hi) return;
-  FT *y = (FT *)D.y;
-  FT acc = (FT)D.r[i];
-  acc = sell_chain<FT, FT, FT, WM, true>(D.l_off, D.l_col, (const FT *)D.l_v, y, c.slice, lane,
-                                         acc);
-  y[i] = acc / ((const FT *)D.d)[i];
+__device__ __forceinline__ void mc_bwd_item(const SysDesc &D, const ColorItem &c, int lane) {
+  FT *z = (FT *)D.z;
+  const FT *y = (const FT *)D.y, *uv = (const FT *)D.u_v, *dg = (const FT *)D.d;
+  for (int s = c.a0; s < c.a1; ++s) {
+    const int i = (s << 6) + lane;
+    if (i < c.lo || i >= c.hi) continue;
+    FT acc = y[i];
+    acc = sell_chain<FT, FT, FT, WM, true>(D.u_off, D.u_col, uv, z, s, lane, acc);
+    z[i] = acc / dg[i];
+  }
 }
 
 template <class FT, int WM>
-__global__ __launch_bounds__(kWG) void kb_mc_bwd(const SysDesc *__restrict__ desc,
-                                                 const ColorItem *__restrict__ items, int nitems,
-                                                 const SysScal *__restrict__ sc) {
+__device__ __forceinline__ void abmc_fwd_item(const SysDesc &D, const ColorItem &c, int lane,
+                                              FT *my) {
+  for (int kb = c.a0; kb < c.a1; ++kb)
+    abmc_fwd_block<FT, FT, WM>(D.bstart[kb], D.bstart[kb + 1], D.nlev_f[kb], D.lev_f, D.l_off,
+                               D.l_col, (const FT *)D.l_v, (const FT *)D.d, D.r, (FT *)D.y, my,
+                               lane);
+}
+
+template <class FT, int WM>
+__device__ __forceinline__ void abmc_bwd_item(const SysDesc &D, const ColorItem &c, int lane,
+                                              FT *my) {
+  double ri;
+  for (int kb = c.a0; kb < c.a1; ++kb)
+    (void)abmc_bwd_block<FT, FT, WM>(D.bstart[kb], D.bstart[kb + 1], D.nlev_b[kb], D.lev_b,
+                                     D.u_off, D.u_col, (const FT *)D.u_v, (const FT *)D.d,
+                                     (const FT *)D.y, (FT *)D.z, nullptr, ri, my, lane);
+}
+
+// one wavefront per color item; every branch below is uniform per wavefront
+template <int WM, bool FWD>
+__global__ __launch_bounds__(kWG) void kb_sweep(const SysDesc *__restrict__ desc,
+                                                const ColorItem *__restrict__ items, int nitems,
+                                                const SysScal *__restrict__ sc) {
+  __shared__ double lds[kWG];
   const int wv = blockIdx.x * (kWG >> 6) + (threadIdx.x >> 6);
   if (wv >= nitems) return;
   const ColorItem c = items[wv];
   if (sc[c.sys].done) return;
   const SysDesc &D = desc[c.sys];
   const int lane = threadIdx.x & 63;
-  const int i = (c.slice << 6) + lane;
-  if (i < c.lo || i >= c.hi) return;
-  FT *z = (FT *)D.z;
-  FT acc = ((const FT *)D.y)[i];
-  acc = sell_chain<FT, FT, FT, WM, true>(D.u_off, D.u_col, (const FT *)D.u_v, z, c.slice, lane,
-                                         acc);
-  z[i] = acc / ((const FT *)D.d)[i];
+  double *my = lds + ((threadIdx.x >> 6) << 6);
+  if (D.bstart) {
+    if (D.mixed) {
+      if (FWD) abmc_fwd_item<float, WM>(D, c, lane, (float *)my);
+      else abmc_bwd_item<float, WM>(D, c, lane, (float *)my);
+    } else {
+      if (FWD) abmc_fwd_item<double, WM>(D, c, lane, my);
+      else abmc_bwd_item<double, WM>(D, c, lane, my);
+    }
+  } else {
+    if (D.mixed) {
+      if (FWD) mc_fwd_item<float, WM>(D, c, lane);
+      else mc_bwd_item<float, WM>(D, c, lane);
+    } else {
+      if (FWD) mc_fwd_item<double, WM>(D, c, lane);
+      else mc_bwd_item<double, WM>(D, c, lane);
+    }
+  }
 }
 
 // finish kernels: workgroup k folds the partials of the k-th listed system
 // (chunks [cptr[k], cptr[k+1])) and applies the single-system scalar logic
+// (cg_crs.cpp:104-160)
 template <int NC>
 __device__ __forceinline__ bool fold_system(const int *cptr, const double *part,
                                             double (&out)[NC]) {
@@ -193,11 +252,13 @@ __device__ __forceinline__ bool fold_system(const int *cptr, const double *part,
 
 __global__ __launch_bounds__(kWG) void kbf_resid(const int *__restrict__ act,
                                                  const int *__restrict__ cptr,
-                                                 const double *__restrict__ part, SysScal *sc,
+                                                 const double *__restrict__ part,
+                                                 const SysDesc *__restrict__ desc, SysScal *sc,
                                                  int mode) {
   double out[2];
   if (!fold_system<2>(cptr, part, out)) return;
-  SysScal &S = sc[act[blockIdx.x]];
+  const int id = act[blockIdx.x];
+  SysScal &S = sc[id];
   double normb = sqrt(out[0]);
   if (normb == 0.0) normb = 1.0;
   S.normb = normb;
@@ -206,7 +267,11 @@ __global__ __launch_bounds__(kWG) void kbf_resid(const int *__restrict__ act,
     return;
   }
   const double rel = sqrt(out[1]) / normb;
-  if (mode == 1) S.iters = 0;
+  if (mode == 1) {
+    S.iters = 0;
+    S.tol = desc[id].tol;
+    S.max_iter = desc[id].max_iter;
+  }
   S.rel = rel;
   S.converged = 0;
   S.breakdown = 0;
@@ -281,18 +346,20 @@ using namespace iccg;
 // ---------------------------------------------------------------------------
 namespace {
 
-// Item lists of a set of systems, resident on the device.
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
+
+// Item lists of a set of systems, resident on the device in one buffer.
 struct ItemSet {
-  int nsys = 0, nchunks = 0, maxc = 0;
-  std::vector<int> ncolor_items;  // per color
+  int nsys = 0, nchunks = 0, ncolors = 0;
+  std::vector<int> color_off;  // ncolors + 1 offsets into d_color
+  std::vector<int> chunks_of;  // per listed system
+  char *buf = nullptr;
   int *d_act = nullptr, *d_cptr = nullptr;
   ChunkItem *d_chunks = nullptr;
-  std::vector<ColorItem *> d_color;
+  ColorItem *d_color = nullptr;
   void clear() {
-    dfree(d_act);
-    dfree(d_cptr);
-    dfree(d_chunks);
-    for (auto p : d_color) dfree(p);
+    dfree(buf);
     *this = ItemSet();
   }
 };
@@ -302,14 +369,23 @@ struct Slot {
   iccg_csr_t A = nullptr;
   iccg_precond_t M = nullptr;
   int n = 0;
+  // device vectors x r p Ap b y z in one allocation, kept for reuse
+  char *dbuf = nullptr;
+  size_t dcap = 0;
+  // pinned staging of b (and x0) until admission, kept for reuse
+  double *hstage = nullptr;
+  size_t hcap = 0;
+  bool has_x0 = false;
   double *x = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr, *b = nullptr;
   void *y = nullptr, *z = nullptr;
   int max_iter = 0, flags = 0, restarts = 0;
   double tol = 0;
-  bool checked = false;
+  int listed_chunks = 0;  // chunk items of this slot in the running set
   std::chrono::steady_clock::time_point t_add;
   iccg_cg_result res{};
 };
+
+int wm_class(int w) { return w <= 4 ? 4 : w <= 8 ? 8 : 16; }
 
 }  // namespace
 
@@ -319,36 +395,39 @@ struct iccg_batch_s {
   int cap = 0;
   std::vector<Slot> slots;
   SysDesc *d_desc = nullptr;
+  SysDesc *h_desc = nullptr;  // pinned mirror
   SysScal *d_sc = nullptr;
-  SysScal *h_sc = nullptr;  // pinned
+  SysScal *h_sc = nullptr;    // pinned
   double *part = nullptr;
   size_t part_slots = 0;
-  bool a32 = true, mixed = false, have_kind = false;
-  int wm = 4;  // widest sliced row of any admitted system (template width)
-  ItemSet running;          // items of the running set
+  int wm = 4;                 // template width of the running set
+  int chunk = 16;             // slices per chunk item
+  int cw = 1;                 // slices (MC) or blocks (ABMC) per sweep wavefront
+  bool autotune = true;
+  long tuned_rows = 0;        // running rows at the last tuning
+  ItemSet running;            // items of the running set
   bool running_dirty = true;
+  int dead_chunks = 0;        // listed chunks of systems no longer running
   bool graphs = true;
   std::vector<std::pair<int, hipGraphExec_t>> execs;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
   ~iccg_batch_s() {
     (void)hipSetDevice(device);
-    for (auto &e : execs) (void)hipGraphExecDestroy(e.second);
+    if (st) (void)hipStreamSynchronize(st);
+    drop_graphs();
     running.clear();
-    for (auto &s : slots) free_slot(s);
+    for (auto &s : slots) {
+      dfree(s.dbuf);
+      if (s.hstage) (void)hipHostFree(s.hstage);
+    }
     dfree(d_desc);
     dfree(d_sc);
     dfree(part);
     if (h_sc) (void)hipHostFree(h_sc);
+    if (h_desc) (void)hipHostFree(h_desc);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
     if (st) (void)hipStreamDestroy(st);
-  }
-  static void free_slot(Slot &s) {
-    dfree(s.x);
-    dfree(s.r);
-    dfree(s.p);
-    dfree(s.Ap);
-    dfree(s.b);
-    dfree(s.y);
-    dfree(s.z);
-    s = Slot();
   }
   void drop_graphs() {
     for (auto &e : execs) (void)hipGraphExecDestroy(e.second);
@@ -362,13 +441,15 @@ void ensure_part(iccg_batch_s &B, size_t slots) {
   if (slots <= B.part_slots) return;
   HIPCHK(hipStreamSynchronize(B.st));
   dfree(B.part);
+  B.part = nullptr;
+  B.part_slots = 0;
   B.part = dalloc<double>(slots * 2);
   B.part_slots = slots;
   B.drop_graphs();
 }
 
-// item lists for the given slots
-void build_items(iccg_batch_s &B, const std::vector<int> &ids, ItemSet &I) {
+// item lists for the given slots with chunk / sweep granularities ch, cw
+void build_items(iccg_batch_s &B, const std::vector<int> &ids, ItemSet &I, int ch, int cw) {
   I.clear();
   std::vector<int> act, cptr{0};
   std::vector<ChunkItem> chunks;
@@ -377,83 +458,100 @@ void build_items(iccg_batch_s &B, const std::vector<int> &ids, ItemSet &I) {
     const Slot &s = B.slots[id];
     const int ns = (s.n + 63) / 64;
     act.push_back(id);
-    for (int a = 0; a < ns; a += kChunk) chunks.push_back({id, a, std::min(ns, a + kChunk), 0});
+    const int c_before = (int)chunks.size();
+    for (int a = 0; a < ns; a += ch) chunks.push_back({id, a, std::min(ns, a + ch), 0});
     cptr.push_back((int)chunks.size());
-    const std::vector<int> &cp = s.M->sc.h_color_ptr_v;
-    const int nc = (int)cp.size() - 1;
+    I.chunks_of.push_back((int)chunks.size() - c_before);
+    const DevSched &sc = s.M->sc;
+    const int nc = sc.nc;
     if ((int)color.size() < nc) color.resize(nc);
     for (int c = 0; c < nc; ++c) {
-      const int lo = cp[c], hi = cp[c + 1];
-      if (hi <= lo) continue;
-      for (int sl = lo >> 6; sl <= (hi - 1) >> 6; ++sl) color[c].push_back({id, sl, lo, hi});
+      if (sc.kind == 2) {
+        const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
+        for (int kb = b0; kb < b1; kb += cw)
+          color[c].push_back({id, kb, std::min(b1, kb + cw), 0, 0, {0, 0, 0}});
+      } else {
+        const int lo = sc.h_color_ptr_v[c], hi = sc.h_color_ptr_v[c + 1];
+        if (hi <= lo) continue;
+        const int s1 = ((hi - 1) >> 6) + 1;
+        for (int sl = lo >> 6; sl < s1; sl += cw)
+          color[c].push_back({id, sl, std::min(s1, sl + cw), lo, hi, {0, 0, 0}});
+      }
     }
   }
   I.nsys = (int)act.size();
   I.nchunks = (int)chunks.size();
-  I.maxc = (int)color.size();
+  I.ncolors = (int)color.size();
+  I.color_off.assign(1, 0);
+  for (auto &v : color) I.color_off.push_back(I.color_off.back() + (int)v.size());
   if (I.nsys == 0) return;
-  I.d_act = dupload(act.data(), act.size(), B.st);
-  I.d_cptr = dupload(cptr.data(), cptr.size(), B.st);
-  I.d_chunks = dupload(chunks.data(), chunks.size(), B.st);
+  // one device buffer: act | cptr | chunks | color items
+  const size_t o_cptr = align_up(act.size() * sizeof(int));
+  const size_t o_chunks = o_cptr + align_up(cptr.size() * sizeof(int));
+  const size_t o_color = o_chunks + align_up(chunks.size() * sizeof(ChunkItem));
+  const size_t total = o_color + (size_t)I.color_off.back() * sizeof(ColorItem);
+  std::vector<char> h(total);
+  std::memcpy(h.data(), act.data(), act.size() * sizeof(int));
+  std::memcpy(h.data() + o_cptr, cptr.data(), cptr.size() * sizeof(int));
+  std::memcpy(h.data() + o_chunks, chunks.data(), chunks.size() * sizeof(ChunkItem));
+  size_t o = o_color;
   for (auto &v : color) {
-    I.ncolor_items.push_back((int)v.size());
-    I.d_color.push_back(v.empty() ? nullptr : dupload(v.data(), v.size(), B.st));
+    if (!v.empty()) std::memcpy(h.data() + o, v.data(), v.size() * sizeof(ColorItem));
+    o += v.size() * sizeof(ColorItem);
   }
   ensure_part(B, (size_t)I.nchunks * 3);
+  I.buf = dalloc<char>(total);
+  I.d_act = (int *)I.buf;
+  I.d_cptr = (int *)(I.buf + o_cptr);
+  I.d_chunks = (ChunkItem *)(I.buf + o_chunks);
+  I.d_color = (ColorItem *)(I.buf + o_color);
+  HIPCHK(hipMemcpyAsync(I.buf, h.data(), total, hipMemcpyHostToDevice, B.st));
   HIPCHK(hipStreamSynchronize(B.st));
 }
 
-#define BATCH_WM(wm, K, AV, ...)                                           \
-  do {                                                                     \
-    if ((wm) <= 4) hipLaunchKernelGGL((K<AV, 4>), __VA_ARGS__);            \
-    else if ((wm) <= 8) hipLaunchKernelGGL((K<AV, 8>), __VA_ARGS__);       \
-    else hipLaunchKernelGGL((K<AV, 16>), __VA_ARGS__);                     \
+#define BATCH_WM(wm, K, ...)                                          \
+  do {                                                                \
+    if ((wm) <= 4) hipLaunchKernelGGL((K<4>), __VA_ARGS__);           \
+    else if ((wm) <= 8) hipLaunchKernelGGL((K<8>), __VA_ARGS__);      \
+    else hipLaunchKernelGGL((K<16>), __VA_ARGS__);                    \
+  } while (0)
+
+#define BATCH_WM2(wm, K, F, ...)                                      \
+  do {                                                                \
+    if ((wm) <= 4) hipLaunchKernelGGL((K<4, F>), __VA_ARGS__);        \
+    else if ((wm) <= 8) hipLaunchKernelGGL((K<8, F>), __VA_ARGS__);   \
+    else hipLaunchKernelGGL((K<16, F>), __VA_ARGS__);                 \
   } while (0)
 
 void launch_resid_items(iccg_batch_s &B, const ItemSet &I, int mode) {
   if (I.nsys == 0) return;
   dim3 g(I.nchunks), b(kWG);
-  if (B.a32)
-    BATCH_WM(B.wm, kb_resid, float, g, b, 0, B.st, B.d_desc, I.d_chunks, B.part, mode);
-  else
-    BATCH_WM(B.wm, kb_resid, double, g, b, 0, B.st, B.d_desc, I.d_chunks, B.part, mode);
-  hipLaunchKernelGGL(kbf_resid, dim3(I.nsys), b, 0, B.st, I.d_act, I.d_cptr, B.part, B.d_sc,
-                     mode);
+  BATCH_WM(B.wm, kb_resid, g, b, 0, B.st, B.d_desc, I.d_chunks, B.part, mode);
+  hipLaunchKernelGGL(kbf_resid, dim3(I.nsys), b, 0, B.st, I.d_act, I.d_cptr, B.part, B.d_desc,
+                     B.d_sc, mode);
 }
 
 void launch_sweeps(iccg_batch_s &B, const ItemSet &I) {
   dim3 b(kWG);
-  for (int c = 0; c < I.maxc; ++c) {
-    const int ni = I.ncolor_items[c];
-    if (!ni) continue;
-    dim3 g((ni + 3) / 4);
-    if (B.mixed)
-      BATCH_WM(B.wm, kb_mc_fwd, float, g, b, 0, B.st, B.d_desc, I.d_color[c], ni, B.d_sc);
-    else
-      BATCH_WM(B.wm, kb_mc_fwd, double, g, b, 0, B.st, B.d_desc, I.d_color[c], ni, B.d_sc);
+  for (int c = 0; c < I.ncolors; ++c) {
+    const int ni = I.color_off[c + 1] - I.color_off[c];
+    if (ni)
+      BATCH_WM2(B.wm, kb_sweep, true, dim3((ni + 3) / 4), b, 0, B.st, B.d_desc,
+                I.d_color + I.color_off[c], ni, B.d_sc);
   }
-  for (int c = I.maxc - 1; c >= 0; --c) {
-    const int ni = I.ncolor_items[c];
-    if (!ni) continue;
-    dim3 g((ni + 3) / 4);
-    if (B.mixed)
-      BATCH_WM(B.wm, kb_mc_bwd, float, g, b, 0, B.st, B.d_desc, I.d_color[c], ni, B.d_sc);
-    else
-      BATCH_WM(B.wm, kb_mc_bwd, double, g, b, 0, B.st, B.d_desc, I.d_color[c], ni, B.d_sc);
+  for (int c = I.ncolors - 1; c >= 0; --c) {
+    const int ni = I.color_off[c + 1] - I.color_off[c];
+    if (ni)
+      BATCH_WM2(B.wm, kb_sweep, false, dim3((ni + 3) / 4), b, 0, B.st, B.d_desc,
+                I.d_color + I.color_off[c], ni, B.d_sc);
   }
 }
 
 void launch_rz_p(iccg_batch_s &B, const ItemSet &I, int init) {
   dim3 g(I.nchunks), b(kWG);
-  if (B.mixed) {
-    hipLaunchKernelGGL(kb_rz<float>, g, b, 0, B.st, B.d_desc, I.d_chunks, B.d_sc, B.part);
-    hipLaunchKernelGGL(kbf_rz, dim3(I.nsys), b, 0, B.st, I.d_act, I.d_cptr, B.part, B.d_sc, init);
-    hipLaunchKernelGGL(kb_update_p<float>, g, b, 0, B.st, B.d_desc, I.d_chunks, B.d_sc, init);
-  } else {
-    hipLaunchKernelGGL(kb_rz<double>, g, b, 0, B.st, B.d_desc, I.d_chunks, B.d_sc, B.part);
-    hipLaunchKernelGGL(kbf_rz, dim3(I.nsys), b, 0, B.st, I.d_act, I.d_cptr, B.part, B.d_sc, init);
-    hipLaunchKernelGGL(kb_update_p<double>, g, b, 0, B.st, B.d_desc, I.d_chunks, B.d_sc, init);
-  }
+  hipLaunchKernelGGL(kb_rz, g, b, 0, B.st, B.d_desc, I.d_chunks, B.d_sc, B.part);
+  hipLaunchKernelGGL(kbf_rz, dim3(I.nsys), b, 0, B.st, I.d_act, I.d_cptr, B.part, B.d_sc, init);
+  hipLaunchKernelGGL(kb_update_p, g, b, 0, B.st, B.d_desc, I.d_chunks, B.d_sc, init);
 }
 
 // z0 = M^-1 r0, rz_old, p = z0 (cg_crs.cpp:112-115) for the listed systems
@@ -463,12 +561,10 @@ void launch_precond_init(iccg_batch_s &B, const ItemSet &I) {
   launch_rz_p(B, I, 1);
 }
 
+// one CG iteration (cg_crs.cpp:125-160) of every listed system
 void launch_iteration(iccg_batch_s &B, const ItemSet &I) {
   dim3 g(I.nchunks), b(kWG);
-  if (B.a32)
-    BATCH_WM(B.wm, kb_spmv, float, g, b, 0, B.st, B.d_desc, I.d_chunks, B.d_sc, B.part);
-  else
-    BATCH_WM(B.wm, kb_spmv, double, g, b, 0, B.st, B.d_desc, I.d_chunks, B.d_sc, B.part);
+  BATCH_WM(B.wm, kb_spmv, g, b, 0, B.st, B.d_desc, I.d_chunks, B.d_sc, B.part);
   hipLaunchKernelGGL(kbf_alpha, dim3(I.nsys), b, 0, B.st, I.d_act, I.d_cptr, B.part, B.d_sc);
   hipLaunchKernelGGL(kb_xr, g, b, 0, B.st, B.d_desc, I.d_chunks, B.d_sc, B.part);
   hipLaunchKernelGGL(kbf_xr, dim3(I.nsys), b, 0, B.st, I.d_act, I.d_cptr, B.part, B.d_sc);
@@ -504,6 +600,11 @@ void read_scalars(iccg_batch_s &B) {
   HIPCHK(hipStreamSynchronize(B.st));
 }
 
+void retire_listed(iccg_batch_s &B, Slot &s) {
+  B.dead_chunks += s.listed_chunks;
+  s.listed_chunks = 0;
+}
+
 void finish_slot(iccg_batch_s &B, int id) {
   Slot &s = B.slots[id];
   const SysScal &S = B.h_sc[id];
@@ -516,22 +617,20 @@ void finish_slot(iccg_batch_s &B, int id) {
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - s.t_add)
           .count();
   s.state = 3;
-  B.running_dirty = true;
+  retire_listed(B, s);
 }
 
-// Systems whose recurrence converged get the FP64 true residual; those
-// above tol restart from it (ICCG_SOLVE_TRUE_RESID), the others finish.
+// Systems whose recurrence stopped get the FP64 true residual; converged
+// ones above tol restart from it (ICCG_SOLVE_TRUE_RESID), the others finish.
 void settle(iccg_batch_s &B) {
-  std::vector<int> check, done_now;
+  std::vector<int> done_now;
   for (int id = 0; id < B.cap; ++id) {
     Slot &s = B.slots[id];
-    if (s.state != 2 || !B.h_sc[id].done) continue;
-    done_now.push_back(id);
+    if (s.state == 2 && B.h_sc[id].done) done_now.push_back(id);
   }
   if (done_now.empty()) return;
-  // true residual of every system that stopped (reported, and tested when asked)
   ItemSet I;
-  build_items(B, done_now, I);
+  build_items(B, done_now, I, B.chunk, B.cw);
   launch_resid_items(B, I, 0);
   HIPCHK(hipGetLastError());
   read_scalars(B);
@@ -553,12 +652,14 @@ void settle(iccg_batch_s &B) {
   }
   if (!restart.empty()) {
     ItemSet R;
-    build_items(B, restart, R);
+    build_items(B, restart, R, B.chunk, B.cw);
     launch_resid_items(B, R, 2);
     launch_precond_init(B, R);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(B.st));
     R.clear();
+    // restarted systems run again: their scalars are fresh on the device
+    read_scalars(B);
   }
 }
 
@@ -567,8 +668,24 @@ void admit_pending(iccg_batch_s &B) {
   for (int id = 0; id < B.cap; ++id)
     if (B.slots[id].state == 1) ids.push_back(id);
   if (ids.empty()) return;
+  for (int id : ids) {
+    const Slot &s = B.slots[id];
+    const size_t nb = sizeof(double) * (size_t)s.n;
+    HIPCHK(hipMemcpyAsync(s.b, s.hstage, nb, hipMemcpyHostToDevice, B.st));
+    if (s.has_x0)
+      HIPCHK(hipMemcpyAsync(s.x, s.hstage + s.n, nb, hipMemcpyHostToDevice, B.st));
+    else
+      HIPCHK(hipMemsetAsync(s.x, 0, nb, B.st));
+  }
+  HIPCHK(hipMemcpyAsync(B.d_desc, B.h_desc, sizeof(SysDesc) * (size_t)B.cap,
+                        hipMemcpyHostToDevice, B.st));
+  int wm = 4;
+  for (const Slot &s : B.slots)
+    if (s.state == 1 || s.state == 2)
+      wm = std::max({wm, s.A->d.sell.maxw, s.M->F.ls.maxw, s.M->F.us.maxw});
+  B.wm = wm_class(wm);
   ItemSet I;
-  build_items(B, ids, I);
+  build_items(B, ids, I, B.chunk, B.cw);
   launch_resid_items(B, I, 1);
   launch_precond_init(B, I);
   HIPCHK(hipGetLastError());
@@ -578,14 +695,79 @@ void admit_pending(iccg_batch_s &B) {
   B.running_dirty = true;
 }
 
+double time_ms(iccg_batch_s &B, const std::function<void()> &f, int reps) {
+  f();  // warm
+  HIPCHK(hipEventRecord(B.ev0, B.st));
+  for (int r = 0; r < reps; ++r) f();
+  HIPCHK(hipEventRecord(B.ev1, B.st));
+  HIPCHK(hipEventSynchronize(B.ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, B.ev0, B.ev1));
+  return ms / reps;
+}
+
+// Autotuner: times the candidate work decompositions on the running systems
+// and keeps the fastest.  The probes are side-effect free at an iteration
+// boundary: the SpMV rewrites Ap and the sweeps rewrite y and z with the
+// values they already hold; partials are scratch.
+void tune(iccg_batch_s &B, const std::vector<int> &ids) {
+  if (!B.ev0) {
+    HIPCHK(hipEventCreate(&B.ev0));
+    HIPCHK(hipEventCreate(&B.ev1));
+  }
+  const int chunks[] = {4, 8, 16, 32};
+  const int cws[] = {1, 2, 4};
+  double best = 1e300;
+  int best_ch = B.chunk;
+  ItemSet T;
+  for (int ch : chunks) {
+    build_items(B, ids, T, ch, B.cw);
+    const double ms = time_ms(B, [&] {
+      dim3 g(T.nchunks), b(kWG);
+      BATCH_WM(B.wm, kb_spmv, g, b, 0, B.st, B.d_desc, T.d_chunks, B.d_sc, B.part);
+      hipLaunchKernelGGL(kb_rz, g, b, 0, B.st, B.d_desc, T.d_chunks, B.d_sc, B.part);
+    }, 3);
+    if (ms < best) {
+      best = ms;
+      best_ch = ch;
+    }
+  }
+  best = 1e300;
+  int best_cw = B.cw;
+  for (int cw : cws) {
+    build_items(B, ids, T, best_ch, cw);
+    const double ms = time_ms(B, [&] { launch_sweeps(B, T); }, 3);
+    if (ms < best) {
+      best = ms;
+      best_cw = cw;
+    }
+  }
+  HIPCHK(hipGetLastError());
+  T.clear();
+  B.chunk = best_ch;
+  B.cw = best_cw;
+}
+
 void refresh_running(iccg_batch_s &B) {
-  if (!B.running_dirty) return;
+  if (!B.running_dirty && B.dead_chunks * 4 <= B.running.nchunks) return;
   std::vector<int> ids;
+  long rows = 0;
   for (int id = 0; id < B.cap; ++id)
-    if (B.slots[id].state == 2) ids.push_back(id);
+    if (B.slots[id].state == 2 && !B.h_sc[id].done) {
+      ids.push_back(id);
+      rows += B.slots[id].n;
+    }
   B.drop_graphs();
-  build_items(B, ids, B.running);
+  if (B.autotune && rows >= (1L << 16) &&
+      (B.tuned_rows == 0 || rows > 2 * B.tuned_rows || 2 * rows < B.tuned_rows)) {
+    tune(B, ids);
+    B.tuned_rows = rows;
+  }
+  for (Slot &s : B.slots) s.listed_chunks = 0;
+  build_items(B, ids, B.running, B.chunk, B.cw);
+  for (size_t k = 0; k < ids.size(); ++k) B.slots[ids[k]].listed_chunks = B.running.chunks_of[k];
   B.running_dirty = false;
+  B.dead_chunks = 0;
 }
 
 }  // namespace
@@ -606,9 +788,16 @@ int iccg_batch_create(int capacity, iccg_batch_t *out) {
     HIPCHK(hipStreamCreateWithFlags(&B->st, hipStreamNonBlocking));
     B->d_desc = dalloc<SysDesc>((size_t)capacity);
     B->d_sc = dalloc<SysScal>((size_t)capacity);
-    HIPCHK(hipMemsetAsync(B->d_sc, 0, sizeof(SysScal) * (size_t)capacity, B->st));
     HIPCHK(hipHostMalloc((void **)&B->h_sc, sizeof(SysScal) * (size_t)capacity,
                          hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&B->h_desc, sizeof(SysDesc) * (size_t)capacity,
+                         hipHostMallocDefault));
+    std::memset((void *)B->h_sc, 0, sizeof(SysScal) * (size_t)capacity);
+    for (int k = 0; k < capacity; ++k) B->h_desc[k] = SysDesc();
+    HIPCHK(hipMemcpyAsync(B->d_sc, B->h_sc, sizeof(SysScal) * (size_t)capacity,
+                          hipMemcpyHostToDevice, B->st));
+    HIPCHK(hipMemcpyAsync(B->d_desc, B->h_desc, sizeof(SysDesc) * (size_t)capacity,
+                          hipMemcpyHostToDevice, B->st));
     ensure_part(*B, 4096);
     HIPCHK(hipStreamSynchronize(B->st));
     *out = B.release();
@@ -625,19 +814,11 @@ int iccg_batch_add(iccg_batch_t B, iccg_csr_t A, iccg_precond_t M, const double 
   return guard([&] {
     require(B && A && M && b && slot, ICCG_ERR_ARG, "null argument");
     require(M->A == A, ICCG_ERR_ARG, "preconditioner was built for another matrix");
-    require(M->kind == ICCG_PRECOND_IC0 && M->sc.kind == 1 && M->sc.d_rows == nullptr &&
-                M->F.sell && A->d.sell.off,
+    require(M->kind == ICCG_PRECOND_IC0 && sliced_plan(M->sc, M->F) && A->d.sell.off &&
+                A->d.sell.maxw <= 16,
             ICCG_ERR_ARG,
-            "batched solves need an IC0 preconditioner with a contiguous multicolor schedule");
-    const bool a32 = A->d.sell_v32 != nullptr;
-    bool any = false;
-    for (const Slot &o : B->slots) any |= o.state != 0;
-    require(!any || (B->mixed == M->mixed && B->a32 == a32), ICCG_ERR_ARG,
-            "all systems of a batch share one precision and one matrix value type");
-    const int wm = std::max({any ? B->wm : 0, A->d.sell.maxw, M->F.ls.maxw, M->F.us.maxw});
-    require(wm <= 16, ICCG_ERR_ARG, "rows too long for the batched sliced kernels");
-    B->mixed = M->mixed;
-    B->a32 = a32;
+            "batched solves need an IC0 preconditioner with a contiguous multicolor or ABMC "
+            "(blocks of at most 64 rows) schedule and rows of at most 16 entries");
     int id = -1;
     for (int k = 0; k < B->cap; ++k)
       if (B->slots[k].state == 0) {
@@ -648,6 +829,35 @@ int iccg_batch_add(iccg_batch_t B, iccg_csr_t A, iccg_precond_t M, const double 
     HIPCHK(hipSetDevice(B->device));
     Slot &s = B->slots[id];
     const int n = A->d.n;
+    const size_t vb = M->mixed ? sizeof(float) : sizeof(double);
+    const size_t v8 = align_up(sizeof(double) * (size_t)n), vf = align_up(vb * (size_t)n);
+    const size_t need = 5 * v8 + 2 * vf;
+    if (s.dcap < need) {
+      dfree(s.dbuf);
+      s.dbuf = nullptr;
+      s.dcap = 0;
+      s.dbuf = dalloc<char>(need);
+      s.dcap = need;
+    }
+    const size_t hneed = sizeof(double) * 2 * (size_t)n;
+    if (s.hcap < hneed) {
+      if (s.hstage) (void)hipHostFree(s.hstage);
+      s.hstage = nullptr;
+      s.hcap = 0;
+      HIPCHK(hipHostMalloc((void **)&s.hstage, hneed, hipHostMallocDefault));
+      s.hcap = hneed;
+    }
+    std::memcpy(s.hstage, b, sizeof(double) * (size_t)n);
+    s.has_x0 = x0 != nullptr;
+    if (x0) std::memcpy(s.hstage + n, x0, sizeof(double) * (size_t)n);
+    char *q = s.dbuf;
+    s.x = (double *)q;
+    s.r = (double *)(q += v8);
+    s.p = (double *)(q += v8);
+    s.Ap = (double *)(q += v8);
+    s.b = (double *)(q += v8);
+    s.y = (void *)(q += v8);
+    s.z = (void *)(q += vf);
     s.A = A;
     s.M = M;
     s.n = n;
@@ -655,24 +865,16 @@ int iccg_batch_add(iccg_batch_t B, iccg_csr_t A, iccg_precond_t M, const double 
     s.tol = tol;
     s.flags = flags;
     s.restarts = 0;
+    s.listed_chunks = 0;
     s.res = iccg_cg_result{};
     s.res.true_rel_resid = -1.0;
-    const size_t vb = M->mixed ? sizeof(float) : sizeof(double);
-    s.x = dalloc<double>((size_t)n);
-    s.r = dalloc<double>((size_t)n);
-    s.p = dalloc<double>((size_t)n);
-    s.Ap = dalloc<double>((size_t)n);
-    s.b = dupload(b, (size_t)n, B->st);
-    s.y = dalloc<char>(vb * (size_t)n);
-    s.z = dalloc<char>(vb * (size_t)n);
-    if (x0)
-      HIPCHK(hipMemcpyAsync(s.x, x0, sizeof(double) * n, hipMemcpyHostToDevice, B->st));
-    else
-      HIPCHK(hipMemsetAsync(s.x, 0, sizeof(double) * n, B->st));
-    SysDesc D;
+    SysDesc &D = B->h_desc[id];
+    D = SysDesc();
     D.n = n;
     D.nslice = (n + 63) / 64;
     D.mixed = M->mixed;
+    D.max_iter = max_iter;
+    D.tol = tol;
     D.a_off = A->d.sell.off;
     D.a_col = A->d.sell.col;
     D.a_v64 = A->d.sell.v64;
@@ -684,6 +886,13 @@ int iccg_batch_add(iccg_batch_t B, iccg_csr_t A, iccg_precond_t M, const double 
     D.l_v = M->mixed ? (const void *)M->F.ls.v32 : (const void *)M->F.ls.v64;
     D.u_v = M->mixed ? (const void *)M->F.us.v32 : (const void *)M->F.us.v64;
     D.d = M->mixed ? (const void *)M->F.d32 : (const void *)M->F.d64;
+    if (M->sc.kind == 2) {
+      D.bstart = M->sc.d_bstart;
+      D.lev_f = M->sc.d_lev_f;
+      D.lev_b = M->sc.d_lev_b;
+      D.nlev_f = M->sc.d_nlev_f;
+      D.nlev_b = M->sc.d_nlev_b;
+    }
     D.x = s.x;
     D.r = s.r;
     D.p = s.p;
@@ -691,13 +900,6 @@ int iccg_batch_add(iccg_batch_t B, iccg_csr_t A, iccg_precond_t M, const double 
     D.b = s.b;
     D.y = s.y;
     D.z = s.z;
-    B->wm = wm;
-    SysScal S{};
-    S.tol = tol;
-    S.max_iter = max_iter;
-    HIPCHK(hipMemcpyAsync(B->d_desc + id, &D, sizeof(D), hipMemcpyHostToDevice, B->st));
-    HIPCHK(hipMemcpyAsync(B->d_sc + id, &S, sizeof(S), hipMemcpyHostToDevice, B->st));
-    HIPCHK(hipStreamSynchronize(B->st));
     s.t_add = std::chrono::steady_clock::now();
     s.state = 1;
     *slot = id;
@@ -713,10 +915,10 @@ int iccg_batch_run(iccg_batch_t B, int max_steps, int *n_running) {
     settle(*B);  // systems already converged at iteration 0
     int steps = 0, chunk = 4;
     for (;;) {
-      refresh_running(*B);
       int running = 0;
       for (auto &s : B->slots) running += s.state == 2;
       if (running == 0 || steps >= max_steps) break;
+      refresh_running(*B);
       const int k = std::min(chunk, max_steps - steps);
       enqueue_iterations(*B, k);
       HIPCHK(hipGetLastError());
@@ -745,7 +947,7 @@ int iccg_batch_get_x(iccg_batch_t B, int slot, double *x) {
   return guard([&] {
     require(B && slot >= 0 && slot < B->cap && x, ICCG_ERR_ARG, "bad argument");
     const Slot &s = B->slots[slot];
-    require(s.state != 0, ICCG_ERR_STATE, "empty batch slot");
+    require(s.state >= 2, ICCG_ERR_STATE, "batch slot holds no solution");
     HIPCHK(hipSetDevice(B->device));
     HIPCHK(hipMemcpyAsync(x, s.x, sizeof(double) * (size_t)s.n, hipMemcpyDeviceToHost, B->st));
     HIPCHK(hipStreamSynchronize(B->st));
@@ -759,11 +961,16 @@ int iccg_batch_remove(iccg_batch_t B, int slot) {
     if (s.state == 0) return;
     HIPCHK(hipSetDevice(B->device));
     HIPCHK(hipStreamSynchronize(B->st));
-    if (s.state == 2) B->running_dirty = true;
-    iccg_batch_s::free_slot(s);
+    retire_listed(*B, s);
+    // the vector and staging buffers stay with the slot for its next system
+    s.state = 0;
+    s.A = nullptr;
+    s.M = nullptr;
     SysScal S{};
     S.done = 1;
-    HIPCHK(hipMemcpyAsync(B->d_sc + slot, &S, sizeof(S), hipMemcpyHostToDevice, B->st));
+    B->h_sc[slot] = S;
+    HIPCHK(hipMemcpyAsync(B->d_sc + slot, B->h_sc + slot, sizeof(S), hipMemcpyHostToDevice,
+                          B->st));
     HIPCHK(hipStreamSynchronize(B->st));
   });
 }
@@ -772,6 +979,34 @@ int iccg_batch_set_graphs(iccg_batch_t B, int on) {
   return guard([&] {
     require(B, ICCG_ERR_ARG, "null argument");
     B->graphs = on != 0;
+  });
+}
+
+int iccg_batch_set_autotune(iccg_batch_t B, int on) {
+  return guard([&] {
+    require(B, ICCG_ERR_ARG, "null argument");
+    B->autotune = on != 0;
+    B->tuned_rows = 0;
+  });
+}
+
+int iccg_batch_get_tuning(iccg_batch_t B, int *chunk_slices, int *sweep_units) {
+  return guard([&] {
+    require(B && chunk_slices && sweep_units, ICCG_ERR_ARG, "null argument");
+    *chunk_slices = B->chunk;
+    *sweep_units = B->cw;
+  });
+}
+
+int iccg_batch_set_tuning(iccg_batch_t B, int chunk_slices, int sweep_units) {
+  return guard([&] {
+    require(B && chunk_slices >= 1 && chunk_slices <= 1024 && sweep_units >= 1 &&
+                sweep_units <= 64,
+            ICCG_ERR_ARG, "bad argument");
+    B->chunk = chunk_slices;
+    B->cw = sweep_units;
+    B->autotune = false;
+    B->running_dirty = true;
   });
 }
 

@@ -125,11 +125,16 @@ struct Factor {
 struct SysDesc {
   int n = 0, nslice = 0;
   int mixed = 0;                     // factor, y, z in f32
+  int max_iter = 0;
+  double tol = 0;
   const int *a_off = nullptr, *a_col = nullptr;  // A, sliced ELL
   const double *a_v64 = nullptr;
   const float *a_v32 = nullptr;      // set when every value of A is an exact float
   const int *l_off = nullptr, *l_col = nullptr, *u_off = nullptr, *u_col = nullptr;
   const void *l_v = nullptr, *u_v = nullptr, *d = nullptr;
+  // ABMC blocks (nullptr for a multicolor system)
+  const int *bstart = nullptr, *nlev_f = nullptr, *nlev_b = nullptr;
+  const uint16_t *lev_f = nullptr, *lev_b = nullptr;
   double *x = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr;
   const double *b = nullptr;
   void *y = nullptr, *z = nullptr;
@@ -144,8 +149,11 @@ struct alignas(64) SysScal {
 struct ChunkItem {  // rows [64*s0, min(64*s1, n)) of slot sys: one workgroup
   int sys, s0, s1, pad;
 };
-struct ColorItem {  // rows [lo, hi) of slot sys inside 64-row slice `slice`
-  int sys, slice, lo, hi;
+// One wavefront of a sweep over one color of slot sys:
+//   multicolor: 64-row slices [a0, a1), rows restricted to [lo, hi);
+//   ABMC      : blocks [a0, a1).
+struct ColorItem {
+  int sys, a0, a1, lo, hi, pad[3];
 };
 
 // ---- launchers (kernels.hip) ----------------------------------------------
@@ -183,6 +191,9 @@ void launch_bwd(const DevSched &sc, const Factor &F, bool mixed, const void *y, 
                 const Scal *S, hipStream_t st, const double *r = nullptr, Scal *Sw = nullptr,
                 double *part = nullptr, int rz = 0);
 // true when launch_bwd fuses r.z for this schedule (sliced MC / ABMC plans)
+// the triangular solves run on the sliced-ELL sweeps (contiguous MC colors,
+// or contiguous ABMC blocks of at most 64 rows; rows at most 16 wide)
+bool sliced_plan(const DevSched &sc, const Factor &F);
 bool bwd_fuses_rz(const DevSched &sc, const Factor &F);
 int bwd_partial_slots(const DevSched &sc, const Factor &F);
 void launch_scatter_sell(const Factor &F, bool mixed, hipStream_t st);

@@ -224,5 +224,127 @@ __device__ __forceinline__ AT sell_chain(const int *__restrict__ off, const int 
   return acc;
 }
 
+// ---------------------------------------------------------------------------
+// ABMC sweeps of one block of at most 64 rows (ic0.cpp:263-339) by one
+// wavefront.  The block's rows are the contiguous new ids [bs, be) and lane
+// t owns row bs + t; my is the wave's 64-entry LDS window.
+//   forward : entries with column < bs belong to earlier colors (final) and
+//             come first in the ascending row; they are summed before the
+//             level loop.  In-block entries [bs, i) are read from LDS as the
+//             in-block levels complete.
+//   backward: in-block entries (i, be) come first, then columns >= be of
+//             later colors, whose values are prefetched before the levels.
+// The per-row operation order equals the sequential reference, bit for bit.
+// Both store the block's outputs once, coalesced, after the level loop.
+// ---------------------------------------------------------------------------
+template <class VT, class YT, int WM>
+__device__ __forceinline__ void abmc_fwd_block(int bs, int be, int nl,
+                                               const uint16_t *__restrict__ lev,
+                                               const int *__restrict__ off,
+                                               const int *__restrict__ col,
+                                               const VT *__restrict__ lv,
+                                               const VT *__restrict__ diag,
+                                               const double *__restrict__ r, YT *__restrict__ y,
+                                               YT *my, int lane) {
+  const int i = bs + lane;
+  const bool act = i < be;
+  int cc[WM];
+  VT vv[WM];
+  YT acc = (YT)0, d = (YT)1;
+  int mylev = -1;
+  if (act) {
+    const int s = i >> 6, ln = i & 63;
+    const int o0 = off[s];
+    const int wd = (off[s + 1] - o0) >> 6;
+    const int base = o0 + ln;
+#pragma unroll
+    for (int k = 0; k < WM; ++k) {
+      cc[k] = -1;
+      vv[k] = (VT)0;
+      if (k < wd) {
+        cc[k] = col[base + 64 * k];
+        vv[k] = lv[base + 64 * k];
+      }
+    }
+    mylev = lev[i];
+    d = (YT)diag[i];
+    acc = (YT)r[i];
+    YT xo[WM];
+#pragma unroll
+    for (int k = 0; k < WM; ++k) xo[k] = (cc[k] >= 0 && cc[k] < bs) ? y[cc[k]] : (YT)0;
+#pragma unroll
+    for (int k = 0; k < WM; ++k)
+      if (cc[k] >= 0 && cc[k] < bs) acc = fma_t(-(YT)vv[k], xo[k], acc);
+  }
+  YT yi = (YT)0;
+  for (int l = 0; l < nl; ++l) {
+    if (mylev == l) {
+#pragma unroll
+      for (int k = 0; k < WM; ++k)
+        if (cc[k] >= bs) acc = fma_t(-(YT)vv[k], my[cc[k] - bs], acc);
+      yi = acc / d;
+      my[lane] = yi;
+    }
+    wave_sync();
+  }
+  if (act) y[i] = yi;
+}
+
+// returns z_i (0 on lanes past the block); with r, also r_i in ri
+template <class VT, class YT, int WM>
+__device__ __forceinline__ YT abmc_bwd_block(int bs, int be, int nl,
+                                             const uint16_t *__restrict__ lev,
+                                             const int *__restrict__ off,
+                                             const int *__restrict__ col,
+                                             const VT *__restrict__ uv,
+                                             const VT *__restrict__ diag,
+                                             const YT *__restrict__ y, YT *__restrict__ z,
+                                             const double *__restrict__ r, double &ri, YT *my,
+                                             int lane) {
+  const int i = bs + lane;
+  const bool act = i < be;
+  int cc[WM];
+  VT vv[WM];
+  YT xo[WM];
+  YT acc = (YT)0, d = (YT)1, zi = (YT)0;
+  int mylev = -1;
+  ri = 0.0;
+  if (act) {
+    const int s = i >> 6, ln = i & 63;
+    const int o0 = off[s];
+    const int wd = (off[s + 1] - o0) >> 6;
+    const int base = o0 + ln;
+#pragma unroll
+    for (int k = 0; k < WM; ++k) {
+      cc[k] = -1;
+      vv[k] = (VT)0;
+      if (k < wd) {
+        cc[k] = col[base + 64 * k];
+        vv[k] = uv[base + 64 * k];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < WM; ++k) xo[k] = cc[k] >= be ? z[cc[k]] : (YT)0;
+    mylev = lev[i];
+    d = (YT)diag[i];
+    acc = y[i];
+    if (r) ri = r[i];
+  }
+  for (int lv = 0; lv < nl; ++lv) {
+    if (mylev == lv) {
+#pragma unroll
+      for (int k = 0; k < WM; ++k)
+        if (cc[k] >= 0 && cc[k] < be) acc = fma_t(-(YT)vv[k], my[cc[k] - bs], acc);
+#pragma unroll
+      for (int k = 0; k < WM; ++k)
+        if (cc[k] >= be) acc = fma_t(-(YT)vv[k], xo[k], acc);
+      zi = acc / d;
+      my[lane] = zi;
+    }
+    wave_sync();
+  }
+  if (act) z[i] = zi;
+  return zi;
+}
 
 }  // namespace iccg

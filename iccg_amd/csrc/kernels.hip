@@ -512,16 +512,8 @@ __global__ __launch_bounds__(kWG) void k_mc_bwd(int c0, int c1, Sell U, const VT
 }
 
 // ---------------------------------------------------------------------------
-// ABMC sweeps for blocks of at most 64 rows (ic0.cpp:263-339): one wavefront
-// per block, blocks of a color spread over the grid.  A block's rows are the
-// contiguous new ids [bs, be) and lane t owns row bs + t.
-//   forward : entries with column < bs belong to earlier colors (final) and
-//             come first in the ascending row; they are summed before the
-//             level loop.  In-block entries [bs, i) are read from LDS as the
-//             in-block levels complete.
-//   backward: in-block entries (i, be) come first, then columns >= be of
-//             later colors, whose values are prefetched before the levels.
-// The per-row operation order equals the sequential reference, bit for bit.
+// ABMC sweeps for blocks of at most 64 rows: one wavefront per block
+// (abmc_fwd_block / abmc_bwd_block), blocks of a color spread over the grid.
 // ---------------------------------------------------------------------------
 template <class VT, class YT, int WM>
 __global__ __launch_bounds__(kWG) void k_abmc_fwd_wave(int kb0, int kb1,
@@ -537,52 +529,9 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_wave(int kb0, int kb1,
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   YT *my = ys + (w << 6);
   const WaveRange wr = xcd_waves(kb0, kb1);
-  for (int kb = wr.first; kb < wr.end; kb += wr.step) {
-    const int bs = bstart[kb], be = bstart[kb + 1];
-    const int i = bs + lane;
-    const bool act = i < be;
-    int cc[WM];
-    VT vv[WM];
-    YT acc = (YT)0, d = (YT)1;
-    int mylev = -1;
-    if (act) {
-      const int s = i >> 6, ln = i & 63;
-      const int o0 = L.off[s];
-      const int wd = (L.off[s + 1] - o0) >> 6;
-      const int base = o0 + ln;
-#pragma unroll
-      for (int k = 0; k < WM; ++k) {
-        cc[k] = -1;
-        vv[k] = (VT)0;
-        if (k < wd) {
-          cc[k] = L.col[base + 64 * k];
-          vv[k] = lv[base + 64 * k];
-        }
-      }
-      mylev = lev[i];
-      d = (YT)diag[i];
-      acc = (YT)r[i];
-      YT xo[WM];
-#pragma unroll
-      for (int k = 0; k < WM; ++k) xo[k] = (cc[k] >= 0 && cc[k] < bs) ? y[cc[k]] : (YT)0;
-#pragma unroll
-      for (int k = 0; k < WM; ++k)
-        if (cc[k] >= 0 && cc[k] < bs) acc = fma_t(-(YT)vv[k], xo[k], acc);
-    }
-    const int nl = nlev[kb];
-    YT yi = (YT)0;
-    for (int l = 0; l < nl; ++l) {
-      if (mylev == l) {
-#pragma unroll
-        for (int k = 0; k < WM; ++k)
-          if (cc[k] >= bs) acc = fma_t(-(YT)vv[k], my[cc[k] - bs], acc);
-        yi = acc / d;
-        my[lane] = yi;
-      }
-      wave_sync();
-    }
-    if (act) y[i] = yi;  // one coalesced store per block
-  }
+  for (int kb = wr.first; kb < wr.end; kb += wr.step)
+    abmc_fwd_block<VT, YT, WM>(bstart[kb], bstart[kb + 1], nlev[kb], lev, L.off, L.col, lv, diag,
+                               r, y, my, lane);
 }
 
 template <class VT, class YT, int WM, bool RZ>
@@ -599,51 +548,10 @@ __global__ __launch_bounds__(kWG) void k_abmc_bwd_wave(
   const WaveRange wr = xcd_waves(kb0, kb1);
   for (int kb = wr.first; kb < wr.end; kb += wr.step) {
     const int bs = bstart[kb], be = bstart[kb + 1];
-    const int i = bs + lane;
-    const bool act = i < be;
-    int cc[WM];
-    VT vv[WM];
-    YT xo[WM];
-    YT acc = (YT)0, d = (YT)1, zi = (YT)0;
-    double ri = 0.0;
-    int mylev = -1;
-    if (act) {
-      const int s = i >> 6, ln = i & 63;
-      const int o0 = U.off[s];
-      const int wd = (U.off[s + 1] - o0) >> 6;
-      const int base = o0 + ln;
-#pragma unroll
-      for (int k = 0; k < WM; ++k) {
-        cc[k] = -1;
-        vv[k] = (VT)0;
-        if (k < wd) {
-          cc[k] = U.col[base + 64 * k];
-          vv[k] = uv[base + 64 * k];
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < WM; ++k) xo[k] = cc[k] >= be ? z[cc[k]] : (YT)0;
-      mylev = lev[i];
-      d = (YT)diag[i];
-      acc = y[i];
-      if (RZ) ri = r[i];
-    }
-    const int nl = nlev[kb];
-    for (int lv = 0; lv < nl; ++lv) {
-      if (mylev == lv) {
-#pragma unroll
-        for (int k = 0; k < WM; ++k)
-          if (cc[k] >= 0 && cc[k] < be) acc = fma_t(-(YT)vv[k], my[cc[k] - bs], acc);
-#pragma unroll
-        for (int k = 0; k < WM; ++k)
-          if (cc[k] >= be) acc = fma_t(-(YT)vv[k], xo[k], acc);
-        zi = acc / d;
-        my[lane] = zi;
-      }
-      wave_sync();
-    }
-    if (act) z[i] = zi;  // one coalesced store per block
-    if (RZ && act) dd_fma(h[0], l[0], ri, (double)zi);
+    double ri;
+    const YT zi = abmc_bwd_block<VT, YT, WM>(bs, be, nlev[kb], lev, U.off, U.col, uv, diag, y, z,
+                                             RZ ? r : nullptr, ri, my, lane);
+    if (RZ && bs + lane < be) dd_fma(h[0], l[0], ri, (double)zi);
   }
   if (RZ) block_partial<1>(h, l, part, gidx0 + blockIdx.x);
 }
@@ -857,7 +765,7 @@ static void run_csr_plan(const DevSched &sc, F f, const Scal *S, hipStream_t st,
 
 // sliced plans are used for MC with contiguous colors and ABMC with
 // contiguous blocks of <= 64 rows, when the factor has sliced copies
-static bool sliced_plan(const DevSched &sc, const Factor &F) {
+bool sliced_plan(const DevSched &sc, const Factor &F) {
   if (!F.sell || std::max(F.ls.maxw, F.us.maxw) > 16) return false;
   if (sc.kind == 1) return sc.d_rows == nullptr;
   if (sc.kind == 2) return sc.d_brows == nullptr && sc.max_block <= 64;

@@ -251,14 +251,16 @@ int iccg_solver_get_timing(iccg_solver_t S, int cls, long *launches,
 int iccg_solver_reset_timing(iccg_solver_t S);
 
 /* ---- many systems at once (no reference counterpart) -----------------------
- * Independent systems, each with its own matrix, IC0 preconditioner (a
- * contiguous multicolor schedule, e.g. from iccg_greedy_color) and vectors
- * resident in HBM, advanced together: every CG phase is one launch over all
- * running systems, each with its own alpha/beta/stop test.  Systems can be
- * added while others run (continuous batching); a system's iterates equal
- * those of iccg_cg_solve on it alone.  All systems of one batch share the
- * preconditioner precision.  The matrix and preconditioner must outlive the
- * slot. */
+ * Independent systems, each with its own matrix, IC0 preconditioner and
+ * vectors resident in HBM, advanced together: every CG phase is one launch
+ * over all running systems, each with its own alpha/beta/stop test.  The
+ * preconditioner needs a sliced plan: a contiguous multicolor schedule (e.g.
+ * from iccg_greedy_color) or an ABMC schedule of contiguous blocks of at most
+ * 64 rows; rows of A, L and L^T hold at most 16 entries.  Systems may differ
+ * in size, ordering kind and precision, and can be added while others run
+ * (continuous batching); a system's iterates equal those of iccg_cg_solve on
+ * it alone.  The matrix and preconditioner must outlive the slot.  One batch
+ * is driven by one thread at a time. */
 typedef struct iccg_batch_s *iccg_batch_t;
 int iccg_batch_create(int capacity, iccg_batch_t *out);
 int iccg_batch_destroy(iccg_batch_t B);
@@ -274,6 +276,17 @@ int iccg_batch_get_x(iccg_batch_t B, int slot, double *x);
 /* Frees a slot (finished or not). */
 int iccg_batch_remove(iccg_batch_t B, int slot);
 int iccg_batch_set_graphs(iccg_batch_t B, int on);
+/* Work decomposition of the batched kernels: 64-row slices per workgroup of
+ * the SpMV / vector kernels (chunk_slices) and 64-row slices (multicolor)
+ * or blocks (ABMC) per wavefront of the triangular sweeps (sweep_units).
+ * With autotune on (the default), the batch times the candidates on its
+ * running systems whenever their total size changes by more than 2x
+ * (from 65,536 rows up) and keeps the fastest; set_tuning fixes them and
+ * turns autotune off.  The arithmetic of every row is the same for every
+ * decomposition. */
+int iccg_batch_set_autotune(iccg_batch_t B, int on);
+int iccg_batch_get_tuning(iccg_batch_t B, int *chunk_slices, int *sweep_units);
+int iccg_batch_set_tuning(iccg_batch_t B, int chunk_slices, int sweep_units);
 
 /* ---- measurement helpers -------------------------------------------------- */
 /* Device copy of nbytes (HBM -> HBM) repeated reps times; average ms per
