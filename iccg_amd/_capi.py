@@ -36,6 +36,23 @@ class CGResultC(C.Structure):
                 ("restarts", C.c_int), ("solve_ms", C.c_double)]
 
 
+SERVE_BATCHED, SERVE_SEQUENTIAL = 0, 1
+
+
+class ServerConfigC(C.Structure):
+    _fields_ = [("mode", C.c_int), ("precision", C.c_int), ("max_slots", C.c_int),
+                ("workers", C.c_int), ("block_size", C.c_int), ("flags", C.c_int),
+                ("hbm_budget", C.c_longlong)]
+
+
+class ServerStatsC(C.Structure):
+    _fields_ = [("submitted", C.c_longlong), ("finished", C.c_longlong),
+                ("failed", C.c_longlong), ("hbm_in_use", C.c_longlong), ("waiting", C.c_int),
+                ("preparing", C.c_int), ("ready", C.c_int), ("running", C.c_int)]
+
+
+_llp = C.POINTER(C.c_longlong)
+
 # name -> (restype, argtypes)
 _SIGS = {
     "iccg_last_error": (C.c_char_p, []),
@@ -99,6 +116,15 @@ _SIGS = {
     "iccg_batch_set_autotune": (_i, [_vp, _i]),
     "iccg_batch_get_tuning": (_i, [_vp, _ip, _ip]),
     "iccg_batch_set_tuning": (_i, [_vp, _i, _i]),
+    "iccg_batch_trim": (_i, [_vp]),
+    "iccg_device_get": (_i, [_ip]),
+    "iccg_server_create": (_i, [C.POINTER(ServerConfigC), C.POINTER(_vp)]),
+    "iccg_server_destroy": (_i, [_vp]),
+    "iccg_server_submit": (_i, [_vp, _i, _l, _vp, _vp, _vp, _vp, _i, _vp, _i, _d, _llp]),
+    "iccg_server_wait": (_i, [_vp, C.c_longlong, _i, _llp]),
+    "iccg_server_result": (_i, [_vp, C.c_longlong, _vp, _vp, _dp, _dp]),
+    "iccg_server_release": (_i, [_vp, C.c_longlong]),
+    "iccg_server_get_stats": (_i, [_vp, C.POINTER(ServerStatsC)]),
 }
 
 _lib = None

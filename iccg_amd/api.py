@@ -608,3 +608,70 @@ class Batch:
     @tuning.setter
     def tuning(self, v):
         check(lib().iccg_batch_set_tuning(self._h.h, int(v[0]), int(v[1])))
+
+
+class Server:
+    """Request loop over one GPU (iccg_server_*).
+
+    mode "batched": requests are admitted by HBM budget, prepared (ABMC
+    ordering, P^T A P, IC(0)) on `workers` host threads and solved with
+    continuous batching, every right-hand side a slot of one Batch.
+    mode "sequential": the plain one-at-a-time path (prepare, then solve each
+    right-hand side alone) for comparison.  submit() returns at once; wait()
+    and result() collect finished requests."""
+
+    def __init__(self, mode="batched", precision="mixed", max_slots=256, workers=8,
+                 block_size=64, true_residual=None, hbm_budget=64 << 30):
+        prec = {"fp64": _capi.FP64, "mixed": _capi.MIXED}[precision]
+        if true_residual is None:
+            true_residual = prec == _capi.MIXED
+        cfg = _capi.ServerConfigC(
+            {"batched": _capi.SERVE_BATCHED, "sequential": _capi.SERVE_SEQUENTIAL}[mode], prec,
+            int(max_slots), int(workers), int(block_size),
+            _capi.SOLVE_TRUE_RESID if true_residual else 0, int(hbm_budget))
+        h = _vp()
+        check(lib().iccg_server_create(C.byref(cfg), C.byref(h)))
+        self._h = _Handle(h, lib().iccg_server_destroy)
+        self._shape = {}
+
+    def submit(self, A: CRS, b, grid=None, max_iter=10000, tol=1e-8):
+        b = np.ascontiguousarray(np.atleast_2d(np.asarray(b, np.float64)))
+        if b.shape[1] != A.n:
+            raise ValueError("b must be (n,) or (nrhs, n)")
+        g = None if grid is None else np.asarray(list(grid) + [1] * (3 - len(grid)), np.int32)
+        rid = C.c_longlong(0)
+        rp, ci, v = _i32(A.rowptr), _i32(A.colind), _f64(A.val)
+        check(lib().iccg_server_submit(self._h.h, A.n, len(ci), _p(rp), _p(ci), _p(v), _p(g),
+                                       b.shape[0], _p(b), int(max_iter), float(tol),
+                                       C.byref(rid)))
+        self._shape[rid.value] = b.shape
+        return rid.value
+
+    def wait(self, rid=-1, timeout_ms=-1):
+        """Request id once finished (rid < 0: any finished request not yet
+        returned), or None on timeout."""
+        out = C.c_longlong(-1)
+        check(lib().iccg_server_wait(self._h.h, int(rid), int(timeout_ms), C.byref(out)))
+        return None if out.value < 0 else out.value
+
+    def result(self, rid):
+        """(x of shape (nrhs, n), [CGResult], latency_ms, prepare_ms)"""
+        shape = self._shape[rid]
+        x = np.zeros(shape)
+        res = (_capi.CGResultC * shape[0])()
+        lat, prep = C.c_double(0), C.c_double(0)
+        check(lib().iccg_server_result(self._h.h, rid, _p(x), C.cast(res, _vp), C.byref(lat),
+                                       C.byref(prep)))
+        return x, [_result(r) for r in res], lat.value, prep.value
+
+    def release(self, rid):
+        check(lib().iccg_server_release(self._h.h, rid))
+        self._shape.pop(rid, None)
+
+    def stats(self):
+        st = _capi.ServerStatsC()
+        check(lib().iccg_server_get_stats(self._h.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def close(self):
+        self._h = None

@@ -653,6 +653,13 @@ int iccg_device_set(int device) {
   });
 }
 
+int iccg_device_get(int *device) {
+  return guard([&] {
+    require(device, ICCG_ERR_ARG, "null argument");
+    *device = g_device;
+  });
+}
+
 int iccg_csr_create(int n, long nnz, const int *rowptr, const int *colind, const double *val,
                     iccg_csr_t *out) {
   return guard([&] {

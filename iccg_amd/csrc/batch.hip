@@ -975,6 +975,23 @@ int iccg_batch_remove(iccg_batch_t B, int slot) {
   });
 }
 
+int iccg_batch_trim(iccg_batch_t B) {
+  return guard([&] {
+    require(B, ICCG_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(B->device));
+    HIPCHK(hipStreamSynchronize(B->st));
+    for (Slot &s : B->slots)
+      if (s.state == 0) {
+        dfree(s.dbuf);
+        s.dbuf = nullptr;
+        s.dcap = 0;
+        if (s.hstage) (void)hipHostFree(s.hstage);
+        s.hstage = nullptr;
+        s.hcap = 0;
+      }
+  });
+}
+
 int iccg_batch_set_graphs(iccg_batch_t B, int on) {
   return guard([&] {
     require(B, ICCG_ERR_ARG, "null argument");

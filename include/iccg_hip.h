@@ -76,6 +76,8 @@ const char *iccg_status_string(int status);
 int iccg_device_count(int *count);
 /* Selects the HIP device used by objects created afterwards on this thread. */
 int iccg_device_set(int device);
+/* Device used by this thread's calls (iccg_device_set, default 0). */
+int iccg_device_get(int *device);
 
 /* ---- L0: CRS I/O (crs_io.hpp:10-56) ---------------------------------------- */
 
@@ -287,6 +289,70 @@ int iccg_batch_set_graphs(iccg_batch_t B, int on);
 int iccg_batch_set_autotune(iccg_batch_t B, int on);
 int iccg_batch_get_tuning(iccg_batch_t B, int *chunk_slices, int *sweep_units);
 int iccg_batch_set_tuning(iccg_batch_t B, int chunk_slices, int sweep_units);
+/* Frees the device vectors that free slots keep for reuse (iccg_batch_remove
+ * keeps them, so a slot refilled with a system no larger allocates nothing). */
+int iccg_batch_trim(iccg_batch_t B);
+
+/* ---- request loop (no reference counterpart) -------------------------------
+ * Serves solve requests of different sizes on one GPU.  A request is a
+ * symmetric matrix (full CSR, sorted rows, caller's numbering) with nrhs
+ * right-hand sides; it is ordered ABMC (bx*by*bz boxes when the caller gives
+ * its structured-grid shape, breadth-first blocks of block_size rows
+ * otherwise, blocks colored first-fit), permuted, factorized IC(0) with
+ * shift 0 and solved to ||r||/||b|| < tol; x comes back in the caller's
+ * numbering.
+ *   ICCG_SERVE_BATCHED    requests are admitted in arrival order while their
+ *                         estimated HBM footprint fits hbm_budget, prepared
+ *                         by `workers` host threads (the IC(0) factorization
+ *                         runs on the GPU beside the running solves), and
+ *                         every right-hand side runs as one slot of a batch
+ *                         of max_slots (continuous batching: slots freed by
+ *                         finished solves take the next ready request).
+ *   ICCG_SERVE_SEQUENTIAL the plain one-at-a-time path: one request is
+ *                         prepared and its right-hand sides solved one after
+ *                         another with the single-system solver before the
+ *                         next request starts.
+ * A right-hand side gives the same x, bit for bit, in either mode and
+ * whatever else is being served. */
+enum { ICCG_SERVE_BATCHED = 0, ICCG_SERVE_SEQUENTIAL = 1 };
+typedef struct {
+  int mode;               /* ICCG_SERVE_* */
+  int precision;          /* ICCG_FP64 or ICCG_MIXED */
+  int max_slots;          /* right-hand sides in flight (batched mode) */
+  int workers;            /* host threads preparing requests (0 = 8) */
+  int block_size;         /* rows per ABMC block of a general matrix, <= 64 */
+  int flags;              /* ICCG_SOLVE_TRUE_RESID */
+  long long hbm_budget;   /* bytes of HBM for admitted requests */
+} iccg_server_config;
+typedef struct {
+  long long submitted, finished, failed, hbm_in_use;
+  int waiting, preparing, ready, running;
+} iccg_server_stats;
+typedef struct iccg_server_s *iccg_server_t;
+int iccg_server_create(const iccg_server_config *cfg, iccg_server_t *out);
+/* Stops the loop; requests still in flight are dropped. */
+int iccg_server_destroy(iccg_server_t S);
+/* Queues a request (everything is copied); grid = {nx, ny, nz} of a
+ * structured grid in natural order, or NULL.  b[k*n + i] is entry i of
+ * right-hand side k. */
+int iccg_server_submit(iccg_server_t S, int n, long nnz, const int *rowptr,
+                       const int *colind, const double *val, const int *grid,
+                       int nrhs, const double *b, int max_iter, double tol,
+                       long long *id);
+/* Waits up to timeout_ms (< 0: without limit) for request id to finish, or
+ * for any finished request not yet reported when id < 0; *done_id is the
+ * finished request or -1 on timeout. */
+int iccg_server_wait(iccg_server_t S, long long id, int timeout_ms,
+                     long long *done_id);
+/* x[k*n + i] and res[k] per right-hand side; latency = submit to finish,
+ * prepare = submit to ready for the GPU (ms).  A failed request returns its
+ * error status and message. */
+int iccg_server_result(iccg_server_t S, long long id, double *x,
+                       iccg_cg_result *res, double *latency_ms,
+                       double *prepare_ms);
+/* Forgets a finished request. */
+int iccg_server_release(iccg_server_t S, long long id);
+int iccg_server_get_stats(iccg_server_t S, iccg_server_stats *st);
 
 /* ---- measurement helpers -------------------------------------------------- */
 /* Device copy of nbytes (HBM -> HBM) repeated reps times; average ms per
