@@ -276,6 +276,9 @@ def main_batch(args):
         del Ss, Ms
         return count / w, out, xs1
 
+    if args.no_compare:
+        log("comparison legs skipped (--no-compare)")
+        return
     sps64, res64, xs64 = one_at_a_time("fp64")
     tr64 = np.array([r.true_rel_resid for r in res64])
     log(f"one-at-a-time FP64: {sps64:.1f} solves/s; max true rel {tr64.max():.2e}")
@@ -370,12 +373,17 @@ def main_serve(args):
         f"(median {int(np.median(ns))}), arrivals over {arrivals[-1]:.2f}s "
         f"({time.time() - t0:.1f}s to generate)")
 
-    def summary(out, span, label):
+    def summary(out, span, label, mixed):
         lat = np.array([o[0] for o in out])
         prep = np.array([o[1] for o in out])
         trel = max(r.true_rel_resid for o in out for r in o[2])
         conv = all(r.converged for o in out for r in o[2])
+        # algorithmic HBM bytes of every iteration of every right-hand side
+        byt = sum(iter_bytes(A.n, A.nnz, (A.nnz + A.n) // 2, mixed) * r.iters
+                  for (A, _, _), o in zip(reqs, out) for r in o[2])
         d = {"solves_per_s": round(n_rhs / span, 3), "requests_per_s": round(count / span, 3),
+             "alg_bytes": int(byt), "alg_gbs": round(byt / span / 1e9, 1),
+             "sum_rhs_solve_s": round(sum(r.solve_ms for o in out for r in o[2]) / 1e3, 3),
              "p50_latency_ms": round(float(np.percentile(lat, 50)), 2),
              "p99_latency_ms": round(float(np.percentile(lat, 99)), 2),
              "mean_prepare_ms": round(float(prep.mean()), 2), "span_s": round(span, 3),
@@ -387,11 +395,11 @@ def main_serve(args):
                            workers=args.workers, hbm_budget=int(args.hbm_budget * (1 << 30)))
     out_b, span_b = drive(served_mode, reqs, arrivals)
     del served_mode
-    sb = summary(out_b, span_b, "batched mixed")
+    sb = summary(out_b, span_b, "batched mixed", True)
     seq = I.Server("sequential", precision="fp64")
     out_s, span_s = drive(seq, reqs, arrivals)
     del seq
-    ss = summary(out_s, span_s, "one-at-a-time fp64")
+    ss = summary(out_s, span_s, "one-at-a-time fp64", False)
     err = max(float(np.linalg.norm(a[3] - c[3]) / np.linalg.norm(c[3]))
               for a, c in zip(out_b, out_s))
     line = {
@@ -423,6 +431,8 @@ def main():
                          "serve: the request loop")
     ap.add_argument("--systems", type=int, default=512)
     ap.add_argument("--requests", type=int, default=64)
+    ap.add_argument("--no-compare", action="store_true",
+                    help="batch workload: time the batched solve only (for profiling)")
     ap.add_argument("--rate", type=float, default=20.0, help="request arrivals per second")
     ap.add_argument("--slots", type=int, default=256)
     ap.add_argument("--workers", type=int, default=12)

@@ -117,6 +117,10 @@ _SIGS = {
     "iccg_batch_get_tuning": (_i, [_vp, _ip, _ip]),
     "iccg_batch_set_tuning": (_i, [_vp, _i, _i]),
     "iccg_batch_trim": (_i, [_vp]),
+    "iccg_batch_set_max_group": (_i, [_vp, _i]),
+    "iccg_batch_set_timing": (_i, [_vp, _i]),
+    "iccg_batch_get_timing": (_i, [_vp, _i, _lp, _dp]),
+    "iccg_batch_reset_timing": (_i, [_vp]),
     "iccg_device_get": (_i, [_ip]),
     "iccg_server_create": (_i, [C.POINTER(ServerConfigC), C.POINTER(_vp)]),
     "iccg_server_destroy": (_i, [_vp]),

@@ -597,6 +597,22 @@ class Batch:
     def set_autotune(self, on=True):
         check(lib().iccg_batch_set_autotune(self._h.h, int(on)))
 
+    def set_max_group(self, k):
+        check(lib().iccg_batch_set_max_group(self._h.h, int(k)))
+
+    def set_timing(self, on=True):
+        check(lib().iccg_batch_set_timing(self._h.h, int(on)))
+
+    def reset_timing(self):
+        check(lib().iccg_batch_reset_timing(self._h.h))
+
+    def timing(self, cls):
+        """(launch groups, total ms) of kernel class cls: 0 SpMV, 1 forward
+        sweeps, 2 backward sweeps, 3 vector updates and finish kernels."""
+        n, ms = C.c_long(0), C.c_double(0)
+        check(lib().iccg_batch_get_timing(self._h.h, cls, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
     @property
     def tuning(self):
         """(64-row slices per workgroup of the SpMV/vector kernels, slices or

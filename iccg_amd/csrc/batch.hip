@@ -2,25 +2,30 @@
 // its own matrix, IC(0) factor and Krylov vectors resident in HBM, advanced
 // together so that every CG phase is ONE launch over all active systems.
 //
-// Work is described by item lists that name a system slot:
-//   ChunkItem (slot, s0, s1): 64-row slices [s0, s1) of one system, one
-//     workgroup -- SpMV, vector updates and dot-product partials;
-//   ColorItem (slot, a0, a1): one wavefront of a triangular sweep over one
-//     color of one system -- 64-row slices (multicolor) or blocks (ABMC).
-// Every system keeps its own scalars (SysScal); a finish kernel with one
-// workgroup per system folds that system's partials.  Converged systems
-// skip their items, and systems can be admitted or retired between launches:
-// the serving loop's continuous batching.  Each system keeps its own
-// precision (FP64, or FP32 factor with FP64 CG) and its own ordering
-// (multicolor or ABMC); per system, the arithmetic is the single-system
-// solver's: same row chains, same double-double dots, so a system gives the
-// same iterates in a batch as alone.
+// Every right-hand side is a slot with its own vectors and scalars (SysDesc,
+// SysScal).  Slots of the same matrix and preconditioner form a group of up
+// to kMaxRhs right-hand sides; the kernels read each matrix and factor entry
+// once for the whole group.  Work is described by item lists:
+//   ChunkItem (group, s0, s1): 64-row slices [s0, s1), one workgroup --
+//     SpMV, vector updates and dot-product partials (one plane per group
+//     member);
+//   ColorItem (group, a0, a1): one wavefront of a triangular sweep over one
+//     color -- 64-row slices (multicolor) or blocks (ABMC);
+//   FinItem (slot, chunk range, plane): a finish kernel's workgroup folds
+//     one slot's partials and applies its scalar logic.
+// Converged slots skip their work, and slots can be admitted or retired
+// between launches: the serving loop's continuous batching.  Systems keep
+// their own precision (FP64, or FP32 factor with FP64 CG) and ordering
+// (multicolor or ABMC); per right-hand side, the arithmetic is the
+// single-system solver's -- same row chains, same double-double dots -- so
+// it gives the same iterates in a batch as alone.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -34,75 +39,118 @@ namespace iccg {
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-// (A x)_i of the lane's row in slice s, from the f32 copy of A when every
-// value is an exact float (same products, same fma chain)
-template <int WM>
-__device__ __forceinline__ double a_row(const SysDesc &D, const double *x, int s, int lane) {
-  if (D.a_v32)
-    return sell_chain<float, double, double, WM, false>(D.a_off, D.a_col, D.a_v32, x, s, lane,
-                                                         0.0);
-  return sell_chain<double, double, double, WM, false>(D.a_off, D.a_col, D.a_v64, x, s, lane,
-                                                        0.0);
+__device__ __forceinline__ double *plane(double *part, int j, int nch, int nc) {
+  return part + (size_t)j * nch * nc * 2;
 }
 
-// r = b - A x with b.b and r.r partials (mode 1 / 2 write r; mode 0 only
-// measures the true residual)
-template <int WM>
-__global__ __launch_bounds__(kWG) void kb_resid(const SysDesc *__restrict__ desc,
-                                                const ChunkItem *__restrict__ items,
-                                                double *part, int mode) {
-  const ChunkItem c = items[blockIdx.x];
-  const SysDesc &D = desc[c.sys];
+// A x for every member of the group over the item's slices (A's entries
+// loaded once per row and pair of members, from the f32 copy when every
+// value is an exact float): mode 0/1/2 = true residual / start / restart
+// (r = b - A x with b.b, r.r), mode 3 = CG SpMV (Ap = A p with p.Ap, p.p,
+// Ap.Ap).  Members are taken two at a time (the double-double partials of
+// more would cost occupancy); the later pairs find the item's entries in
+// cache.
+template <class VT, int WM, int KM>
+__device__ __forceinline__ void ax_group(const GroupDesc &g, const SysDesc *__restrict__ desc,
+                                         const SysScal *__restrict__ sc, const ChunkItem &c,
+                                         const VT *__restrict__ av, double *part, int nch,
+                                         int mode) {
+  constexpr int KS = KM < 2 ? KM : 2;
+  const SysDesc &D0 = desc[g.slot[0]];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  double h[2] = {0, 0}, l[2] = {0, 0};
-  for (int s = c.s0 + w; s < c.s1; s += kWG >> 6) {
-    const double ax = a_row<WM>(D, D.x, s, lane);
-    const int i = (s << 6) + lane;
-    if (i < D.n) {
-      const double bi = D.b[i], ri = bi - ax;
-      if (mode != 0) D.r[i] = ri;
-      dd_fma(h[0], l[0], bi, bi);
-      dd_fma(h[1], l[1], ri, ri);
+#pragma unroll
+  for (int j0 = 0; j0 < KM; j0 += KS) {
+    double h[KS][3], l[KS][3];
+    bool live[KS], any = false;
+#pragma unroll
+    for (int q = 0; q < KS; ++q) {
+      const int j = j0 + q;
+      live[q] = j < g.k && (mode != 3 || !sc[g.slot[j]].done);
+      any |= live[q];
+#pragma unroll
+      for (int t = 0; t < 3; ++t) h[q][t] = l[q][t] = 0.0;
     }
-  }
-  block_partial<2>(h, l, part, blockIdx.x);
-}
-
-// Ap = A p with p.Ap, p.p, Ap.Ap partials
-template <int WM>
-__global__ __launch_bounds__(kWG) void kb_spmv(const SysDesc *__restrict__ desc,
-                                               const ChunkItem *__restrict__ items,
-                                               const SysScal *__restrict__ sc, double *part) {
-  const ChunkItem c = items[blockIdx.x];
-  double h[3] = {0, 0, 0}, l[3] = {0, 0, 0};
-  if (!sc[c.sys].done) {
-    const SysDesc &D = desc[c.sys];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (!any) continue;  // uniform over the workgroup
     for (int s = c.s0 + w; s < c.s1; s += kWG >> 6) {
-      const double y = a_row<WM>(D, D.p, s, lane);
+      int cc[WM];
+      VT vv[WM];
+      sell_load<VT, WM>(D0.a_off, D0.a_col, av, s, lane, cc, vv);
       const int i = (s << 6) + lane;
-      if (i < D.n) {
-        D.Ap[i] = y;
-        const double pi = D.p[i];
-        dd_fma(h[0], l[0], pi, y);
-        dd_fma(h[1], l[1], pi, pi);
-        dd_fma(h[2], l[2], y, y);
+#pragma unroll
+      for (int q = 0; q < KS; ++q) {
+        if (!live[q]) continue;
+        const SysDesc &D = desc[g.slot[j0 + q]];
+        if (mode == 3) {
+          const double y = sell_apply<double, double, VT, WM, false>(cc, vv, D.p, 0.0);
+          if (i < D0.n) {
+            D.Ap[i] = y;
+            const double pi = D.p[i];
+            dd_fma(h[q][0], l[q][0], pi, y);
+            dd_fma(h[q][1], l[q][1], pi, pi);
+            dd_fma(h[q][2], l[q][2], y, y);
+          }
+        } else {
+          const double ax = sell_apply<double, double, VT, WM, false>(cc, vv, D.x, 0.0);
+          if (i < D0.n) {
+            const double bi = D.b[i], ri = bi - ax;
+            if (mode != 0) D.r[i] = ri;
+            dd_fma(h[q][0], l[q][0], bi, bi);
+            dd_fma(h[q][1], l[q][1], ri, ri);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < KS; ++q) {
+      if (!live[q]) continue;
+      if (mode == 3) {
+        block_partial<3>(h[q], l[q], plane(part, j0 + q, nch, 3), blockIdx.x);
+      } else {
+        double h2[2] = {h[q][0], h[q][1]}, l2[2] = {l[q][0], l[q][1]};
+        block_partial<2>(h2, l2, plane(part, j0 + q, nch, 2), blockIdx.x);
       }
     }
   }
-  block_partial<3>(h, l, part, blockIdx.x);
 }
 
-// x += alpha p; r -= alpha Ap; r.r partial
-__global__ __launch_bounds__(kWG) void kb_xr(const SysDesc *__restrict__ desc,
+template <int WM, int KM>
+__global__ __launch_bounds__(kWG) void kb_ax(const GroupDesc *__restrict__ groups,
+                                             const SysDesc *__restrict__ desc,
                                              const ChunkItem *__restrict__ items,
-                                             const SysScal *__restrict__ sc, double *part) {
+                                             const SysScal *__restrict__ sc, double *part,
+                                             int nch, int mode) {
   const ChunkItem c = items[blockIdx.x];
-  double h[1] = {0}, l[1] = {0};
-  if (!sc[c.sys].done) {
-    const SysDesc &D = desc[c.sys];
-    const double alpha = sc[c.sys].alpha;
+  const GroupDesc g = groups[c.grp];
+  if (mode == 3) {
+    // A group whose members all stopped is not touched: its matrix may
+    // already be freed (the running list keeps retired groups until it is
+    // rebuilt).
+    bool any = false;
+    for (int j = 0; j < g.k; ++j) any |= !sc[g.slot[j]].done;
+    if (!any) return;
+  }
+  const SysDesc &D0 = desc[g.slot[0]];
+  if (D0.a_v32)
+    ax_group<float, WM, KM>(g, desc, sc, c, D0.a_v32, part, nch, mode);
+  else
+    ax_group<double, WM, KM>(g, desc, sc, c, D0.a_v64, part, nch, mode);
+}
+
+// x += alpha p; r -= alpha Ap; r.r partial -- per member
+__global__ __launch_bounds__(kWG) void kb_xr(const GroupDesc *__restrict__ groups,
+                                             const SysDesc *__restrict__ desc,
+                                             const ChunkItem *__restrict__ items,
+                                             const SysScal *__restrict__ sc, double *part,
+                                             int nch) {
+  const ChunkItem c = items[blockIdx.x];
+  const GroupDesc g = groups[c.grp];
+  for (int j = 0; j < g.k; ++j) {
+    const int id = g.slot[j];
+    if (sc[id].done) continue;
+    const SysDesc &D = desc[id];
+    const double alpha = sc[id].alpha;
     const int end = min(D.n, c.s1 << 6);
+    double h[1] = {0}, l[1] = {0};
     for (int i = (c.s0 << 6) + threadIdx.x; i < end; i += kWG) {
       const double xi = fma(alpha, D.p[i], D.x[i]);
       const double ri = fma(-alpha, D.Ap[i], D.r[i]);
@@ -110,8 +158,8 @@ __global__ __launch_bounds__(kWG) void kb_xr(const SysDesc *__restrict__ desc,
       D.r[i] = ri;
       dd_fma(h[0], l[0], ri, ri);
     }
+    block_partial<1>(h, l, plane(part, j, nch, 1), blockIdx.x);
   }
-  block_partial<1>(h, l, part, blockIdx.x);
 }
 
 template <class FT>
@@ -121,20 +169,25 @@ __device__ __forceinline__ void rz_rows(const SysDesc &D, int beg, int end, doub
   for (int i = beg; i < end; i += kWG) dd_fma(h[0], l[0], D.r[i], (double)z[i]);
 }
 
-__global__ __launch_bounds__(kWG) void kb_rz(const SysDesc *__restrict__ desc,
+__global__ __launch_bounds__(kWG) void kb_rz(const GroupDesc *__restrict__ groups,
+                                             const SysDesc *__restrict__ desc,
                                              const ChunkItem *__restrict__ items,
-                                             const SysScal *__restrict__ sc, double *part) {
+                                             const SysScal *__restrict__ sc, double *part,
+                                             int nch) {
   const ChunkItem c = items[blockIdx.x];
-  double h[1] = {0}, l[1] = {0};
-  if (!sc[c.sys].done) {
-    const SysDesc &D = desc[c.sys];
+  const GroupDesc g = groups[c.grp];
+  for (int j = 0; j < g.k; ++j) {
+    const int id = g.slot[j];
+    if (sc[id].done) continue;
+    const SysDesc &D = desc[id];
     const int beg = (c.s0 << 6) + threadIdx.x, end = min(D.n, c.s1 << 6);
+    double h[1] = {0}, l[1] = {0};
     if (D.mixed)
       rz_rows<float>(D, beg, end, h, l);
     else
       rz_rows<double>(D, beg, end, h, l);
+    block_partial<1>(h, l, plane(part, j, nch, 1), blockIdx.x);
   }
-  block_partial<1>(h, l, part, blockIdx.x);
 }
 
 template <class FT>
@@ -147,118 +200,250 @@ __device__ __forceinline__ void p_rows(const SysDesc &D, int beg, int end, doubl
   }
 }
 
-__global__ __launch_bounds__(kWG) void kb_update_p(const SysDesc *__restrict__ desc,
+__global__ __launch_bounds__(kWG) void kb_update_p(const GroupDesc *__restrict__ groups,
+                                                   const SysDesc *__restrict__ desc,
                                                    const ChunkItem *__restrict__ items,
                                                    const SysScal *__restrict__ sc, int init) {
   const ChunkItem c = items[blockIdx.x];
-  if (sc[c.sys].done) return;
-  const SysDesc &D = desc[c.sys];
-  const double beta = sc[c.sys].beta;
-  const int beg = (c.s0 << 6) + threadIdx.x, end = min(D.n, c.s1 << 6);
-  if (D.mixed)
-    p_rows<float>(D, beg, end, beta, init);
-  else
-    p_rows<double>(D, beg, end, beta, init);
+  const GroupDesc g = groups[c.grp];
+  for (int j = 0; j < g.k; ++j) {
+    const int id = g.slot[j];
+    if (sc[id].done) continue;
+    const SysDesc &D = desc[id];
+    const double beta = sc[id].beta;
+    const int beg = (c.s0 << 6) + threadIdx.x, end = min(D.n, c.s1 << 6);
+    if (D.mixed)
+      p_rows<float>(D, beg, end, beta, init);
+    else
+      p_rows<double>(D, beg, end, beta, init);
+  }
 }
 
 // ---- triangular sweeps (ic0.cpp:194-249 multicolor, 263-339 ABMC) --------
-template <class FT, int WM>
-__device__ __forceinline__ void mc_fwd_item(const SysDesc &D, const ColorItem &c, int lane) {
-  FT *y = (FT *)D.y;
-  const FT *lv = (const FT *)D.l_v, *dg = (const FT *)D.d;
+// multicolor: the row's factor entries are loaded once, then every member's
+// chain runs on them
+template <class FT, int WM, int KM, bool FWD>
+__device__ __forceinline__ void mc_item(const GroupDesc &g, const SysDesc *__restrict__ desc,
+                                        const bool (&live)[KM], const ColorItem &c, int lane) {
+  const SysDesc &D0 = desc[g.slot[0]];
+  const int *off = FWD ? D0.l_off : D0.u_off;
+  const int *col = FWD ? D0.l_col : D0.u_col;
+  const FT *fv = (const FT *)(FWD ? D0.l_v : D0.u_v);
+  const FT *dg = (const FT *)D0.d;
   for (int s = c.a0; s < c.a1; ++s) {
     const int i = (s << 6) + lane;
     if (i < c.lo || i >= c.hi) continue;
-    FT acc = (FT)D.r[i];
-    acc = sell_chain<FT, FT, FT, WM, true>(D.l_off, D.l_col, lv, y, s, lane, acc);
-    y[i] = acc / dg[i];
+    int cc[WM];
+    FT vv[WM];
+    sell_load<FT, WM>(off, col, fv, s, lane, cc, vv);
+    const FT d = dg[i];
+#pragma unroll
+    for (int j = 0; j < KM; ++j) {
+      if (!live[j]) continue;
+      const SysDesc &D = desc[g.slot[j]];
+      if (FWD) {
+        FT *y = (FT *)D.y;
+        y[i] = sell_apply<FT, FT, FT, WM, true>(cc, vv, y, (FT)D.r[i]) / d;
+      } else {
+        FT *z = (FT *)D.z;
+        z[i] = sell_apply<FT, FT, FT, WM, true>(cc, vv, z, ((const FT *)D.y)[i]) / d;
+      }
+    }
   }
 }
 
-template <class FT, int WM>
-__device__ __forceinline__ void mc_bwd_item(const SysDesc &D, const ColorItem &c, int lane) {
-  FT *z = (FT *)D.z;
-  const FT *y = (const FT *)D.y, *uv = (const FT *)D.u_v, *dg = (const FT *)D.d;
-  for (int s = c.a0; s < c.a1; ++s) {
-    const int i = (s << 6) + lane;
-    if (i < c.lo || i >= c.hi) continue;
-    FT acc = y[i];
-    acc = sell_chain<FT, FT, FT, WM, true>(D.u_off, D.u_col, uv, z, s, lane, acc);
-    z[i] = acc / dg[i];
+// ABMC blocks of at most 64 rows (contiguous new ids [bs, be), lane t owns
+// row bs + t), all live members in lock-step: the block's factor entries
+// are loaded once, and every in-block level step advances each member's
+// chain, so the level latency is paid once for the group.  Per member and
+// row the operations and their order are abmc_fwd_block's /
+// abmc_bwd_block's: forward, the terms of earlier colors (columns < bs)
+// first, then the in-block terms as their levels complete; backward, the
+// in-block terms first, then the later colors' (columns >= be, prefetched).
+// my holds one 64-entry LDS window per member.
+template <class FT, int WM, int KM>
+__device__ __forceinline__ void abmc_fwd_group(int bs, int be, int nl, const SysDesc &D0,
+                                               const GroupDesc &g,
+                                               const SysDesc *__restrict__ desc,
+                                               const bool (&live)[KM], FT *my, int lane) {
+  const int i = bs + lane;
+  const bool act = i < be;
+  int cc[WM];
+  FT vv[WM];
+  FT acc[KM], d = (FT)1;
+  int mylev = -1;
+#pragma unroll
+  for (int j = 0; j < KM; ++j) acc[j] = (FT)0;
+  if (act) {
+    sell_load<FT, WM>(D0.l_off, D0.l_col, (const FT *)D0.l_v, i >> 6, i & 63, cc, vv);
+    mylev = D0.lev_f[i];
+    d = ((const FT *)D0.d)[i];
+#pragma unroll
+    for (int j = 0; j < KM; ++j) {
+      if (!live[j]) continue;
+      const SysDesc &D = desc[g.slot[j]];
+      const FT *y = (const FT *)D.y;
+      FT a = (FT)D.r[i];
+      FT xo[WM];
+#pragma unroll
+      for (int k = 0; k < WM; ++k) xo[k] = (cc[k] >= 0 && cc[k] < bs) ? y[cc[k]] : (FT)0;
+#pragma unroll
+      for (int k = 0; k < WM; ++k)
+        if (cc[k] >= 0 && cc[k] < bs) a = fma_t(-vv[k], xo[k], a);
+      acc[j] = a;
+    }
+  }
+  FT yi[KM];
+#pragma unroll
+  for (int j = 0; j < KM; ++j) yi[j] = (FT)0;
+  for (int l = 0; l < nl; ++l) {
+    if (mylev == l) {
+#pragma unroll
+      for (int j = 0; j < KM; ++j) {
+        if (!live[j]) continue;
+        FT a = acc[j];
+        const FT *w = my + 64 * j;
+#pragma unroll
+        for (int k = 0; k < WM; ++k)
+          if (cc[k] >= bs) a = fma_t(-vv[k], w[cc[k] - bs], a);
+        yi[j] = a / d;
+        my[64 * j + lane] = yi[j];
+      }
+    }
+    wave_sync();
+  }
+  if (act) {
+#pragma unroll
+    for (int j = 0; j < KM; ++j)
+      if (live[j]) ((FT *)desc[g.slot[j]].y)[i] = yi[j];
   }
 }
 
-template <class FT, int WM>
-__device__ __forceinline__ void abmc_fwd_item(const SysDesc &D, const ColorItem &c, int lane,
-                                              FT *my) {
-  for (int kb = c.a0; kb < c.a1; ++kb)
-    abmc_fwd_block<FT, FT, WM>(D.bstart[kb], D.bstart[kb + 1], D.nlev_f[kb], D.lev_f, D.l_off,
-                               D.l_col, (const FT *)D.l_v, (const FT *)D.d, D.r, (FT *)D.y, my,
-                               lane);
+template <class FT, int WM, int KM>
+__device__ __forceinline__ void abmc_bwd_group(int bs, int be, int nl, const SysDesc &D0,
+                                               const GroupDesc &g,
+                                               const SysDesc *__restrict__ desc,
+                                               const bool (&live)[KM], FT *my, int lane) {
+  const int i = bs + lane;
+  const bool act = i < be;
+  int cc[WM];
+  FT vv[WM];
+  FT acc[KM], xo[KM][WM], d = (FT)1;
+  int mylev = -1;
+#pragma unroll
+  for (int j = 0; j < KM; ++j) {
+    acc[j] = (FT)0;
+#pragma unroll
+    for (int k = 0; k < WM; ++k) xo[j][k] = (FT)0;
+  }
+  if (act) {
+    sell_load<FT, WM>(D0.u_off, D0.u_col, (const FT *)D0.u_v, i >> 6, i & 63, cc, vv);
+    mylev = D0.lev_b[i];
+    d = ((const FT *)D0.d)[i];
+#pragma unroll
+    for (int j = 0; j < KM; ++j) {
+      if (!live[j]) continue;
+      const SysDesc &D = desc[g.slot[j]];
+      const FT *z = (const FT *)D.z;
+#pragma unroll
+      for (int k = 0; k < WM; ++k)
+        if (cc[k] >= be) xo[j][k] = z[cc[k]];
+      acc[j] = ((const FT *)D.y)[i];
+    }
+  }
+  FT zi[KM];
+#pragma unroll
+  for (int j = 0; j < KM; ++j) zi[j] = (FT)0;
+  for (int lv = 0; lv < nl; ++lv) {
+    if (mylev == lv) {
+#pragma unroll
+      for (int j = 0; j < KM; ++j) {
+        if (!live[j]) continue;
+        FT a = acc[j];
+        const FT *w = my + 64 * j;
+#pragma unroll
+        for (int k = 0; k < WM; ++k)
+          if (cc[k] >= 0 && cc[k] < be) a = fma_t(-vv[k], w[cc[k] - bs], a);
+#pragma unroll
+        for (int k = 0; k < WM; ++k)
+          if (cc[k] >= be) a = fma_t(-vv[k], xo[j][k], a);
+        zi[j] = a / d;
+        my[64 * j + lane] = zi[j];
+      }
+    }
+    wave_sync();
+  }
+  if (act) {
+#pragma unroll
+    for (int j = 0; j < KM; ++j)
+      if (live[j]) ((FT *)desc[g.slot[j]].z)[i] = zi[j];
+  }
 }
 
-template <class FT, int WM>
-__device__ __forceinline__ void abmc_bwd_item(const SysDesc &D, const ColorItem &c, int lane,
-                                              FT *my) {
-  double ri;
-  for (int kb = c.a0; kb < c.a1; ++kb)
-    (void)abmc_bwd_block<FT, FT, WM>(D.bstart[kb], D.bstart[kb + 1], D.nlev_b[kb], D.lev_b,
-                                     D.u_off, D.u_col, (const FT *)D.u_v, (const FT *)D.d,
-                                     (const FT *)D.y, (FT *)D.z, nullptr, ri, my, lane);
+template <class FT, int WM, int KM, bool FWD>
+__device__ __forceinline__ void abmc_item(const GroupDesc &g, const SysDesc *__restrict__ desc,
+                                          const bool (&live)[KM], const ColorItem &c, int lane,
+                                          FT *my) {
+  const SysDesc &D0 = desc[g.slot[0]];
+  for (int kb = c.a0; kb < c.a1; ++kb) {
+    const int bs = D0.bstart[kb], be = D0.bstart[kb + 1];
+    if (FWD)
+      abmc_fwd_group<FT, WM, KM>(bs, be, D0.nlev_f[kb], D0, g, desc, live, my, lane);
+    else
+      abmc_bwd_group<FT, WM, KM>(bs, be, D0.nlev_b[kb], D0, g, desc, live, my, lane);
+  }
 }
 
 // one wavefront per color item; every branch below is uniform per wavefront
-template <int WM, bool FWD>
-__global__ __launch_bounds__(kWG) void kb_sweep(const SysDesc *__restrict__ desc,
+template <int WM, int KM, bool FWD>
+__global__ __launch_bounds__(kWG) void kb_sweep(const GroupDesc *__restrict__ groups,
+                                                const SysDesc *__restrict__ desc,
                                                 const ColorItem *__restrict__ items, int nitems,
                                                 const SysScal *__restrict__ sc) {
-  __shared__ double lds[kWG];
+  __shared__ double lds[kWG * KM];  // per wavefront: KM windows of 64
   const int wv = blockIdx.x * (kWG >> 6) + (threadIdx.x >> 6);
   if (wv >= nitems) return;
   const ColorItem c = items[wv];
-  if (sc[c.sys].done) return;
-  const SysDesc &D = desc[c.sys];
+  const GroupDesc g = groups[c.grp];
+  bool live[KM], any = false;
+#pragma unroll
+  for (int j = 0; j < KM; ++j) {
+    live[j] = j < g.k && !sc[g.slot[j]].done;
+    any |= live[j];
+  }
+  if (!any) return;
+  const SysDesc &D0 = desc[g.slot[0]];
   const int lane = threadIdx.x & 63;
-  double *my = lds + ((threadIdx.x >> 6) << 6);
-  if (D.bstart) {
-    if (D.mixed) {
-      if (FWD) abmc_fwd_item<float, WM>(D, c, lane, (float *)my);
-      else abmc_bwd_item<float, WM>(D, c, lane, (float *)my);
-    } else {
-      if (FWD) abmc_fwd_item<double, WM>(D, c, lane, my);
-      else abmc_bwd_item<double, WM>(D, c, lane, my);
-    }
+  double *my = lds + (threadIdx.x >> 6) * 64 * KM;
+  if (D0.bstart) {
+    if (D0.mixed)
+      abmc_item<float, WM, KM, FWD>(g, desc, live, c, lane, (float *)my);
+    else
+      abmc_item<double, WM, KM, FWD>(g, desc, live, c, lane, my);
   } else {
-    if (D.mixed) {
-      if (FWD) mc_fwd_item<float, WM>(D, c, lane);
-      else mc_bwd_item<float, WM>(D, c, lane);
-    } else {
-      if (FWD) mc_fwd_item<double, WM>(D, c, lane);
-      else mc_bwd_item<double, WM>(D, c, lane);
-    }
+    if (D0.mixed)
+      mc_item<float, WM, KM, FWD>(g, desc, live, c, lane);
+    else
+      mc_item<double, WM, KM, FWD>(g, desc, live, c, lane);
   }
 }
 
-// finish kernels: workgroup k folds the partials of the k-th listed system
-// (chunks [cptr[k], cptr[k+1])) and applies the single-system scalar logic
-// (cg_crs.cpp:104-160)
+// finish kernels: workgroup k folds the partials of fin[k] and applies the
+// single-system scalar logic (cg_crs.cpp:104-160)
 template <int NC>
-__device__ __forceinline__ bool fold_system(const int *cptr, const double *part,
-                                            double (&out)[NC]) {
-  const int k = blockIdx.x;
-  const int c0 = cptr[k], c1 = cptr[k + 1];
-  return finish_reduce<NC>(part + (size_t)c0 * NC * 2, c1 - c0, out);
+__device__ __forceinline__ bool fold_slot(const FinItem &f, const double *part, int nch,
+                                          double (&out)[NC]) {
+  return finish_reduce<NC>(part + ((size_t)f.plane * nch + f.c0) * NC * 2, f.c1 - f.c0, out);
 }
 
-__global__ __launch_bounds__(kWG) void kbf_resid(const int *__restrict__ act,
-                                                 const int *__restrict__ cptr,
-                                                 const double *__restrict__ part,
+__global__ __launch_bounds__(kWG) void kbf_resid(const FinItem *__restrict__ fin,
+                                                 const double *__restrict__ part, int nch,
                                                  const SysDesc *__restrict__ desc, SysScal *sc,
                                                  int mode) {
+  const FinItem f = fin[blockIdx.x];
   double out[2];
-  if (!fold_system<2>(cptr, part, out)) return;
-  const int id = act[blockIdx.x];
-  SysScal &S = sc[id];
+  if (!fold_slot<2>(f, part, nch, out)) return;
+  SysScal &S = sc[f.slot];
   double normb = sqrt(out[0]);
   if (normb == 0.0) normb = 1.0;
   S.normb = normb;
@@ -269,8 +454,8 @@ __global__ __launch_bounds__(kWG) void kbf_resid(const int *__restrict__ act,
   const double rel = sqrt(out[1]) / normb;
   if (mode == 1) {
     S.iters = 0;
-    S.tol = desc[id].tol;
-    S.max_iter = desc[id].max_iter;
+    S.tol = desc[f.slot].tol;
+    S.max_iter = desc[f.slot].max_iter;
   }
   S.rel = rel;
   S.converged = 0;
@@ -284,13 +469,14 @@ __global__ __launch_bounds__(kWG) void kbf_resid(const int *__restrict__ act,
   }
 }
 
-__global__ __launch_bounds__(kWG) void kbf_alpha(const int *__restrict__ act,
-                                                 const int *__restrict__ cptr,
-                                                 const double *__restrict__ part, SysScal *sc) {
-  SysScal &S = sc[act[blockIdx.x]];
+__global__ __launch_bounds__(kWG) void kbf_alpha(const FinItem *__restrict__ fin,
+                                                 const double *__restrict__ part, int nch,
+                                                 SysScal *sc) {
+  const FinItem f = fin[blockIdx.x];
+  SysScal &S = sc[f.slot];
   if (S.done) return;
   double out[3];
-  if (!fold_system<3>(cptr, part, out)) return;
+  if (!fold_slot<3>(f, part, nch, out)) return;
   const double pAp = out[0];
   const double thr = DBL_EPSILON * sqrt(out[1]) * sqrt(out[2]);
   S.pAp = pAp;
@@ -302,13 +488,14 @@ __global__ __launch_bounds__(kWG) void kbf_alpha(const int *__restrict__ act,
   }
 }
 
-__global__ __launch_bounds__(kWG) void kbf_xr(const int *__restrict__ act,
-                                              const int *__restrict__ cptr,
-                                              const double *__restrict__ part, SysScal *sc) {
-  SysScal &S = sc[act[blockIdx.x]];
+__global__ __launch_bounds__(kWG) void kbf_xr(const FinItem *__restrict__ fin,
+                                              const double *__restrict__ part, int nch,
+                                              SysScal *sc) {
+  const FinItem f = fin[blockIdx.x];
+  SysScal &S = sc[f.slot];
   if (S.done) return;
   double out[1];
-  if (!fold_system<1>(cptr, part, out)) return;
+  if (!fold_slot<1>(f, part, nch, out)) return;
   const double rel = sqrt(out[0]) / S.normb;
   const int it = S.iters + 1;
   S.iters = it;
@@ -321,14 +508,14 @@ __global__ __launch_bounds__(kWG) void kbf_xr(const int *__restrict__ act,
   }
 }
 
-__global__ __launch_bounds__(kWG) void kbf_rz(const int *__restrict__ act,
-                                              const int *__restrict__ cptr,
-                                              const double *__restrict__ part, SysScal *sc,
-                                              int init) {
-  SysScal &S = sc[act[blockIdx.x]];
+__global__ __launch_bounds__(kWG) void kbf_rz(const FinItem *__restrict__ fin,
+                                              const double *__restrict__ part, int nch,
+                                              SysScal *sc, int init) {
+  const FinItem f = fin[blockIdx.x];
+  SysScal &S = sc[f.slot];
   if (S.done) return;
   double out[1];
-  if (!fold_system<1>(cptr, part, out)) return;
+  if (!fold_slot<1>(f, part, nch, out)) return;
   if (init) {
     S.rz_old = out[0];
   } else {
@@ -349,15 +536,16 @@ namespace {
 constexpr size_t kAlign = 256;
 inline size_t align_up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
 
-// Item lists of a set of systems, resident on the device in one buffer.
+// Item lists of a set of slots, resident on the device in one buffer.
 struct ItemSet {
-  int nsys = 0, nchunks = 0, ncolors = 0;
+  int ngroups = 0, nchunks = 0, nfin = 0, ncolors = 0, kmax = 1;
   std::vector<int> color_off;  // ncolors + 1 offsets into d_color
-  std::vector<int> chunks_of;  // per listed system
+  std::vector<int> work_of;    // per listed slot (fin order): its group's chunks
   char *buf = nullptr;
-  int *d_act = nullptr, *d_cptr = nullptr;
+  GroupDesc *d_groups = nullptr;
   ChunkItem *d_chunks = nullptr;
   ColorItem *d_color = nullptr;
+  FinItem *d_fin = nullptr;
   void clear() {
     dfree(buf);
     *this = ItemSet();
@@ -380,7 +568,7 @@ struct Slot {
   void *y = nullptr, *z = nullptr;
   int max_iter = 0, flags = 0, restarts = 0;
   double tol = 0;
-  int listed_chunks = 0;  // chunk items of this slot in the running set
+  int listed_work = 0;  // chunk items of this slot's group in the running set
   std::chrono::steady_clock::time_point t_add;
   iccg_cg_result res{};
 };
@@ -405,12 +593,27 @@ struct iccg_batch_s {
   int cw = 1;                 // slices (MC) or blocks (ABMC) per sweep wavefront
   bool autotune = true;
   long tuned_rows = 0;        // running rows at the last tuning
+  int max_group = 1;          // right-hand sides per group (see set_max_group)
   ItemSet running;            // items of the running set
   bool running_dirty = true;
-  int dead_chunks = 0;        // listed chunks of systems no longer running
+  long running_work = 0;      // sum of listed_work over the running set
+  long dead_work = 0;         // listed work of slots no longer running
   bool graphs = true;
   std::vector<std::pair<int, hipGraphExec_t>> execs;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // timing mode: HIP events around every launch group, kernels launched
+  // one by one (no graphs)
+  struct Mark {
+    int cls;
+    hipEvent_t a, b;
+  };
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  hipEvent_t open_ev = nullptr;
+  std::vector<Mark> marks;
+  double t_ms[4] = {0, 0, 0, 0};
+  long t_cnt[4] = {0, 0, 0, 0};
   ~iccg_batch_s() {
     (void)hipSetDevice(device);
     if (st) (void)hipStreamSynchronize(st);
@@ -427,6 +630,7 @@ struct iccg_batch_s {
     if (h_desc) (void)hipHostFree(h_desc);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     if (st) (void)hipStreamDestroy(st);
   }
   void drop_graphs() {
@@ -448,20 +652,43 @@ void ensure_part(iccg_batch_s &B, size_t slots) {
   B.drop_graphs();
 }
 
-// item lists for the given slots with chunk / sweep granularities ch, cw
+// Item lists for the given slots with chunk / sweep granularities ch, cw.
+// Slots of one matrix and preconditioner are grouped, kMaxRhs at most, in
+// slot order; fin lists the slots in the order of ids.
 void build_items(iccg_batch_s &B, const std::vector<int> &ids, ItemSet &I, int ch, int cw) {
   I.clear();
-  std::vector<int> act, cptr{0};
-  std::vector<ChunkItem> chunks;
-  std::vector<std::vector<ColorItem>> color;
+  std::vector<GroupDesc> groups;
+  std::map<std::pair<const void *, const void *>, int> open;  // (A, M) -> group with room
+  std::vector<std::pair<int, int>> where;                      // per id: (group, member)
   for (int id : ids) {
     const Slot &s = B.slots[id];
+    const auto key = std::make_pair((const void *)s.A, (const void *)s.M);
+    auto it = open.find(key);
+    int gi;
+    if (it == open.end() || groups[(size_t)it->second].k == B.max_group) {
+      gi = (int)groups.size();
+      GroupDesc g{};
+      groups.push_back(g);
+      open[key] = gi;
+    } else {
+      gi = it->second;
+    }
+    GroupDesc &g = groups[(size_t)gi];
+    where.push_back({gi, g.k});
+    g.slot[g.k++] = id;
+  }
+  std::vector<int> gchunk0, gchunks;
+  std::vector<ChunkItem> chunks;
+  std::vector<std::vector<ColorItem>> color;
+  int kmax = 1;
+  for (int gi = 0; gi < (int)groups.size(); ++gi) {
+    const GroupDesc &g = groups[(size_t)gi];
+    kmax = std::max(kmax, g.k);
+    const Slot &s = B.slots[g.slot[0]];
     const int ns = (s.n + 63) / 64;
-    act.push_back(id);
-    const int c_before = (int)chunks.size();
-    for (int a = 0; a < ns; a += ch) chunks.push_back({id, a, std::min(ns, a + ch), 0});
-    cptr.push_back((int)chunks.size());
-    I.chunks_of.push_back((int)chunks.size() - c_before);
+    gchunk0.push_back((int)chunks.size());
+    for (int a = 0; a < ns; a += ch) chunks.push_back({gi, a, std::min(ns, a + ch), 0});
+    gchunks.push_back((int)chunks.size() - gchunk0.back());
     const DevSched &sc = s.M->sc;
     const int nc = sc.nc;
     if ((int)color.size() < nc) color.resize(nc);
@@ -469,112 +696,178 @@ void build_items(iccg_batch_s &B, const std::vector<int> &ids, ItemSet &I, int c
       if (sc.kind == 2) {
         const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
         for (int kb = b0; kb < b1; kb += cw)
-          color[c].push_back({id, kb, std::min(b1, kb + cw), 0, 0, {0, 0, 0}});
+          color[c].push_back({gi, kb, std::min(b1, kb + cw), 0, 0, {0, 0, 0}});
       } else {
         const int lo = sc.h_color_ptr_v[c], hi = sc.h_color_ptr_v[c + 1];
         if (hi <= lo) continue;
         const int s1 = ((hi - 1) >> 6) + 1;
         for (int sl = lo >> 6; sl < s1; sl += cw)
-          color[c].push_back({id, sl, std::min(s1, sl + cw), lo, hi, {0, 0, 0}});
+          color[c].push_back({gi, sl, std::min(s1, sl + cw), lo, hi, {0, 0, 0}});
       }
     }
   }
-  I.nsys = (int)act.size();
+  std::vector<FinItem> fin;
+  for (size_t k = 0; k < ids.size(); ++k) {
+    const int gi = where[k].first;
+    fin.push_back({ids[k], gchunk0[(size_t)gi], gchunk0[(size_t)gi] + gchunks[(size_t)gi],
+                   where[k].second});
+    I.work_of.push_back(gchunks[(size_t)gi]);
+  }
+  I.ngroups = (int)groups.size();
   I.nchunks = (int)chunks.size();
+  I.nfin = (int)fin.size();
   I.ncolors = (int)color.size();
+  I.kmax = kmax;
   I.color_off.assign(1, 0);
   for (auto &v : color) I.color_off.push_back(I.color_off.back() + (int)v.size());
-  if (I.nsys == 0) return;
-  // one device buffer: act | cptr | chunks | color items
-  const size_t o_cptr = align_up(act.size() * sizeof(int));
-  const size_t o_chunks = o_cptr + align_up(cptr.size() * sizeof(int));
-  const size_t o_color = o_chunks + align_up(chunks.size() * sizeof(ChunkItem));
+  if (I.nfin == 0) return;
+  // one device buffer: groups | chunks | fin | color items
+  const size_t o_chunks = align_up(groups.size() * sizeof(GroupDesc));
+  const size_t o_fin = o_chunks + align_up(chunks.size() * sizeof(ChunkItem));
+  const size_t o_color = o_fin + align_up(fin.size() * sizeof(FinItem));
   const size_t total = o_color + (size_t)I.color_off.back() * sizeof(ColorItem);
   std::vector<char> h(total);
-  std::memcpy(h.data(), act.data(), act.size() * sizeof(int));
-  std::memcpy(h.data() + o_cptr, cptr.data(), cptr.size() * sizeof(int));
+  std::memcpy(h.data(), groups.data(), groups.size() * sizeof(GroupDesc));
   std::memcpy(h.data() + o_chunks, chunks.data(), chunks.size() * sizeof(ChunkItem));
+  std::memcpy(h.data() + o_fin, fin.data(), fin.size() * sizeof(FinItem));
   size_t o = o_color;
   for (auto &v : color) {
     if (!v.empty()) std::memcpy(h.data() + o, v.data(), v.size() * sizeof(ColorItem));
     o += v.size() * sizeof(ColorItem);
   }
-  ensure_part(B, (size_t)I.nchunks * 3);
+  ensure_part(B, (size_t)I.nchunks * 3 * (size_t)kmax);
   I.buf = dalloc<char>(total);
-  I.d_act = (int *)I.buf;
-  I.d_cptr = (int *)(I.buf + o_cptr);
+  I.d_groups = (GroupDesc *)I.buf;
   I.d_chunks = (ChunkItem *)(I.buf + o_chunks);
+  I.d_fin = (FinItem *)(I.buf + o_fin);
   I.d_color = (ColorItem *)(I.buf + o_color);
   HIPCHK(hipMemcpyAsync(I.buf, h.data(), total, hipMemcpyHostToDevice, B.st));
   HIPCHK(hipStreamSynchronize(B.st));
 }
 
-#define BATCH_WM(wm, K, ...)                                          \
-  do {                                                                \
-    if ((wm) <= 4) hipLaunchKernelGGL((K<4>), __VA_ARGS__);           \
-    else if ((wm) <= 8) hipLaunchKernelGGL((K<8>), __VA_ARGS__);      \
-    else hipLaunchKernelGGL((K<16>), __VA_ARGS__);                    \
-  } while (0)
+int km_class(int k) { return k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : 8; }
 
-#define BATCH_WM2(wm, K, F, ...)                                      \
-  do {                                                                \
-    if ((wm) <= 4) hipLaunchKernelGGL((K<4, F>), __VA_ARGS__);        \
-    else if ((wm) <= 8) hipLaunchKernelGGL((K<8, F>), __VA_ARGS__);   \
-    else hipLaunchKernelGGL((K<16, F>), __VA_ARGS__);                 \
+// K<WM, KM, ...extra> over the template widths of the set
+#define BATCH_DISPATCH(wm, km, K, EXTRA, ...)                                                  \
+  do {                                                                                        \
+    const int _w = (wm) <= 4 ? 4 : (wm) <= 8 ? 8 : 16, _k = km_class(km);                     \
+    if (_w == 4 && _k == 1) hipLaunchKernelGGL((K<4, 1 EXTRA>), __VA_ARGS__);                  \
+    else if (_w == 4 && _k == 2) hipLaunchKernelGGL((K<4, 2 EXTRA>), __VA_ARGS__);             \
+    else if (_w == 4 && _k == 4) hipLaunchKernelGGL((K<4, 4 EXTRA>), __VA_ARGS__);             \
+    else if (_w == 4) hipLaunchKernelGGL((K<4, 8 EXTRA>), __VA_ARGS__);                        \
+    else if (_w == 8 && _k == 1) hipLaunchKernelGGL((K<8, 1 EXTRA>), __VA_ARGS__);             \
+    else if (_w == 8 && _k == 2) hipLaunchKernelGGL((K<8, 2 EXTRA>), __VA_ARGS__);             \
+    else if (_w == 8 && _k == 4) hipLaunchKernelGGL((K<8, 4 EXTRA>), __VA_ARGS__);             \
+    else if (_w == 8) hipLaunchKernelGGL((K<8, 8 EXTRA>), __VA_ARGS__);                        \
+    else if (_k == 1) hipLaunchKernelGGL((K<16, 1 EXTRA>), __VA_ARGS__);                       \
+    else if (_k == 2) hipLaunchKernelGGL((K<16, 2 EXTRA>), __VA_ARGS__);                       \
+    else if (_k == 4) hipLaunchKernelGGL((K<16, 4 EXTRA>), __VA_ARGS__);                       \
+    else hipLaunchKernelGGL((K<16, 8 EXTRA>), __VA_ARGS__);                                    \
   } while (0)
+#define NOEXTRA
+#define FWD_T , true
+#define BWD_T , false
+
+// kernel classes of the timing mode
+enum { T_SPMV = 0, T_FWD = 1, T_BWD = 2, T_VEC = 3, T_NCLASS = 4 };
+
+void mark(iccg_batch_s &B, int cls, bool begin);
+
+void launch_ax(iccg_batch_s &B, const ItemSet &I, int mode) {
+  BATCH_DISPATCH(B.wm, I.kmax, kb_ax, NOEXTRA, dim3(I.nchunks), dim3(kWG), 0, B.st, I.d_groups,
+                 B.d_desc, I.d_chunks, B.d_sc, B.part, I.nchunks, mode);
+}
 
 void launch_resid_items(iccg_batch_s &B, const ItemSet &I, int mode) {
-  if (I.nsys == 0) return;
-  dim3 g(I.nchunks), b(kWG);
-  BATCH_WM(B.wm, kb_resid, g, b, 0, B.st, B.d_desc, I.d_chunks, B.part, mode);
-  hipLaunchKernelGGL(kbf_resid, dim3(I.nsys), b, 0, B.st, I.d_act, I.d_cptr, B.part, B.d_desc,
-                     B.d_sc, mode);
+  if (I.nfin == 0) return;
+  launch_ax(B, I, mode);
+  hipLaunchKernelGGL(kbf_resid, dim3(I.nfin), dim3(kWG), 0, B.st, I.d_fin, B.part, I.nchunks,
+                     B.d_desc, B.d_sc, mode);
 }
 
 void launch_sweeps(iccg_batch_s &B, const ItemSet &I) {
   dim3 b(kWG);
+  mark(B, T_FWD, true);
   for (int c = 0; c < I.ncolors; ++c) {
     const int ni = I.color_off[c + 1] - I.color_off[c];
     if (ni)
-      BATCH_WM2(B.wm, kb_sweep, true, dim3((ni + 3) / 4), b, 0, B.st, B.d_desc,
-                I.d_color + I.color_off[c], ni, B.d_sc);
+      BATCH_DISPATCH(B.wm, I.kmax, kb_sweep, FWD_T, dim3((ni + 3) / 4), b, 0, B.st, I.d_groups,
+                     B.d_desc, I.d_color + I.color_off[c], ni, B.d_sc);
   }
+  mark(B, T_FWD, false);
+  mark(B, T_BWD, true);
   for (int c = I.ncolors - 1; c >= 0; --c) {
     const int ni = I.color_off[c + 1] - I.color_off[c];
     if (ni)
-      BATCH_WM2(B.wm, kb_sweep, false, dim3((ni + 3) / 4), b, 0, B.st, B.d_desc,
-                I.d_color + I.color_off[c], ni, B.d_sc);
+      BATCH_DISPATCH(B.wm, I.kmax, kb_sweep, BWD_T, dim3((ni + 3) / 4), b, 0, B.st, I.d_groups,
+                     B.d_desc, I.d_color + I.color_off[c], ni, B.d_sc);
   }
+  mark(B, T_BWD, false);
 }
 
 void launch_rz_p(iccg_batch_s &B, const ItemSet &I, int init) {
   dim3 g(I.nchunks), b(kWG);
-  hipLaunchKernelGGL(kb_rz, g, b, 0, B.st, B.d_desc, I.d_chunks, B.d_sc, B.part);
-  hipLaunchKernelGGL(kbf_rz, dim3(I.nsys), b, 0, B.st, I.d_act, I.d_cptr, B.part, B.d_sc, init);
-  hipLaunchKernelGGL(kb_update_p, g, b, 0, B.st, B.d_desc, I.d_chunks, B.d_sc, init);
+  mark(B, T_VEC, true);
+  hipLaunchKernelGGL(kb_rz, g, b, 0, B.st, I.d_groups, B.d_desc, I.d_chunks, B.d_sc, B.part,
+                     I.nchunks);
+  hipLaunchKernelGGL(kbf_rz, dim3(I.nfin), b, 0, B.st, I.d_fin, B.part, I.nchunks, B.d_sc, init);
+  hipLaunchKernelGGL(kb_update_p, g, b, 0, B.st, I.d_groups, B.d_desc, I.d_chunks, B.d_sc, init);
+  mark(B, T_VEC, false);
 }
 
-// z0 = M^-1 r0, rz_old, p = z0 (cg_crs.cpp:112-115) for the listed systems
+// z0 = M^-1 r0, rz_old, p = z0 (cg_crs.cpp:112-115) for the listed slots
 void launch_precond_init(iccg_batch_s &B, const ItemSet &I) {
-  if (I.nsys == 0) return;
+  if (I.nfin == 0) return;
   launch_sweeps(B, I);
   launch_rz_p(B, I, 1);
 }
 
-// one CG iteration (cg_crs.cpp:125-160) of every listed system
+// one CG iteration (cg_crs.cpp:125-160) of every listed slot
 void launch_iteration(iccg_batch_s &B, const ItemSet &I) {
   dim3 g(I.nchunks), b(kWG);
-  BATCH_WM(B.wm, kb_spmv, g, b, 0, B.st, B.d_desc, I.d_chunks, B.d_sc, B.part);
-  hipLaunchKernelGGL(kbf_alpha, dim3(I.nsys), b, 0, B.st, I.d_act, I.d_cptr, B.part, B.d_sc);
-  hipLaunchKernelGGL(kb_xr, g, b, 0, B.st, B.d_desc, I.d_chunks, B.d_sc, B.part);
-  hipLaunchKernelGGL(kbf_xr, dim3(I.nsys), b, 0, B.st, I.d_act, I.d_cptr, B.part, B.d_sc);
+  mark(B, T_SPMV, true);
+  launch_ax(B, I, 3);
+  mark(B, T_SPMV, false);
+  mark(B, T_VEC, true);
+  hipLaunchKernelGGL(kbf_alpha, dim3(I.nfin), b, 0, B.st, I.d_fin, B.part, I.nchunks, B.d_sc);
+  hipLaunchKernelGGL(kb_xr, g, b, 0, B.st, I.d_groups, B.d_desc, I.d_chunks, B.d_sc, B.part,
+                     I.nchunks);
+  hipLaunchKernelGGL(kbf_xr, dim3(I.nfin), b, 0, B.st, I.d_fin, B.part, I.nchunks, B.d_sc);
+  mark(B, T_VEC, false);
   launch_sweeps(B, I);
   launch_rz_p(B, I, 0);
 }
 
+void mark(iccg_batch_s &B, int cls, bool begin) {
+  if (!B.timing) return;
+  if (B.ev_used == B.ev_pool.size()) {
+    hipEvent_t e = nullptr;
+    HIPCHK(hipEventCreate(&e));
+    B.ev_pool.push_back(e);
+  }
+  hipEvent_t e = B.ev_pool[B.ev_used++];
+  HIPCHK(hipEventRecord(e, B.st));
+  if (begin)
+    B.open_ev = e;
+  else
+    B.marks.push_back({cls, B.open_ev, e});
+}
+
+// after a stream synchronization: fold the recorded launch groups
+void harvest(iccg_batch_s &B) {
+  for (const auto &m : B.marks) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, m.a, m.b));
+    B.t_ms[m.cls] += ms;
+    B.t_cnt[m.cls]++;
+  }
+  B.marks.clear();
+  B.ev_used = 0;
+}
+
 void enqueue_iterations(iccg_batch_s &B, int k) {
   const ItemSet &I = B.running;
-  if (!B.graphs) {
+  if (!B.graphs || B.timing) {
     for (int j = 0; j < k; ++j) launch_iteration(B, I);
     return;
   }
@@ -598,11 +891,12 @@ void read_scalars(iccg_batch_s &B) {
   HIPCHK(hipMemcpyAsync(B.h_sc, B.d_sc, sizeof(SysScal) * (size_t)B.cap, hipMemcpyDeviceToHost,
                         B.st));
   HIPCHK(hipStreamSynchronize(B.st));
+  if (B.timing) harvest(B);
 }
 
 void retire_listed(iccg_batch_s &B, Slot &s) {
-  B.dead_chunks += s.listed_chunks;
-  s.listed_chunks = 0;
+  B.dead_work += s.listed_work;
+  s.listed_work = 0;
 }
 
 void finish_slot(iccg_batch_s &B, int id) {
@@ -720,12 +1014,14 @@ void tune(iccg_batch_s &B, const std::vector<int> &ids) {
   double best = 1e300;
   int best_ch = B.chunk;
   ItemSet T;
+  const bool timing = B.timing;
+  B.timing = false;  // the probes are not counted
   for (int ch : chunks) {
     build_items(B, ids, T, ch, B.cw);
     const double ms = time_ms(B, [&] {
-      dim3 g(T.nchunks), b(kWG);
-      BATCH_WM(B.wm, kb_spmv, g, b, 0, B.st, B.d_desc, T.d_chunks, B.d_sc, B.part);
-      hipLaunchKernelGGL(kb_rz, g, b, 0, B.st, B.d_desc, T.d_chunks, B.d_sc, B.part);
+      launch_ax(B, T, 3);
+      hipLaunchKernelGGL(kb_rz, dim3(T.nchunks), dim3(kWG), 0, B.st, T.d_groups, B.d_desc,
+                         T.d_chunks, B.d_sc, B.part, T.nchunks);
     }, 3);
     if (ms < best) {
       best = ms;
@@ -746,10 +1042,11 @@ void tune(iccg_batch_s &B, const std::vector<int> &ids) {
   T.clear();
   B.chunk = best_ch;
   B.cw = best_cw;
+  B.timing = timing;
 }
 
 void refresh_running(iccg_batch_s &B) {
-  if (!B.running_dirty && B.dead_chunks * 4 <= B.running.nchunks) return;
+  if (!B.running_dirty && B.dead_work * 4 <= B.running_work) return;
   std::vector<int> ids;
   long rows = 0;
   for (int id = 0; id < B.cap; ++id)
@@ -763,11 +1060,15 @@ void refresh_running(iccg_batch_s &B) {
     tune(B, ids);
     B.tuned_rows = rows;
   }
-  for (Slot &s : B.slots) s.listed_chunks = 0;
+  for (Slot &s : B.slots) s.listed_work = 0;
   build_items(B, ids, B.running, B.chunk, B.cw);
-  for (size_t k = 0; k < ids.size(); ++k) B.slots[ids[k]].listed_chunks = B.running.chunks_of[k];
+  B.running_work = 0;
+  for (size_t k = 0; k < ids.size(); ++k) {
+    B.slots[ids[k]].listed_work = B.running.work_of[k];
+    B.running_work += B.running.work_of[k];
+  }
   B.running_dirty = false;
-  B.dead_chunks = 0;
+  B.dead_work = 0;
 }
 
 }  // namespace
@@ -865,7 +1166,7 @@ int iccg_batch_add(iccg_batch_t B, iccg_csr_t A, iccg_precond_t M, const double 
     s.tol = tol;
     s.flags = flags;
     s.restarts = 0;
-    s.listed_chunks = 0;
+    s.listed_work = 0;
     s.res = iccg_cg_result{};
     s.res.true_rel_resid = -1.0;
     SysDesc &D = B->h_desc[id];
@@ -996,6 +1297,39 @@ int iccg_batch_set_graphs(iccg_batch_t B, int on) {
   return guard([&] {
     require(B, ICCG_ERR_ARG, "null argument");
     B->graphs = on != 0;
+  });
+}
+
+int iccg_batch_set_timing(iccg_batch_t B, int on) {
+  return guard([&] {
+    require(B, ICCG_ERR_ARG, "null argument");
+    B->timing = on != 0;
+  });
+}
+
+int iccg_batch_get_timing(iccg_batch_t B, int cls, long *launches, double *total_ms) {
+  return guard([&] {
+    require(B && cls >= 0 && cls < 4 && launches && total_ms, ICCG_ERR_ARG, "bad argument");
+    *launches = B->t_cnt[cls];
+    *total_ms = B->t_ms[cls];
+  });
+}
+
+int iccg_batch_reset_timing(iccg_batch_t B) {
+  return guard([&] {
+    require(B, ICCG_ERR_ARG, "null argument");
+    for (int c = 0; c < 4; ++c) {
+      B->t_ms[c] = 0;
+      B->t_cnt[c] = 0;
+    }
+  });
+}
+
+int iccg_batch_set_max_group(iccg_batch_t B, int k) {
+  return guard([&] {
+    require(B && k >= 1 && k <= kMaxRhs, ICCG_ERR_ARG, "group size must be in [1, 8]");
+    B->max_group = k;
+    B->running_dirty = true;
   });
 }
 

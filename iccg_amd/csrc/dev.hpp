@@ -146,14 +146,25 @@ struct alignas(64) SysScal {
   int iters, done, converged, breakdown, max_iter, pad[3];
 };
 
-struct ChunkItem {  // rows [64*s0, min(64*s1, n)) of slot sys: one workgroup
-  int sys, s0, s1, pad;
+// Right-hand sides of one matrix and preconditioner run as a group: the
+// kernels read each matrix entry once for all of them.
+constexpr int kMaxRhs = 8;
+struct GroupDesc {
+  int k, pad;
+  int slot[kMaxRhs];  // slot[0] also names the shared matrix and factor
 };
-// One wavefront of a sweep over one color of slot sys:
+struct ChunkItem {  // rows [64*s0, min(64*s1, n)) of group grp: one workgroup
+  int grp, s0, s1, pad;
+};
+// One wavefront of a sweep over one color of group grp:
 //   multicolor: 64-row slices [a0, a1), rows restricted to [lo, hi);
 //   ABMC      : blocks [a0, a1).
 struct ColorItem {
-  int sys, a0, a1, lo, hi, pad[3];
+  int grp, a0, a1, lo, hi, pad[3];
+};
+// Finish work of one slot: its partials are chunks [c0, c1) of plane `plane`.
+struct FinItem {
+  int slot, c0, c1, plane;
 };
 
 // ---- launchers (kernels.hip) ----------------------------------------------

@@ -59,6 +59,7 @@ __device__ void block_partial(double (&h)[NC], double (&l)[NC], double *part, in
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nw = blockDim.x >> 6;
   wave_dd_reduce<NC>(h, l);
+  __syncthreads();  // a previous call's reads of sh are complete
   if (lane == 0) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
@@ -185,6 +186,39 @@ __device__ __forceinline__ void st4(double *p, int i, const double (&o)[4]) {
 // ---------------------------------------------------------------------------
 // Sliced-ELL row chains
 // ---------------------------------------------------------------------------
+// The lane's row of slice s as registers: columns (padding -1) and values.
+template <class VT, int WM>
+__device__ __forceinline__ void sell_load(const int *__restrict__ off, const int *__restrict__ col,
+                                          const VT *__restrict__ v, int s, int lane,
+                                          int (&cc)[WM], VT (&vv)[WM]) {
+  const int o0 = off[s];
+  const int w = (off[s + 1] - o0) >> 6;
+  const int base = o0 + lane;
+#pragma unroll
+  for (int k = 0; k < WM; ++k) {
+    cc[k] = -1;
+    vv[k] = (VT)0;
+    if (k < w) {
+      cc[k] = col[base + 64 * k];
+      vv[k] = v[base + 64 * k];
+    }
+  }
+}
+
+// acc op= sum_k vv_k * x[cc_k] in ascending k over loaded entries: the same
+// operations, in the same order, as sell_chain
+template <class AT, class XT, class VT, int WM, bool NEG>
+__device__ __forceinline__ AT sell_apply(const int (&cc)[WM], const VT (&vv)[WM],
+                                         const XT *__restrict__ x, AT acc) {
+  AT xv[WM];
+#pragma unroll
+  for (int k = 0; k < WM; ++k) xv[k] = cc[k] >= 0 ? (AT)x[cc[k]] : (AT)0;
+#pragma unroll
+  for (int k = 0; k < WM; ++k)
+    if (cc[k] >= 0) acc = NEG ? fma_t(-(AT)vv[k], xv[k], acc) : fma_t((AT)vv[k], xv[k], acc);
+  return acc;
+}
+
 // acc op= sum_k v_k * x[c_k] over the entries of the lane's row in slice s,
 // ascending k; NEG selects acc - v*x (triangular solves) over acc + v*x.
 template <class VT, class AT, class XT, int WM, bool NEG>

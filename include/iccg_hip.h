@@ -255,7 +255,9 @@ int iccg_solver_reset_timing(iccg_solver_t S);
 /* ---- many systems at once (no reference counterpart) -----------------------
  * Independent systems, each with its own matrix, IC0 preconditioner and
  * vectors resident in HBM, advanced together: every CG phase is one launch
- * over all running systems, each with its own alpha/beta/stop test.  The
+ * over all running systems, each with its own alpha/beta/stop test.  Slots
+ * that share a matrix and preconditioner (several right-hand sides) can run
+ * as groups (iccg_batch_set_max_group).  The
  * preconditioner needs a sliced plan: a contiguous multicolor schedule (e.g.
  * from iccg_greedy_color) or an ABMC schedule of contiguous blocks of at most
  * 64 rows; rows of A, L and L^T hold at most 16 entries.  Systems may differ
@@ -289,6 +291,20 @@ int iccg_batch_set_graphs(iccg_batch_t B, int on);
 int iccg_batch_set_autotune(iccg_batch_t B, int on);
 int iccg_batch_get_tuning(iccg_batch_t B, int *chunk_slices, int *sweep_units);
 int iccg_batch_set_tuning(iccg_batch_t B, int chunk_slices, int sweep_units);
+/* Timing mode: kernels launched one by one (no graphs) with HIP events on
+ * the batch's stream around every launch group.  Classes: 0 SpMV (+dots),
+ * 1 forward sweeps, 2 backward sweeps, 3 vector updates and finish kernels;
+ * launches = launch groups, total_ms summed since the last reset. */
+int iccg_batch_set_timing(iccg_batch_t B, int on);
+int iccg_batch_get_timing(iccg_batch_t B, int cls, long *launches, double *total_ms);
+int iccg_batch_reset_timing(iccg_batch_t B);
+/* Right-hand sides per group of slots sharing a matrix, 1..8 (default 1:
+ * every right-hand side its own work items).  Grouped members read each
+ * matrix and factor entry once and step through the ABMC in-block levels
+ * together; on gfx950 this measured no faster than separate items (the
+ * kernels are bound by per-right-hand-side gathers and occupancy, not by
+ * matrix bytes), so it is off by default.  Results do not depend on it. */
+int iccg_batch_set_max_group(iccg_batch_t B, int k);
 /* Frees the device vectors that free slots keep for reuse (iccg_batch_remove
  * keeps them, so a slot refilled with a system no larger allocates nothing). */
 int iccg_batch_trim(iccg_batch_t B);

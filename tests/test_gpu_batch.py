@@ -149,3 +149,65 @@ def test_batch_rejects_unsupported():
     B.add(o.A, I.IC0(o.A, 0.0, sched=o.sched), np.ones(A.n))
     with pytest.raises(I.IccgError, match="full"):
         B.add(o.A, I.IC0(o.A, 0.0, sched=o.sched), np.ones(A.n))
+
+
+@pytest.mark.parametrize("kind,precision", [("mc", "mixed"), ("abmc", "mixed"), ("abmc", "fp64")])
+def test_batch_shared_matrix_groups(kind, precision):
+    """Right-hand sides of one matrix run as groups (up to 8 per group, here
+    11 right-hand sides = 8 + 3) that read the matrix once; every one gives
+    its solo iterates.  A second system with 2 right-hand sides joins
+    mid-run."""
+    dims = (40, 36, 1) if kind == "mc" else (12, 12, 12)
+    o, _ = _system(*dims, 700, kind=kind)
+    M = I.IC0(o.A, 0.0, sched=o.sched, precision=precision)
+    rhs = [I.gen_normal(o.A.n, 800 + k) for k in range(11)]
+    o2, _ = _system(30, 30, 1, 701, kind="abmc")
+    M2 = I.IC0(o2.A, 0.0, sched=o2.sched, precision=precision)
+    rhs2 = [I.gen_normal(o2.A.n, 900 + k) for k in range(2)]
+    B = I.Batch(16)
+    B.set_max_group(8)
+    slots = [B.add(o.A, M, b) for b in rhs]
+    B.run(max_steps=6)
+    slots2 = [B.add(o2.A, M2, b) for b in rhs2]
+    while B.run(max_steps=9):
+        pass
+    _check_same(B, slots, [_alone(o, b, precision) for b in rhs])
+    _check_same(B, slots2, [_alone(o2, b, precision) for b in rhs2])
+
+
+def test_batch_timing_mode_counts_and_keeps_results():
+    systems = [_system(n, n, 1, 950 + n) for n in (48, 64)]
+    alone = [_alone(o, b, "mixed") for o, b in systems]
+    B = I.Batch(4)
+    B.set_timing(True)
+    slots = [B.add(o.A, I.IC0(o.A, 0.0, sched=o.sched, precision="mixed"), b)
+             for o, b in systems]
+    assert B.run() == 0
+    _check_same(B, slots, alone)
+    iters = max(B.result(s)[1].iters for s in slots)
+    n_spmv, ms_spmv = B.timing(0)
+    assert n_spmv >= iters and ms_spmv > 0
+    assert all(B.timing(c)[0] > 0 for c in range(4))
+    B.reset_timing()
+    assert B.timing(0) == (0, 0.0)
+
+
+def test_batch_retired_system_may_be_freed_while_others_run():
+    """A finished and removed system's matrix and preconditioner can be
+    destroyed at once while the others keep running: retired items left in
+    the running list must not touch them.  The retired system is small, so
+    the running list is not rebuilt when it leaves."""
+    big = [_system(n, n, 1, 1000 + n) for n in (96, 100, 104)]
+    small_o, small_b = _system(12, 12, 1, 1099)
+    Ms = [I.IC0(o.A, 0.0, sched=o.sched, precision="mixed") for o, _ in big]
+    dA_small = I.DeviceCSR(small_o.A)
+    M_small = I.IC0(dA_small, 0.0, sched=small_o.sched, precision="mixed")
+    B = I.Batch(4)
+    slots = [B.add(o.A, M, b) for (o, b), M in zip(big, Ms)]
+    s_small = B.add(dA_small, M_small, small_b)
+    while not B.result(s_small)[0]:
+        B.run(max_steps=2)
+    B.remove(s_small)
+    del M_small, dA_small  # destroyed now (the batch keeps no reference)
+    assert B.run() == 0
+    _check_same(B, slots, [_alone(o, b, "mixed") for o, b in big])
