@@ -210,6 +210,43 @@ def cpu_baseline_systems(systems, tol, budget_s, label):
                        f"threads), FP64, tol {tol:g}; CG phase timed")}
 
 
+def batch_kernel_table(B, systems, res, count, max_iter, tol, M32, dA):
+    """One more batched solve of every system with the batch's timing mode
+    (kernels launched one by one, HIP events on the batch's stream around
+    every launch group).  Per class: summed event time and the algorithmic
+    bytes of every system over the iterations it ran (FP32 A values, FP32
+    factor and y/z), hence GB/s over the whole batched solve."""
+    for s in range(count):
+        B.remove(s)
+    B.set_timing(True)
+    B.reset_timing()
+    slots = [B.add(dA[k], M32[k], systems[k][2], max_iter=max_iter, tol=tol)
+             for k in range(count)]
+    while B.run():
+        pass
+    B.set_timing(False)
+    it = [B.result(s)[1].iters for s in slots]
+    fwd = bwd = spmv = vec = 0
+    for (A, _, _), k in zip(systems, it):
+        n, nnz = A.n, A.nnz
+        nnz_l = (nnz + n) // 2
+        spmv += k * spmv_bytes(n, nnz, s=4)
+        fwd += k * trsv_bytes(n, nnz_l, 4, 8, 4)
+        bwd += k * trsv_bytes(n, nnz_l, 4, 4, 4)
+        vec += k * 80 * n
+    out = {}
+    for c, (nm, byt) in enumerate([("spmv_dots", spmv), ("fwd_sweep", fwd), ("bwd_sweep", bwd),
+                                   ("vector_ops", vec)]):
+        cnt, ms = B.timing(c)
+        gbs = byt / (ms * 1e-3) / 1e9 if ms > 0 else None
+        out[nm] = {"launch_groups": cnt, "total_ms": round(ms, 3), "bytes": int(byt),
+                   "gbs": round(gbs, 1) if gbs else None,
+                   "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None}
+    log("batched kernel classes: " + ", ".join(f"{k} {v['total_ms']} ms {v['gbs']} GB/s"
+                                              for k, v in out.items()))
+    return out
+
+
 def main_batch(args):
     """Many-system workload: one Batch solving every system (mixed precision,
     stop on the FP64 true residual) against the plain FP64 single-system
@@ -249,6 +286,7 @@ def main_batch(args):
     wall = time.perf_counter() - t0
     xs = [B.x(s) for s in slots]
     chunk, cw = B.tuning
+    kernels = batch_kernel_table(B, systems, res, count, max_iter, tol, M32, dA)
     sps = count * args.steps / wall
     iters = np.array([r.iters for r in res])
     trel = np.array([r.true_rel_resid for r in res])
@@ -311,6 +349,7 @@ def main_batch(args):
         "fp64_max_true_rel_resid": float(tr64.max()),
         "max_err_vs_fp64": err,
         "tuning": {"chunk_slices": chunk, "sweep_slices_per_wave": cw},
+        "kernels": kernels,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -24,7 +24,6 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
-#include <functional>
 #include <map>
 #include <memory>
 #include <vector>
@@ -41,6 +40,20 @@ namespace iccg {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double *plane(double *part, int j, int nch, int nc) {
   return part + (size_t)j * nch * nc * 2;
+}
+
+// the item's group: one-member groups come from the item itself
+template <int KM, class Item>
+__device__ __forceinline__ GroupDesc group_of(const GroupDesc *__restrict__ groups,
+                                              const Item &c) {
+  GroupDesc g;
+  if (KM == 1) {
+    g.k = 1;
+    g.slot[0] = c.slot0;
+  } else {
+    g = groups[c.grp];
+  }
+  return g;
 }
 
 // A x for every member of the group over the item's slices (A's entries
@@ -120,7 +133,7 @@ __global__ __launch_bounds__(kWG) void kb_ax(const GroupDesc *__restrict__ group
                                              const SysScal *__restrict__ sc, double *part,
                                              int nch, int mode) {
   const ChunkItem c = items[blockIdx.x];
-  const GroupDesc g = groups[c.grp];
+  const GroupDesc g = group_of<KM>(groups, c);
   if (mode == 3) {
     // A group whose members all stopped is not touched: its matrix may
     // already be freed (the running list keeps retired groups until it is
@@ -137,14 +150,15 @@ __global__ __launch_bounds__(kWG) void kb_ax(const GroupDesc *__restrict__ group
 }
 
 // x += alpha p; r -= alpha Ap; r.r partial -- per member
+template <int KM>
 __global__ __launch_bounds__(kWG) void kb_xr(const GroupDesc *__restrict__ groups,
                                              const SysDesc *__restrict__ desc,
                                              const ChunkItem *__restrict__ items,
                                              const SysScal *__restrict__ sc, double *part,
                                              int nch) {
   const ChunkItem c = items[blockIdx.x];
-  const GroupDesc g = groups[c.grp];
-  for (int j = 0; j < g.k; ++j) {
+  const GroupDesc g = group_of<KM>(groups, c);
+  for (int j = 0; j < (KM == 1 ? 1 : g.k); ++j) {
     const int id = g.slot[j];
     if (sc[id].done) continue;
     const SysDesc &D = desc[id];
@@ -169,14 +183,15 @@ __device__ __forceinline__ void rz_rows(const SysDesc &D, int beg, int end, doub
   for (int i = beg; i < end; i += kWG) dd_fma(h[0], l[0], D.r[i], (double)z[i]);
 }
 
+template <int KM>
 __global__ __launch_bounds__(kWG) void kb_rz(const GroupDesc *__restrict__ groups,
                                              const SysDesc *__restrict__ desc,
                                              const ChunkItem *__restrict__ items,
                                              const SysScal *__restrict__ sc, double *part,
                                              int nch) {
   const ChunkItem c = items[blockIdx.x];
-  const GroupDesc g = groups[c.grp];
-  for (int j = 0; j < g.k; ++j) {
+  const GroupDesc g = group_of<KM>(groups, c);
+  for (int j = 0; j < (KM == 1 ? 1 : g.k); ++j) {
     const int id = g.slot[j];
     if (sc[id].done) continue;
     const SysDesc &D = desc[id];
@@ -200,13 +215,14 @@ __device__ __forceinline__ void p_rows(const SysDesc &D, int beg, int end, doubl
   }
 }
 
+template <int KM>
 __global__ __launch_bounds__(kWG) void kb_update_p(const GroupDesc *__restrict__ groups,
                                                    const SysDesc *__restrict__ desc,
                                                    const ChunkItem *__restrict__ items,
                                                    const SysScal *__restrict__ sc, int init) {
   const ChunkItem c = items[blockIdx.x];
-  const GroupDesc g = groups[c.grp];
-  for (int j = 0; j < g.k; ++j) {
+  const GroupDesc g = group_of<KM>(groups, c);
+  for (int j = 0; j < (KM == 1 ? 1 : g.k); ++j) {
     const int id = g.slot[j];
     if (sc[id].done) continue;
     const SysDesc &D = desc[id];
@@ -404,7 +420,7 @@ __global__ __launch_bounds__(kWG) void kb_sweep(const GroupDesc *__restrict__ gr
   const int wv = blockIdx.x * (kWG >> 6) + (threadIdx.x >> 6);
   if (wv >= nitems) return;
   const ColorItem c = items[wv];
-  const GroupDesc g = groups[c.grp];
+  const GroupDesc g = group_of<KM>(groups, c);
   bool live[KM], any = false;
 #pragma unroll
   for (int j = 0; j < KM; ++j) {
@@ -594,6 +610,7 @@ struct iccg_batch_s {
   bool autotune = true;
   long tuned_rows = 0;        // running rows at the last tuning
   int max_group = 1;          // right-hand sides per group (see set_max_group)
+  std::map<int, std::pair<int, int>> tuned;  // log2(running rows) -> (chunk, cw)
   ItemSet running;            // items of the running set
   bool running_dirty = true;
   long running_work = 0;      // sum of listed_work over the running set
@@ -687,7 +704,7 @@ void build_items(iccg_batch_s &B, const std::vector<int> &ids, ItemSet &I, int c
     const Slot &s = B.slots[g.slot[0]];
     const int ns = (s.n + 63) / 64;
     gchunk0.push_back((int)chunks.size());
-    for (int a = 0; a < ns; a += ch) chunks.push_back({gi, a, std::min(ns, a + ch), 0});
+    for (int a = 0; a < ns; a += ch) chunks.push_back({gi, a, std::min(ns, a + ch), g.slot[0]});
     gchunks.push_back((int)chunks.size() - gchunk0.back());
     const DevSched &sc = s.M->sc;
     const int nc = sc.nc;
@@ -696,13 +713,13 @@ void build_items(iccg_batch_s &B, const std::vector<int> &ids, ItemSet &I, int c
       if (sc.kind == 2) {
         const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
         for (int kb = b0; kb < b1; kb += cw)
-          color[c].push_back({gi, kb, std::min(b1, kb + cw), 0, 0, {0, 0, 0}});
+          color[c].push_back({gi, kb, std::min(b1, kb + cw), 0, 0, g.slot[0], {0, 0}});
       } else {
         const int lo = sc.h_color_ptr_v[c], hi = sc.h_color_ptr_v[c + 1];
         if (hi <= lo) continue;
         const int s1 = ((hi - 1) >> 6) + 1;
         for (int sl = lo >> 6; sl < s1; sl += cw)
-          color[c].push_back({gi, sl, std::min(s1, sl + cw), lo, hi, {0, 0, 0}});
+          color[c].push_back({gi, sl, std::min(s1, sl + cw), lo, hi, g.slot[0], {0, 0}});
       }
     }
   }
@@ -764,6 +781,12 @@ int km_class(int k) { return k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : 8; }
     else if (_k == 4) hipLaunchKernelGGL((K<16, 4 EXTRA>), __VA_ARGS__);                       \
     else hipLaunchKernelGGL((K<16, 8 EXTRA>), __VA_ARGS__);                                    \
   } while (0)
+// vector kernels: a one-member path, and one for groups
+#define VEC_LAUNCH(km, K, ...)                                  \
+  do {                                                          \
+    if ((km) <= 1) hipLaunchKernelGGL((K<1>), __VA_ARGS__);     \
+    else hipLaunchKernelGGL((K<kMaxRhs>), __VA_ARGS__);         \
+  } while (0)
 #define NOEXTRA
 #define FWD_T , true
 #define BWD_T , false
@@ -808,10 +831,10 @@ void launch_sweeps(iccg_batch_s &B, const ItemSet &I) {
 void launch_rz_p(iccg_batch_s &B, const ItemSet &I, int init) {
   dim3 g(I.nchunks), b(kWG);
   mark(B, T_VEC, true);
-  hipLaunchKernelGGL(kb_rz, g, b, 0, B.st, I.d_groups, B.d_desc, I.d_chunks, B.d_sc, B.part,
+  VEC_LAUNCH(I.kmax, kb_rz, g, b, 0, B.st, I.d_groups, B.d_desc, I.d_chunks, B.d_sc, B.part,
                      I.nchunks);
   hipLaunchKernelGGL(kbf_rz, dim3(I.nfin), b, 0, B.st, I.d_fin, B.part, I.nchunks, B.d_sc, init);
-  hipLaunchKernelGGL(kb_update_p, g, b, 0, B.st, I.d_groups, B.d_desc, I.d_chunks, B.d_sc, init);
+  VEC_LAUNCH(I.kmax, kb_update_p, g, b, 0, B.st, I.d_groups, B.d_desc, I.d_chunks, B.d_sc, init);
   mark(B, T_VEC, false);
 }
 
@@ -830,7 +853,7 @@ void launch_iteration(iccg_batch_s &B, const ItemSet &I) {
   mark(B, T_SPMV, false);
   mark(B, T_VEC, true);
   hipLaunchKernelGGL(kbf_alpha, dim3(I.nfin), b, 0, B.st, I.d_fin, B.part, I.nchunks, B.d_sc);
-  hipLaunchKernelGGL(kb_xr, g, b, 0, B.st, I.d_groups, B.d_desc, I.d_chunks, B.d_sc, B.part,
+  VEC_LAUNCH(I.kmax, kb_xr, g, b, 0, B.st, I.d_groups, B.d_desc, I.d_chunks, B.d_sc, B.part,
                      I.nchunks);
   hipLaunchKernelGGL(kbf_xr, dim3(I.nfin), b, 0, B.st, I.d_fin, B.part, I.nchunks, B.d_sc);
   mark(B, T_VEC, false);
@@ -989,77 +1012,77 @@ void admit_pending(iccg_batch_s &B) {
   B.running_dirty = true;
 }
 
-double time_ms(iccg_batch_s &B, const std::function<void()> &f, int reps) {
-  f();  // warm
-  HIPCHK(hipEventRecord(B.ev0, B.st));
-  for (int r = 0; r < reps; ++r) f();
-  HIPCHK(hipEventRecord(B.ev1, B.st));
-  HIPCHK(hipEventSynchronize(B.ev1));
-  float ms = 0.f;
-  HIPCHK(hipEventElapsedTime(&ms, B.ev0, B.ev1));
-  return ms / reps;
-}
+constexpr int kTuneChunks[] = {4, 8, 16, 32};
+constexpr int kTuneSweeps[] = {1, 2, 4};
+constexpr int kTuneIters = 2;  // timed iterations per candidate
+constexpr int kTuneSteps =
+    1 + kTuneIters * (int)(sizeof(kTuneChunks) / sizeof(int)) * (int)(sizeof(kTuneSweeps) / sizeof(int));
 
-// Autotuner: times the candidate work decompositions on the running systems
-// and keeps the fastest.  The probes are side-effect free at an iteration
-// boundary: the SpMV rewrites Ap and the sweeps rewrite y and z with the
-// values they already hold; partials are scratch.
-void tune(iccg_batch_s &B, const std::vector<int> &ids) {
-  if (!B.ev0) {
-    HIPCHK(hipEventCreate(&B.ev0));
-    HIPCHK(hipEventCreate(&B.ev1));
-  }
-  const int chunks[] = {4, 8, 16, 32};
-  const int cws[] = {1, 2, 4};
-  double best = 1e300;
-  int best_ch = B.chunk;
-  ItemSet T;
-  const bool timing = B.timing;
-  B.timing = false;  // the probes are not counted
-  for (int ch : chunks) {
-    build_items(B, ids, T, ch, B.cw);
-    const double ms = time_ms(B, [&] {
-      launch_ax(B, T, 3);
-      hipLaunchKernelGGL(kb_rz, dim3(T.nchunks), dim3(kWG), 0, B.st, T.d_groups, B.d_desc,
-                         T.d_chunks, B.d_sc, B.part, T.nchunks);
-    }, 3);
-    if (ms < best) {
-      best = ms;
-      best_ch = ch;
-    }
-  }
-  best = 1e300;
-  int best_cw = B.cw;
-  for (int cw : cws) {
-    build_items(B, ids, T, best_ch, cw);
-    const double ms = time_ms(B, [&] { launch_sweeps(B, T); }, 3);
-    if (ms < best) {
-      best = ms;
-      best_cw = cw;
-    }
-  }
-  HIPCHK(hipGetLastError());
-  T.clear();
-  B.chunk = best_ch;
-  B.cw = best_cw;
-  B.timing = timing;
-}
-
-void refresh_running(iccg_batch_s &B) {
-  if (!B.running_dirty && B.dead_work * 4 <= B.running_work) return;
-  std::vector<int> ids;
+// the running slots (state 2, not stopped) and their rows
+long running_ids(iccg_batch_s &B, std::vector<int> &ids) {
+  ids.clear();
   long rows = 0;
   for (int id = 0; id < B.cap; ++id)
     if (B.slots[id].state == 2 && !B.h_sc[id].done) {
       ids.push_back(id);
       rows += B.slots[id].n;
     }
-  B.drop_graphs();
-  if (B.autotune && rows >= (1L << 16) &&
-      (B.tuned_rows == 0 || rows > 2 * B.tuned_rows || 2 * rows < B.tuned_rows)) {
-    tune(B, ids);
-    B.tuned_rows = rows;
+  return rows;
+}
+
+bool tune_due(iccg_batch_s &B, long rows) {
+  return B.autotune && rows >= (1L << 16) &&
+         (B.tuned_rows == 0 || rows > 2 * B.tuned_rows || 2 * rows < B.tuned_rows);
+}
+
+// Online autotuner: runs real CG iterations of the running systems under
+// every candidate decomposition (kTuneIters each, after one warm-up) with
+// HIP events around them and keeps the fastest.  The iterations are the
+// solve's own -- the decomposition changes no row's arithmetic -- so tuning
+// costs only the launches it makes without graphs.  Returns the iterations
+// run (kTuneSteps).
+int tune_online(iccg_batch_s &B, const std::vector<int> &ids) {
+  if (!B.ev0) {
+    HIPCHK(hipEventCreate(&B.ev0));
+    HIPCHK(hipEventCreate(&B.ev1));
   }
+  const bool timing = B.timing;
+  B.timing = false;  // the tuning launches are not counted
+  B.drop_graphs();
+  ItemSet T;
+  build_items(B, ids, T, B.chunk, B.cw);
+  launch_iteration(B, T);  // warm-up
+  double best = 1e300;
+  int best_ch = B.chunk, best_cw = B.cw;
+  for (int ch : kTuneChunks)
+    for (int cw : kTuneSweeps) {
+      build_items(B, ids, T, ch, cw);
+      HIPCHK(hipEventRecord(B.ev0, B.st));
+      for (int r = 0; r < kTuneIters; ++r) launch_iteration(B, T);
+      HIPCHK(hipEventRecord(B.ev1, B.st));
+      HIPCHK(hipEventSynchronize(B.ev1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, B.ev0, B.ev1));
+      if (ms < best) {
+        best = ms;
+        best_ch = ch;
+        best_cw = cw;
+      }
+    }
+  HIPCHK(hipGetLastError());
+  T.clear();
+  B.chunk = best_ch;
+  B.cw = best_cw;
+  B.timing = timing;
+  B.running_dirty = true;  // rebuild with the chosen decomposition
+  return kTuneSteps;
+}
+
+void refresh_running(iccg_batch_s &B) {
+  if (!B.running_dirty && B.dead_work * 4 <= B.running_work) return;
+  std::vector<int> ids;
+  running_ids(B, ids);
+  B.drop_graphs();
   for (Slot &s : B.slots) s.listed_work = 0;
   build_items(B, ids, B.running, B.chunk, B.cw);
   B.running_work = 0;
@@ -1215,10 +1238,33 @@ int iccg_batch_run(iccg_batch_t B, int max_steps, int *n_running) {
     read_scalars(*B);
     settle(*B);  // systems already converged at iteration 0
     int steps = 0, chunk = 4;
+    std::vector<int> ids;
     for (;;) {
       int running = 0;
       for (auto &s : B->slots) running += s.state == 2;
       if (running == 0 || steps >= max_steps) break;
+      const long rows = running_ids(*B, ids);
+      if (tune_due(*B, rows)) {
+        // a size class seen before keeps its choice; a new one is tuned
+        int bucket = 0;
+        while ((2L << bucket) <= rows) ++bucket;
+        auto it = B->tuned.find(bucket);
+        if (it != B->tuned.end()) {
+          if (B->chunk != it->second.first || B->cw != it->second.second) {
+            B->chunk = it->second.first;
+            B->cw = it->second.second;
+            B->running_dirty = true;
+          }
+          B->tuned_rows = rows;
+        } else if (max_steps - steps >= kTuneSteps) {
+          steps += tune_online(*B, ids);
+          B->tuned[bucket] = {B->chunk, B->cw};
+          B->tuned_rows = rows;
+          read_scalars(*B);
+          settle(*B);
+          continue;
+        }
+      }
       refresh_running(*B);
       const int k = std::min(chunk, max_steps - steps);
       enqueue_iterations(*B, k);
@@ -1338,6 +1384,7 @@ int iccg_batch_set_autotune(iccg_batch_t B, int on) {
     require(B, ICCG_ERR_ARG, "null argument");
     B->autotune = on != 0;
     B->tuned_rows = 0;
+    B->tuned.clear();
   });
 }
 

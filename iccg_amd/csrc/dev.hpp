@@ -153,14 +153,16 @@ struct GroupDesc {
   int k, pad;
   int slot[kMaxRhs];  // slot[0] also names the shared matrix and factor
 };
+// Items name their group and its first slot (slot0): one-member groups
+// (the default) reach their system without the group table.
 struct ChunkItem {  // rows [64*s0, min(64*s1, n)) of group grp: one workgroup
-  int grp, s0, s1, pad;
+  int grp, s0, s1, slot0;
 };
 // One wavefront of a sweep over one color of group grp:
 //   multicolor: 64-row slices [a0, a1), rows restricted to [lo, hi);
 //   ABMC      : blocks [a0, a1).
 struct ColorItem {
-  int grp, a0, a1, lo, hi, pad[3];
+  int grp, a0, a1, lo, hi, slot0, pad[2];
 };
 // Finish work of one slot: its partials are chunks [c0, c1) of plane `plane`.
 struct FinItem {

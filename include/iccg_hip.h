@@ -283,11 +283,14 @@ int iccg_batch_set_graphs(iccg_batch_t B, int on);
 /* Work decomposition of the batched kernels: 64-row slices per workgroup of
  * the SpMV / vector kernels (chunk_slices) and 64-row slices (multicolor)
  * or blocks (ABMC) per wavefront of the triangular sweeps (sweep_units).
- * With autotune on (the default), the batch times the candidates on its
- * running systems whenever their total size changes by more than 2x
- * (from 65,536 rows up) and keeps the fastest; set_tuning fixes them and
- * turns autotune off.  The arithmetic of every row is the same for every
- * decomposition. */
+ * With autotune on (the default), whenever the running systems' total size
+ * changes by more than 2x (from 65,536 rows up), the batch runs two real
+ * iterations under each candidate (4, 8, 16, 32 slices x 1, 2, 4 per
+ * wavefront; 25 iterations in all, counted in max_steps) and keeps the
+ * fastest, remembered per power-of-two size class so that a recurring load
+ * is tuned once; set_autotune(1) forgets the choices, set_tuning fixes
+ * them and turns autotune off.  The arithmetic of
+ * every row is the same for every decomposition. */
 int iccg_batch_set_autotune(iccg_batch_t B, int on);
 int iccg_batch_get_tuning(iccg_batch_t B, int *chunk_slices, int *sweep_units);
 int iccg_batch_set_tuning(iccg_batch_t B, int chunk_slices, int sweep_units);
