@@ -210,8 +210,78 @@ HostCsr read_matrix_market(const std::string &path) {
 }
 
 // crs_io.cpp:163-227
+// Lower CSR with strictly ascending rows: row r of the full matrix is its
+// own lower entries followed by column r of the strict lower part in
+// ascending row order -- built in parallel (atomic cursors for the upper
+// entries, then a per-row sort of that part), the same arrays as the
+// sequential build below.
+static HostCsr expand_sorted_lower(int n, const int *rp, const int *ci, const double *v) {
+  HostCsr F;
+  F.n = n;
+  std::vector<int> cnt((size_t)n, 0);
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int i = 0; i < n; ++i) {
+#pragma omp atomic
+    cnt[i] += rp[i + 1] - rp[i];
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      const int j = ci[k];
+      if (j != i) {
+#pragma omp atomic
+        cnt[j]++;
+      }
+    }
+  }
+  F.rowptr.assign((size_t)n + 1, 0);
+  for (int i = 0; i < n; ++i) F.rowptr[i + 1] = F.rowptr[i] + cnt[i];
+  F.colind.resize((size_t)F.rowptr[n]);
+  F.val.resize((size_t)F.rowptr[n]);
+  std::vector<int> cur((size_t)n);
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int i = 0; i < n; ++i) {
+    const int own = rp[i + 1] - rp[i];
+    std::copy(ci + rp[i], ci + rp[i + 1], F.colind.begin() + F.rowptr[i]);
+    std::copy(v + rp[i], v + rp[i + 1], F.val.begin() + F.rowptr[i]);
+    cur[i] = F.rowptr[i] + own;
+  }
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int i = 0; i < n; ++i)
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      const int j = ci[k];
+      if (j == i) continue;
+      int pos;
+#pragma omp atomic capture
+      pos = cur[j]++;
+      F.colind[pos] = i;
+      F.val[pos] = v[k];
+    }
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int r = 0; r < n; ++r) {
+    const int s = F.rowptr[r] + (rp[r + 1] - rp[r]), e = F.rowptr[r + 1];
+    for (int a = s + 1; a < e; ++a) {
+      const int cc = F.colind[a];
+      const double vv = F.val[a];
+      int b = a - 1;
+      while (b >= s && F.colind[b] > cc) {
+        F.colind[b + 1] = F.colind[b];
+        F.val[b + 1] = F.val[b];
+        --b;
+      }
+      F.colind[b + 1] = cc;
+      F.val[b + 1] = vv;
+    }
+  }
+  return F;
+}
+
 HostCsr expand_lower_to_full(int n, const int *rp, const int *ci,
                              const double *v) {
+  int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (int i = 0; i < n; ++i) {
+    for (int k = rp[i] + 1; k < rp[i + 1]; ++k) bad |= ci[k - 1] >= ci[k];
+    if (rp[i + 1] > rp[i]) bad |= ci[rp[i + 1] - 1] > i;
+  }
+  if (!bad) return expand_sorted_lower(n, rp, ci, v);
   HostCsr F;
   F.n = n;
   std::vector<long> cnt((size_t)n + 1, 0);

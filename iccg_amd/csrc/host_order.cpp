@@ -62,22 +62,66 @@ std::vector<int> perm_by_block_color(int n, const int *block_of_old, int nb,
   return new_of_old;
 }
 
+// Every row strictly ascending (sorted, no repeated column); parallel.
+static bool rows_strictly_sorted(int n, const int *rp, const int *ci) {
+  int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (int i = 0; i < n; ++i)
+    for (int k = rp[i] + 1; k < rp[i + 1]; ++k) bad |= ci[k - 1] >= ci[k];
+  return bad == 0;
+}
+
 // color.cpp:66-141
 HostCsr ptap_lower(int n, const int *rp, const int *ci, const double *v,
                    const int *new_of_old) {
   std::vector<long> cnt((size_t)n + 1, 0);
-  for (int i = 0; i < n; ++i)
-    for (int k = rp[i]; k < rp[i + 1]; ++k) {
-      int j = ci[k];
-      if (j > i) continue;
-      int r = std::max(new_of_old[i], new_of_old[j]);
-      cnt[(size_t)r + 1]++;
-    }
+  // With unique columns in every row, every output entry (r, c) comes from
+  // exactly one input entry, so the scatter can run in parallel in any
+  // order: the per-row sort by column below fixes the result, and there is
+  // nothing to sum.  Otherwise the scatter keeps the input order (duplicates
+  // are summed in it).
+  const bool par = rows_strictly_sorted(n, rp, ci);
+  if (par) {
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int i = 0; i < n; ++i)
+      for (int k = rp[i]; k < rp[i + 1]; ++k) {
+        const int j = ci[k];
+        if (j > i) continue;
+        const int r = std::max(new_of_old[i], new_of_old[j]);
+#pragma omp atomic
+        cnt[(size_t)r + 1]++;
+      }
+  } else {
+    for (int i = 0; i < n; ++i)
+      for (int k = rp[i]; k < rp[i + 1]; ++k) {
+        int j = ci[k];
+        if (j > i) continue;
+        int r = std::max(new_of_old[i], new_of_old[j]);
+        cnt[(size_t)r + 1]++;
+      }
+  }
   for (int r = 0; r < n; ++r) cnt[r + 1] += cnt[r];
   const long tot = cnt[n];
   std::vector<int> bc((size_t)tot);
   std::vector<double> bv((size_t)tot);
-  {
+  if (par) {
+    std::vector<long> cur(cnt.begin(), cnt.end() - 1);
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int i = 0; i < n; ++i) {
+      const int ip = new_of_old[i];
+      for (int k = rp[i]; k < rp[i + 1]; ++k) {
+        const int j = ci[k];
+        if (j > i) continue;
+        const int jp = new_of_old[j];
+        const int r = std::max(ip, jp), c = std::min(ip, jp);
+        long pos;
+#pragma omp atomic capture
+        pos = cur[r]++;
+        bc[pos] = c;
+        bv[pos] = v[k];
+      }
+    }
+  } else {
     std::vector<long> cur(cnt.begin(), cnt.end() - 1);
     for (int i = 0; i < n; ++i) {
       int ip = new_of_old[i];

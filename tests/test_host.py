@@ -253,3 +253,45 @@ def test_validators_flag_bad_colorings():
     color = np.zeros(G.n, np.int32)
     th, hz, sb, cb = I.check_hazards(G, np.arange(G.n, dtype=np.int32), color)
     assert th == (G.nnz - G.n) // 2
+
+
+def test_ptap_and_expand_parallel_and_ordered_paths_match_oracle():
+    """PtAP and lower->full take a parallel scatter when every row is
+    strictly ascending and the ordered one otherwise; both equal the
+    oracle's restatement of color.cpp:66-141 / crs_io.cpp:163-227 bit for
+    bit, including rows with duplicate columns (summed in input order) and
+    unsorted rows."""
+    rng = np.random.default_rng(5)
+    A = P.grid(23, 17, 3, coef=(-2.0, 2.0))
+    p = rng.permutation(A.n).astype(np.int32)
+    Ao = P.orc(A)
+    got = I.Permute_PtAP_LowerCRS_to_LowerCRS(A, p)
+    ref = O.ptap_lower(Ao, p)
+    assert np.array_equal(got.rowptr, ref.rowptr) and np.array_equal(got.colind, ref.colind)
+    assert np.array_equal(got.val, ref.val)
+    F = I.expand_lower_to_full(got)
+    Fr = O.expand_lower_to_full(ref)
+    assert np.array_equal(F.rowptr, Fr.rowptr) and np.array_equal(F.colind, Fr.colind)
+    assert np.array_equal(F.val, Fr.val)
+    # rows out of order and with repeated columns: the ordered path
+    n = 40
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        for j in rng.choice(i + 1, size=min(i + 1, 4), replace=False)[::-1]:
+            rows.append(i); cols.append(int(j)); vals.append(float(rng.normal()))
+        rows.append(i); cols.append(i); vals.append(1.5)  # a repeated diagonal entry
+    order = np.argsort(np.array(rows), kind="stable")
+    rp = np.zeros(n + 1, np.int32)
+    np.add.at(rp, np.array(rows) + 1, 1)
+    rp = np.cumsum(rp).astype(np.int32)
+    ci = np.array(cols, np.int32)[order]
+    v = np.array(vals)[order]
+    U = I.CRS(n, rp, ci, v)
+    Uo = O.Csr(n, rp, ci, v)
+    p = rng.permutation(n).astype(np.int32)
+    got = I.Permute_PtAP_LowerCRS_to_LowerCRS(U, p)
+    ref = O.ptap_lower(Uo, p)
+    assert np.array_equal(got.colind, ref.colind) and np.array_equal(got.val, ref.val)
+    F = I.expand_lower_to_full(U)
+    Fr = O.expand_lower_to_full(Uo)
+    assert np.array_equal(F.colind, Fr.colind) and np.array_equal(F.val, Fr.val)
