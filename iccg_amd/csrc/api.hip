@@ -696,6 +696,11 @@ int iccg_csr_create(int n, long nnz, const int *rowptr, const int *colind, const
         S.off = dupload(off.data(), off.size(), A->st);
         S.col = dupload(col.data(), col.size(), A->st);
         S.v64 = dupload(v.data(), v.size(), A->st);
+        // the single-system SpMV reads 16-bit columns where they apply
+        // (measured 146 vs 156 us at 192^3); the triangular sweeps and the
+        // multi-system SpMV keep 32-bit columns (there the per-slice table
+        // load lengthens a latency-bound chain: measured slower)
+        upload_col16(S, off, col, A->st);
         // values that are all exact floats also get an f32 copy (half the
         // value bytes for the multi-system SpMV, same products bit for bit)
         bool exact = true;

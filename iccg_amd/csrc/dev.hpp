@@ -57,6 +57,11 @@ struct Sell {
   int *col = nullptr;    // off[nslice], -1 = padding
   double *v64 = nullptr;
   float *v32 = nullptr;
+  // 16-bit columns, when every slice references at most 16 distinct 64-row
+  // column blocks: tbl[16 s + t] = t-th block of slice s; an entry is
+  // (t << 6 | column & 63), padding 0xFFFF.  Half the index bytes.
+  uint16_t *col16 = nullptr;
+  int *tbl = nullptr;
 };
 
 struct DevCsr {

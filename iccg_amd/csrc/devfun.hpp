@@ -186,6 +186,19 @@ __device__ __forceinline__ void st4(double *p, int i, const double (&o)[4]) {
 // ---------------------------------------------------------------------------
 // Sliced-ELL row chains
 // ---------------------------------------------------------------------------
+// Column of a 16-bit sliced-ELL entry, given its slice's block table in LDS.
+__device__ __forceinline__ int col16_at(unsigned c, const int *tb) {
+  return c == 0xFFFFu ? -1 : (tb[(c >> 6) & 15u] << 6) | (int)(c & 63u);
+}
+
+// Copies the block table of slice s into the wavefront's LDS window tb (16
+// ints); called by every lane of the wavefront.
+__device__ __forceinline__ void load_tbl16(const int *__restrict__ tbl, int s, int *tb, int lane) {
+  wave_sync();  // earlier reads of tb are complete
+  if (lane < 16) tb[lane] = tbl[16 * s + lane];
+  wave_sync();
+}
+
 // The lane's row of slice s as registers: columns (padding -1) and values.
 template <class VT, int WM>
 __device__ __forceinline__ void sell_load(const int *__restrict__ off, const int *__restrict__ col,
@@ -216,6 +229,46 @@ __device__ __forceinline__ AT sell_apply(const int (&cc)[WM], const VT (&vv)[WM]
 #pragma unroll
   for (int k = 0; k < WM; ++k)
     if (cc[k] >= 0) acc = NEG ? fma_t(-(AT)vv[k], xv[k], acc) : fma_t((AT)vv[k], xv[k], acc);
+  return acc;
+}
+
+// sell_chain over the 16-bit layout (tb: the slice's table, load_tbl16)
+template <class VT, class AT, class XT, int WM, bool NEG>
+__device__ __forceinline__ AT sell_chain16(const int *__restrict__ off,
+                                           const uint16_t *__restrict__ col16, const int *tb,
+                                           const VT *__restrict__ v, const XT *__restrict__ x,
+                                           int s, int lane, AT acc) {
+  const int o0 = off[s];
+  const int w = (off[s + 1] - o0) >> 6;
+  const int base = o0 + lane;
+  if (WM > 0) {
+    constexpr int K = WM > 0 ? WM : 1;
+    int cc[K];
+    VT vv[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      cc[k] = -1;
+      vv[k] = (VT)0;
+      if (k < w) {
+        cc[k] = col16_at(col16[base + 64 * k], tb);
+        vv[k] = v[base + 64 * k];
+      }
+    }
+    AT xv[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) xv[k] = cc[k] >= 0 ? (AT)x[cc[k]] : (AT)0;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (cc[k] >= 0) acc = NEG ? fma_t(-(AT)vv[k], xv[k], acc) : fma_t((AT)vv[k], xv[k], acc);
+  } else {
+    for (int k = 0; k < w; ++k) {
+      const int c = col16_at(col16[base + 64 * k], tb);
+      if (c >= 0) {
+        const AT a = (AT)v[base + 64 * k];
+        acc = NEG ? fma_t(-a, (AT)x[c], acc) : fma_t(a, (AT)x[c], acc);
+      }
+    }
+  }
   return acc;
 }
 

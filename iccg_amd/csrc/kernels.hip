@@ -41,7 +41,19 @@ __device__ __forceinline__ double row_dot(const int *__restrict__ rp, const int 
 // f(i, Ax_i) for every row i < n; lanes past n get no call.
 template <int WM, class F>
 __device__ __forceinline__ void for_rows_Ax(const DevCsr &A, const double *__restrict__ x, F f) {
-  if (A.sell.off) {
+  if (A.sell.off && A.sell.col16) {
+    __shared__ int tbs[kWG >> 6][16];
+    const int lane = threadIdx.x & 63;
+    int *tb = tbs[threadIdx.x >> 6];
+    const WaveRange wr = xcd_waves(0, A.sell.nslice);
+    for (int s = wr.first; s < wr.end; s += wr.step) {
+      load_tbl16(A.sell.tbl, s, tb, lane);
+      const double y = sell_chain16<double, double, double, WM, false>(
+          A.sell.off, A.sell.col16, tb, A.sell.v64, x, s, lane, 0.0);
+      const int i = (s << 6) + lane;
+      if (i < A.n) f(i, y);
+    }
+  } else if (A.sell.off) {
     const int lane = threadIdx.x & 63;
     const WaveRange wr = xcd_waves(0, A.sell.nslice);
     for (int s = wr.first; s < wr.end; s += wr.step) {

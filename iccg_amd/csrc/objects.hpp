@@ -50,7 +50,52 @@ inline void free_sell(Sell &S) {
   dfree(S.col);
   dfree(S.v64);
   dfree(S.v32);
+  dfree(S.col16);
+  dfree(S.tbl);
   S = Sell();
+}
+
+// The 16-bit column layout of a sliced ELL (see Sell): false, and nothing
+// built, when some slice references more than 16 distinct 64-row blocks.
+inline bool sell_col16(const std::vector<int> &off, const std::vector<int> &col,
+                       std::vector<uint16_t> &c16, std::vector<int> &tbl) {
+  const int ns = (int)off.size() - 1;
+  c16.assign(col.size(), (uint16_t)0xFFFF);
+  tbl.assign((size_t)16 * std::max(ns, 0), 0);
+  for (int s = 0; s < ns; ++s) {
+    int t[16], nt = 0;
+    for (int q = off[s]; q < off[s + 1]; ++q) {
+      const int c = col[(size_t)q];
+      if (c < 0) continue;
+      const int b = c >> 6;
+      int x = 0;
+      while (x < nt && t[x] != b) ++x;
+      if (x == nt) {
+        if (nt == 16) {
+          c16.clear();
+          tbl.clear();
+          return false;
+        }
+        t[nt++] = b;
+      }
+      c16[(size_t)q] = (uint16_t)((x << 6) | (c & 63));
+    }
+    for (int x = 0; x < 16; ++x) tbl[(size_t)16 * s + x] = x < nt ? t[x] : 0;
+  }
+  return true;
+}
+
+// uploads the 16-bit column layout of S (host off / col) when it applies
+inline void upload_col16(Sell &S, const std::vector<int> &off, const std::vector<int> &col,
+                         hipStream_t st) {
+  std::vector<uint16_t> c16;
+  std::vector<int> tbl;
+  if (!sell_col16(off, col, c16, tbl)) return;
+  if (c16.empty()) c16.push_back((uint16_t)0xFFFF);
+  if (tbl.empty()) tbl.push_back(0);
+  S.col16 = dupload(c16.data(), c16.size(), st);
+  S.tbl = dupload(tbl.data(), tbl.size(), st);
+  HIPCHK(hipStreamSynchronize(st));
 }
 
 // Host layout of a 64-row sliced ELL over row ranges [beg(i), end(i)) of a

@@ -35,6 +35,25 @@ def test_spmv_bitwise(name):
     assert np.array_equal(y, O.spmv(P.orc(A), x))
 
 
+@pytest.mark.parametrize("spread", [3, 40])
+def test_spmv_16bit_columns_and_fallback(spread):
+    """The SpMV reads 16-bit columns (per-slice tables of at most 16 target
+    64-row blocks) when every slice fits; rows whose entries reach 40
+    different blocks per slice keep 32-bit columns.  Both bitwise."""
+    rng = np.random.default_rng(spread)
+    n = 64 * 48
+    rows, cols = [], []
+    for i in range(n):
+        blocks = rng.choice(48, size=min(spread, 8), replace=False)
+        js = np.unique(blocks * 64 + rng.integers(0, 64, size=len(blocks)))
+        rows += [i] * len(js)
+        cols += list(js)
+    v = rng.normal(size=len(rows))
+    A = I.coo_to_crs(n, np.array(rows), np.array(cols), v)
+    x = I.gen_normal(A.n, 12)
+    assert np.array_equal(I.DeviceCSR(A).spmv(x), O.spmv(P.orc(A), x))
+
+
 def test_spmv_empty_rows():
     # a row with no entries gives y_i = 0 (sum starts at 0.0)
     A = I.CRS(3, np.array([0, 1, 1, 3], np.int32), np.array([0, 0, 2], np.int32),
