@@ -564,6 +564,17 @@ def main():
                 "frac": dom[1]["frac"], "traffic": traffic,
                 "copy_kernel_gbs": copy_gbs,
                 "algorithmic_bytes_per_launch": dom[1]["bytes"], "avg_ms": dom[1]["avg_ms"]}
+    if dom[0] == "spmv_dots":
+        # the kernel's own layout: 64-row slices padded to their widest row,
+        # FP64 values with 16-bit columns, one 16-entry block table per slice
+        w = np.diff(Ap.rowptr)
+        pad = np.zeros(((n + 63) // 64) * 64, np.int64)
+        pad[:n] = w
+        padded = int(pad.reshape(-1, 64).max(axis=1).sum()) * 64
+        ns = (n + 63) // 64
+        lay = padded * (8 + 2) + ns * 64 + 4 * (ns + 1) + 2 * n * 8
+        roofline["kernel_layout_bytes_per_launch"] = int(lay)
+        roofline["kernel_layout_gbs"] = round(lay / (dom[1]["avg_ms"] * 1e-3) / 1e9, 1)
 
     cpu = None
     if not args.no_cpu:
