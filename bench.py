@@ -491,7 +491,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU design "
-              "is described in DESIGN.md and is not run here", file=sys.stderr)
+              "(sharding by row slabs or by whole systems) is not run here", file=sys.stderr)
         sys.exit(2)
 
     import iccg_amd as I

@@ -2,7 +2,7 @@
 // counterpart: the reference reads SuiteSparse files and hard-codes b = 1).
 //
 // Deterministic on the host: splitmix64 stream -> 53-bit uniforms in [0,1);
-// normals by Box-Muller in FP64 with glibc log/sqrt/cos/sin.  The CPU checker
+// normals by Box-Muller in FP64 with glibc log/sqrt/cos/sin.  The CPU oracle
 // and the GPU path receive the same bits.
 #include <cmath>
 #include <cstdint>

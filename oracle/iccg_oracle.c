@@ -1,8 +1,8 @@
 /*
  * iccg_oracle.c -- CPU restatement of the stomo0511/ICCG solve path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker, never the thing measured or
- * shipped).  Loaded by tests/, __graft_entry__.smoke() and the cpu_baseline
+ * TEST INFRASTRUCTURE ONLY (the reference for comparison, never the thing
+ * measured or shipped).  Loaded by tests/, __graft_entry__.smoke() and the cpu_baseline
  * leg of bench.py; the HIP product path never links or calls it.
  *
  * Build: gcc -O2 -std=c11 -ffp-contract=off (see oracle/Makefile).  Every

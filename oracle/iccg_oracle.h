@@ -1,8 +1,8 @@
 /*
  * iccg_oracle.h -- CPU restatement of the stomo0511/ICCG solve path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
- * product path: only tests/, __graft_entry__.smoke() and bench.py's
+ * TEST INFRASTRUCTURE ONLY.  This library is what the HIP product path is
+ * compared against: only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg may load it.  Nothing in the product links it.
  *
  * Every function restates one reference routine in plain C, single-threaded,

@@ -2,7 +2,7 @@
 (_ref/libref.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product never imports it.
+cpu_baseline leg of bench.py, for comparison.  The product never imports it.
 """
 from __future__ import annotations
 

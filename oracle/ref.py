@@ -1,7 +1,7 @@
 """ctypes bindings for oracle/_ref/libref.so: the reference's own sources
 compiled in place (oracle/Makefile) behind oracle/ref_harness.cpp.
 
-TEST INFRASTRUCTURE ONLY (checker / CPU baseline).  available() is False when
+TEST INFRASTRUCTURE ONLY (comparison / CPU baseline).  available() is False when
 the library was not built (no reference sources at build time).
 """
 from __future__ import annotations

@@ -164,7 +164,7 @@ def test_cg_matches_oracle(kind, name):
 
 
 def test_cg_c1_shape():
-    """2D 100x100 Laplacian, b = A*1, x0 = 0, tol 1e-8."""
+    """2D 100x100 Laplacian (the CPU configuration), b = A*1, x0 = 0, tol 1e-8."""
     A = P.grid(100, 100)
     b = P.rhs_ones_times_A(A)
     for kind in ("none", "ic0"):

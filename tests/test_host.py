@@ -214,7 +214,7 @@ def test_generator_matches_restatement():
             s, e = G.rowptr[i], G.rowptr[i + 1]
             assert list(G.colind[s:e]) == [c for c, _ in ent]
             assert list(G.val[s:e]) == [v for _, v in ent]
-    L = I.gen_grid(100, 100)  # diag 4, off-diagonal -1
+    L = I.gen_grid(100, 100)  # the 100x100 Laplacian: diag 4, off-diagonal -1
     assert L.n == 10000 and L.nnz == 49600
     assert set(np.unique(L.val)) == {-1.0, 4.0}
     C3 = I.gen_grid(12, 12, 12)

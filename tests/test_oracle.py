@@ -1,5 +1,5 @@
-"""Pins the CPU oracle (oracle/iccg_oracle.c) before it is trusted as the
-checker of the GPU path.
+"""Pins the CPU oracle (oracle/iccg_oracle.c) before the GPU path is
+compared against it.
 
 Sources of truth, strongest first:
   * anchors recorded from the reference's own binaries (iteration counts and
