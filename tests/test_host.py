@@ -295,3 +295,32 @@ def test_ptap_and_expand_parallel_and_ordered_paths_match_oracle():
     F = I.expand_lower_to_full(U)
     Fr = O.expand_lower_to_full(Uo)
     assert np.array_equal(F.colind, Fr.colind) and np.array_equal(F.val, Fr.val)
+
+
+def test_c_abi_header_is_plain_c_and_links(tmp_path):
+    """include/iccg_hip.h compiles as C99 and a C program that references
+    the batch and request-loop entry points links against the library (no
+    GPU call is made)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "use.c"
+    src.write_text(
+        '#include "iccg_hip.h"\n'
+        "int main(void) {\n"
+        "  iccg_server_config cfg = {ICCG_SERVE_BATCHED, ICCG_MIXED, 8, 2, 64,\n"
+        "                            ICCG_SOLVE_TRUE_RESID, 1LL << 30};\n"
+        "  void *f[] = {(void *)iccg_batch_create, (void *)iccg_batch_run,\n"
+        "               (void *)iccg_server_create, (void *)iccg_server_submit,\n"
+        "               (void *)iccg_cg_solve};\n"
+        "  return (cfg.max_slots == 8 && f[0] && f[4]) ? 0 : 1;\n"
+        "}\n")
+    exe = tmp_path / "use"
+    r = subprocess.run([cc, "-std=c99", "-Wall", "-Werror", "-I", os.path.join(root, "include"),
+                        str(src), "-o", str(exe), "-L", os.path.join(root, "iccg_amd", "lib"),
+                        "-liccg_hip", "-Wl,-rpath," + os.path.join(root, "iccg_amd", "lib")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
