@@ -200,6 +200,13 @@ struct iccg_server_s {
       iccg_sched_t s;
       ~SchedGuard() { iccg_schedule_destroy(s); }
     } sg{sch};
+    if (cfg.mode == ICCG_SERVE_BATCHED) {
+      int wmax = 0;
+      for (int i = 0; i < n; ++i) wmax = std::max(wmax, frp[(size_t)i + 1] - frp[(size_t)i]);
+      require(wmax <= 16, ICCG_ERR_ARG,
+              "the batched request loop takes rows of at most 16 entries (this matrix has " +
+                  std::to_string(wmax) + ")");
+    }
     ck(iccg_csr_create(n, nnz_f, frp.data(), fci.data(), fv.data(), &r.A));
     ck(iccg_precond_create(r.A, ICCG_PRECOND_IC0, 0.0, sch, cfg.precision, &r.M));
     r.bp.resize(r.b.size());
@@ -275,16 +282,30 @@ struct iccg_server_s {
           running_reqs++;
         }
       }
-      try {
-        for (Request *r : take) {
-          r->state = Q_RUN;
+      // a request the batch does not take fails alone
+      for (Request *r : take) {
+        r->state = Q_RUN;
+        try {
           for (int k = 0; k < r->nrhs; ++k) {
             int slot = -1;
             ck(iccg_batch_add(B, r->A, r->M, r->bp.data() + (size_t)k * r->n, nullptr,
                               r->max_iter, r->tol, cfg.flags, &slot));
             owner[(size_t)slot] = {r, k};
           }
+        } catch (const Error &e) {
+          for (int s = 0; s < (int)owner.size(); ++s)
+            if (owner[(size_t)s].first == r) {
+              iccg_batch_remove(B, s);
+              owner[(size_t)s] = {nullptr, 0};
+            }
+          r->free_device();
+          std::lock_guard<std::mutex> lk(mu);
+          free_slots += r->nrhs;
+          running_reqs--;
+          complete_locked(r, e.status, e.what());
         }
+      }
+      try {
         int nrun = 0;
         ck(iccg_batch_run(B, 64, &nrun));
         for (int s = 0; s < (int)owner.size(); ++s) {

@@ -128,3 +128,24 @@ def test_server_general_matrix_and_failures():
     assert S.stats()["failed"] == 1
     with pytest.raises(I.IccgError, match="grid"):
         S.submit(A, b, grid=(31, 30, 1))
+
+
+def test_server_wide_rows_fail_alone():
+    """A request whose rows exceed the batched kernels' width fails with its
+    own error; a request served beside it is unaffected."""
+    n, bw = 300, 10
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        for j in range(max(0, i - bw), min(n, i + bw + 1)):
+            rows.append(i); cols.append(j); vals.append(4.0 * bw if i == j else -1.0)
+    wide = I.coo_to_crs(n, np.array(rows), np.array(cols), np.array(vals))
+    A, b = _request((24, 24, 1), 90, 1)
+    S = I.Server("batched", max_slots=4)
+    r_wide = S.submit(wide, np.ones((1, n)))
+    r_ok = S.submit(A, b, grid=(24, 24, 1))
+    got = _collect(S, [r_ok])
+    assert S.wait(r_wide, timeout_ms=60000) == r_wide
+    with pytest.raises(I.IccgError, match="16 entries"):
+        S.result(r_wide)
+    xd, _ = _direct(A, b, "mixed", dims=(24, 24, 1))
+    assert np.array_equal(got[r_ok][0], xd)
