@@ -292,7 +292,8 @@ struct iccg_server_s {
                               r->max_iter, r->tol, cfg.flags, &slot));
             owner[(size_t)slot] = {r, k};
           }
-        } catch (const Error &e) {
+        } catch (const std::exception &e) {
+          const Error *ie = dynamic_cast<const Error *>(&e);
           for (int s = 0; s < (int)owner.size(); ++s)
             if (owner[(size_t)s].first == r) {
               iccg_batch_remove(B, s);
@@ -302,7 +303,7 @@ struct iccg_server_s {
           std::lock_guard<std::mutex> lk(mu);
           free_slots += r->nrhs;
           running_reqs--;
-          complete_locked(r, e.status, e.what());
+          complete_locked(r, ie ? ie->status : ICCG_ERR_NOMEM, e.what());
         }
       }
       try {
@@ -333,6 +334,8 @@ struct iccg_server_s {
         }
       } catch (const Error &e) {
         fail_all_running(e.status, e.what());
+      } catch (const std::exception &e) {
+        fail_all_running(ICCG_ERR_NOMEM, e.what());
       }
     }
   }
