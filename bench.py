@@ -205,8 +205,9 @@ def cpu_baseline_systems(systems, tol, budget_s, label):
     log(f"reference picg_crs: {done} solves, {cg_s:.2f}s of CG on {threads} threads")
     return {"value": round(done / cg_s, 3), "unit": "solves/s", "cores": threads,
             "kind": "reference",
-            "sample": (f"the first {done} systems of the {label}, solved one after another by "
-                       f"the reference's picg_crs IC(0)-CG (oracle/_ref, OpenMP {threads} "
+            "sample": ((f"the first {done} systems of the {label}, solved one after another"
+                        if len(systems) > 1 else f"the {label}, solved once") +
+                       f" by the reference's picg_crs IC(0)-CG (oracle/_ref, OpenMP {threads} "
                        f"threads), FP64, tol {tol:g}; CG phase timed")}
 
 
@@ -395,6 +396,54 @@ def drive(server, reqs, arrivals):
     return out, time.perf_counter() - t0
 
 
+def main_mc128(args):
+    """picg_crs at the red-black configuration: 3D 7-point Laplacian 128^3
+    (n = 2,097,152), greedy (red-black) multicolor IC(0), FP64, b ~ N(0,1),
+    x0 = 0, tol 1e-8; time to tolerance with SpMV and sweep GB/s."""
+    import iccg_amd as I
+    N = 128
+    t0 = time.time()
+    A = I.gen_grid(N, N, N, seed=SEED)
+    Ap, sched, b = mc_system(A, SEED + 1)
+    n, nnz = Ap.n, Ap.nnz
+    nnz_l = (nnz + n) // 2
+    log(f"grid {N}^3 red-black: n={n} nnz={nnz} colors={sched.nc} ({time.time() - t0:.1f}s)")
+    dA = I.DeviceCSR(Ap)
+    M = I.IC0(dA, 0.0, sched=sched, precision="fp64")
+    S = I.Solver(dA, M)
+    S.set(b)
+    res, wall, prof = time_solves(S, args.steps, args.warmup, args.max_iter, args.tol, False)
+    ms = wall / args.steps * 1e3
+    r = res[-1]
+    k = kernel_table(S, n, nnz, nnz_l, False, sched.nc, prof.iters)
+    gbs = iter_bytes(n, nnz, nnz_l, False) * r.iters / (ms * 1e-3) / 1e9
+    log(f"fp64: {r.iters} iters, rel {r.rel_resid:.3e}, true {r.true_rel_resid:.3e}, "
+        f"{ms:.1f} ms/solve, {gbs:.0f} GB/s")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_systems([(Ap, sched, b)], args.tol, args.cpu_budget,
+                                   "128^3 red-black system")
+    dom = max(k.items(), key=lambda kv: kv[1]["avg_ms"] * kv[1]["count"])
+    line = {
+        "metric": METRIC, "value": round(ms / 1e3, 4), "unit": "s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": (f"picg_crs: 3D 7-pt Laplacian {N}^3 (n={n}, nnz={nnz}), greedy "
+                                f"red-black IC(0), FP64, b~N(0,1), x0=0, tol {args.tol:g}"),
+                   "n": n, "nnz": nnz, "tol": args.tol},
+        "iters": r.iters, "rel_resid": r.rel_resid, "true_rel_resid": r.true_rel_resid,
+        "hbm_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "kernels": k,
+        "roofline": {"bound": "hbm", "kernel": dom[0], "unit_of_launch": dom[1]["unit"],
+                     "achieved": dom[1]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom[1]["frac"], "traffic": None,
+                     "algorithmic_bytes_per_launch": dom[1]["bytes"],
+                     "avg_ms": dom[1]["avg_ms"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main_serve(args):
     """Serving workload: the same Poisson-arrival request stream through the
     batched request loop (mixed precision, continuous batching) and through
@@ -465,9 +514,10 @@ def main_serve(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["abmc192", "batch", "serve"], default="abmc192",
-                    help="abmc192: the single-system headline; batch: many systems at once; "
-                         "serve: the request loop")
+    ap.add_argument("--workload", choices=["abmc192", "mc128", "batch", "serve"],
+                    default="abmc192",
+                    help="abmc192: the single-system headline; mc128: red-black FP64 at 128^3; "
+                         "batch: many systems at once; serve: the request loop")
     ap.add_argument("--systems", type=int, default=512)
     ap.add_argument("--requests", type=int, default=64)
     ap.add_argument("--no-compare", action="store_true",
@@ -502,6 +552,8 @@ def main():
         return main_batch(args)
     if args.workload == "serve":
         return main_serve(args)
+    if args.workload == "mc128":
+        return main_mc128(args)
     N = args.size
     Ap, sched, b, info = build_problem(N, args.block)
     n, nnz = Ap.n, Ap.nnz
