@@ -320,6 +320,7 @@ static void build_schedule(iccg_precond_s &M, const iccg_sched_s *s, const std::
     nlf[kb] = mf;
     nlb[kb] = mb;
   }
+  sc.h_bstart = bstart;
   sc.d_bstart = dupload(bstart.data(), bstart.size(), st);
   if (!contiguous) sc.d_brows = dupload(brows.data(), brows.size(), st);
   sc.d_lev_f = dupload(levf.data(), levf.size(), st);
@@ -327,6 +328,56 @@ static void build_schedule(iccg_precond_s &M, const iccg_sched_s *s, const std::
   sc.d_nlev_f = dupload(nlf.data(), nlf.size(), st);
   sc.d_nlev_b = dupload(nlb.data(), nlb.size(), st);
   HIPCHK(hipStreamSynchronize(st));
+}
+
+// Section width for the ABMC sweeps' sectioned layout (sell_layout_sections),
+// or 0 for the plain layout: only for contiguous blocks of <= 64 rows, at
+// most 4 in-block and 4 outside entries per row (sweep instantiations 2..4),
+// and when the sections pad the factor by at most a quarter.  Fills each
+// row's block bounds and outside counts of L and U.
+static int abmc_sections(const DevSched &sc, int n, const std::vector<int> &lrp,
+                         const std::vector<int> &lci, const std::vector<int> &urp,
+                         const std::vector<int> &uci, std::vector<int> &blo, std::vector<int> &bhi,
+                         std::vector<int> &lout, std::vector<int> &uout) {
+  if (sc.kind != 2 || sc.d_brows != nullptr || sc.max_block > 64 || n == 0) return 0;
+  blo.assign((size_t)n, 0);
+  bhi.assign((size_t)n, 0);
+  const int nb = (int)sc.h_bstart.size() - 1;
+  for (int kb = 0; kb < nb; ++kb)
+    for (int i = sc.h_bstart[kb]; i < sc.h_bstart[kb + 1]; ++i) {
+      blo[(size_t)i] = sc.h_bstart[kb];
+      bhi[(size_t)i] = sc.h_bstart[kb + 1];
+    }
+  std::vector<int> lin((size_t)n), uin((size_t)n);
+  lout.assign((size_t)n, 0);
+  uout.assign((size_t)n, 0);
+  int S = 0;
+  for (int i = 0; i < n; ++i) {
+    for (int k = lrp[i]; k < lrp[i + 1] - 1; ++k) {
+      const bool in = lci[k] >= blo[(size_t)i] && lci[k] < bhi[(size_t)i];
+      (in ? lin : lout)[(size_t)i]++;
+    }
+    for (int k = urp[i] + 1; k < urp[i + 1]; ++k) {
+      const bool in = uci[k] >= blo[(size_t)i] && uci[k] < bhi[(size_t)i];
+      (in ? uin : uout)[(size_t)i]++;
+    }
+    S = std::max({S, lin[(size_t)i], lout[(size_t)i], uin[(size_t)i], uout[(size_t)i]});
+  }
+  if (S > 4) return 0;
+  S = std::max(S, 2);
+  long plain = 0, sect = 0;
+  for (int s0 = 0; s0 < n; s0 += 64) {
+    int wl = 0, wu = 0, ol = 0, ou = 0;
+    for (int i = s0; i < std::min(n, s0 + 64); ++i) {
+      wl = std::max(wl, lin[(size_t)i] + lout[(size_t)i]);
+      wu = std::max(wu, uin[(size_t)i] + uout[(size_t)i]);
+      ol = std::max(ol, lout[(size_t)i]);
+      ou = std::max(ou, uout[(size_t)i]);
+    }
+    plain += wl + wu;
+    sect += 2 * S + ol + ou;
+  }
+  return 4 * sect <= 5 * plain ? S : 0;
 }
 
 // Sliced-ELL copies of the strictly lower / strictly upper factor and its
@@ -342,13 +393,27 @@ static void build_sliced_factor(iccg_precond_s &M, const std::vector<int> &lrp,
   std::vector<int> loff, lcol, uoff, ucol;
   std::vector<int> l2ls((size_t)nnz), upos((size_t)nnz, -1), l2us((size_t)nnz);
   int lw = 0, uw = 0;
-  sell_layout(
-      n, [&](int i, int &b, int &e) { b = lrp[i]; e = lrp[i + 1] - 1; }, lci.data(), loff, lcol,
-      lw, &l2ls);
-  sell_layout(
-      n, [&](int i, int &b, int &e) { b = urp[i] + 1; e = urp[i + 1]; }, uci.data(), uoff, ucol,
-      uw, &upos);
+  auto lrange = [&](int i, int &b, int &e) {
+    b = lrp[i];
+    e = lrp[i + 1] - 1;
+  };
+  auto urange = [&](int i, int &b, int &e) {
+    b = urp[i] + 1;
+    e = urp[i + 1];
+  };
+  std::vector<int> blo, bhi, lout, uout;
+  const int S = abmc_sections(M.sc, n, lrp, lci, urp, uci, blo, bhi, lout, uout);
+  if (S > 0) {
+    sell_layout_sections(n, lrange, lci.data(), blo.data(), bhi.data(), lout, S, loff, lcol, lw,
+                         &l2ls);
+    sell_layout_sections(n, urange, uci.data(), blo.data(), bhi.data(), uout, S, uoff, ucol, uw,
+                         &upos);
+  } else {
+    sell_layout(n, lrange, lci.data(), loff, lcol, lw, &l2ls);
+    sell_layout(n, urange, uci.data(), uoff, ucol, uw, &upos);
+  }
   if (std::max(lw, uw) > 16) return;  // wide rows: CSR sweeps only
+  F.sec = S;
   for (int i = 0; i < n; ++i) l2ls[(size_t)lrp[i + 1] - 1] = -1 - i;
   for (long k = 0; k < nnz; ++k) l2us[k] = upos[(size_t)l2u[k]];
   auto up = [&](Sell &S, std::vector<int> &off, std::vector<int> &col, int w) {

@@ -120,6 +120,9 @@ struct Factor {
   int *l2ls = nullptr;   // L CSR position -> ls entry, or -1 - row (diagonal)
   int *l2us = nullptr;   // L CSR position -> us entry, or -1 (diagonal)
   bool sell = false;
+  // ABMC factors: ls / us in the sectioned layout with S in-block and up to
+  // S outside entries per row (sell_layout_sections), or 0 for the plain one
+  int sec = 0;
 };
 
 // ---- multi-system engine (batch.hip) --------------------------------------

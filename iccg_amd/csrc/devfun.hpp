@@ -168,6 +168,54 @@ static inline int slice_grid(long nslices) { return red_grid((nslices + 3) / 4);
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return fma(a, b, c); }
 
+// ---------------------------------------------------------------------------
+// a / d, correctly rounded, split into the part that depends on d alone
+// (rcp_refined, once per row) and the part that depends on a (div_steps,
+// once per use).  hipcc lowers a / d to: v_div_scale of the divisor and of
+// the numerator, v_rcp of the scaled divisor, Newton refinement of that
+// reciprocal (one step in f32, two in f64), quotient and remainder
+// corrections, v_div_fmas and v_div_fixup.  When |a| and |d| are normal and
+// lie in [2^-40, 2^40) (f32) or [2^-300, 2^300) (f64), v_div_scale leaves
+// both operands as they are (the exponent gap stays below 96 / 768 and
+// nothing approaches the denormal range) and clears the scale flag, so
+// v_div_fmas is a plain fma, and v_div_fixup returns the quotient unchanged:
+// the sequence is exactly rcp_refined followed by div_steps on the same
+// operands, hence the same bits.  div_range_ok tells whether an operand is
+// in that range.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool div_range_ok(float a) {
+  const unsigned u = __float_as_uint(a) & 0x7fffffffu;
+  return u - 0x2B800000u < 0x53800000u - 0x2B800000u;  // 2^-40 <= |a| < 2^40
+}
+__device__ __forceinline__ bool div_range_ok(double a) {
+  const unsigned u = (unsigned)((unsigned long long)__double_as_longlong(a) >> 32) & 0x7fffffffu;
+  return u - 0x2D300000u < 0x52B00000u - 0x2D300000u;  // 2^-300 <= |a| < 2^300
+}
+__device__ __forceinline__ float rcp_refined(float d) {
+  const float r = __builtin_amdgcn_rcpf(d);
+  return fmaf(fmaf(-d, r, 1.0f), r, r);
+}
+__device__ __forceinline__ double rcp_refined(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  return fma(r, fma(-d, r, 1.0), r);
+}
+__device__ __forceinline__ float div_steps(float a, float d, float y) {
+  const float m = a * y;
+  const float q = fmaf(fmaf(-d, m, a), y, m);
+  return fmaf(fmaf(-d, q, a), y, q);
+}
+__device__ __forceinline__ double div_steps(double a, double d, double y) {
+  const double m = a * y;
+  return fma(fma(-d, m, a), y, m);
+}
+// a numerator of +0 also gives the quotient's bits: every step above is
+// then an exact zero of the right sign
+template <class T>
+__device__ __forceinline__ bool div_steps_exact(T a, T d) {
+  return div_range_ok(d) && (div_range_ok(a) || (a == (T)0 && !signbit(a)));
+}
+
 // 4-element vector loads/stores (i multiple of 4; HBM buffers 256-B aligned)
 __device__ __forceinline__ void ld4(const double *p, int i, double (&o)[4]) {
   const double2 u = *reinterpret_cast<const double2 *>(p + i);
@@ -432,6 +480,169 @@ __device__ __forceinline__ YT abmc_bwd_block(int bs, int be, int nl,
   }
   if (act) z[i] = zi;
   return zi;
+}
+
+// ---------------------------------------------------------------------------
+// ABMC sweeps over the sectioned factor layout.  Every 64-row slice holds
+// the block's rows; entry k < S of a row is its k-th in-block entry
+// (columns in [bs, be), ascending, -1 past the last) and entry S + j its j-th
+// entry outside the block (ascending, -1 past the last): for L the earlier
+// colors' columns (< bs), for U the later colors' (>= be).  A row's chain is
+// the sequential reference's (ic0.cpp:263-339): forward, the outside terms
+// first, then the in-block terms; backward, the in-block terms first, then
+// the outside terms.
+//
+// The in-block levels run without masks: at every level every lane
+// recomputes its row from the LDS window and stores it.  A term whose entry
+// is absent reads the window's zero slot (64) with a zero coefficient, and
+// fma(-0, 0, acc) returns acc, so each lane performs the same operations in
+// the same order at every level.  After level L the rows of levels <= L are
+// final (their in-block inputs were final after level L - 1), so after the
+// block's nl levels every row is.  The division runs as rcp_refined once per
+// row and div_steps at each level; if a final numerator of a live row is
+// outside div_steps_exact's range, the block's levels run again with a / d.
+// ---------------------------------------------------------------------------
+constexpr int kSecWindow = 65;  // LDS slots per wavefront: 64 rows + the zero slot
+
+template <class YT, int S, bool OUT>
+__device__ __forceinline__ YT sec_levels(int nl, YT acc, YT d, const int (&ad)[S],
+                                         const YT (&va)[S], const YT (&vo)[S],
+                                         const YT (&xo)[S], YT *my, int lane, bool act) {
+  const YT dy = rcp_refined(d);
+  YT yi = (YT)0, a = acc;
+  for (int l = 0; l < nl; ++l) {
+    YT t[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) t[k] = my[ad[k]];
+    a = acc;
+#pragma unroll
+    for (int k = 0; k < S; ++k) a = fma_t(-va[k], t[k], a);
+    if (OUT) {
+#pragma unroll
+      for (int k = 0; k < S; ++k) a = fma_t(-vo[k], xo[k], a);
+    }
+    yi = div_steps(a, d, dy);
+    my[lane] = yi;
+    wave_sync();
+  }
+  if (!__all(!act || nl == 0 || div_steps_exact(a, d))) {
+    for (int l = 0; l < nl; ++l) {
+      YT t[S];
+#pragma unroll
+      for (int k = 0; k < S; ++k) t[k] = my[ad[k]];
+      a = acc;
+#pragma unroll
+      for (int k = 0; k < S; ++k) a = fma_t(-va[k], t[k], a);
+      if (OUT) {
+#pragma unroll
+        for (int k = 0; k < S; ++k) a = fma_t(-vo[k], xo[k], a);
+      }
+      yi = a / d;
+      my[lane] = yi;
+      wave_sync();
+    }
+  }
+  return yi;
+}
+
+// one block's entries: in-block slots and coefficients, outside columns and
+// coefficients (the row's slice is i >> 6, lane i & 63)
+template <class VT, class YT, int S>
+__device__ __forceinline__ void sec_load(int i, int bs, const int *__restrict__ off,
+                                         const int *__restrict__ col, const VT *__restrict__ v,
+                                         int (&ad)[S], YT (&va)[S], int (&co)[S], VT (&vo)[S]) {
+  const int s = i >> 6, o0 = off[s];
+  const int w = (off[s + 1] - o0) >> 6;
+  const int base = o0 + (i & 63);
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const int c = col[base + 64 * k];
+    ad[k] = c >= 0 ? c - bs : 64;
+    va[k] = c >= 0 ? (YT)v[base + 64 * k] : (YT)0;
+  }
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    co[k] = -1;
+    vo[k] = (VT)0;
+    if (S + k < w) {
+      co[k] = col[base + 64 * (S + k)];
+      vo[k] = v[base + 64 * (S + k)];
+    }
+  }
+}
+
+template <class VT, class YT, int S>
+__device__ __forceinline__ void abmc_fwd_sec(int bs, int be, int nl, const int *__restrict__ off,
+                                             const int *__restrict__ col,
+                                             const VT *__restrict__ lv,
+                                             const VT *__restrict__ diag,
+                                             const double *__restrict__ r, YT *__restrict__ y,
+                                             YT *my, int lane) {
+  const int i = bs + lane;
+  const bool act = i < be;
+  int ad[S], co[S];
+  YT va[S], none[S];
+  VT vo[S];
+  YT acc = (YT)0, d = (YT)1;
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    ad[k] = 64;
+    va[k] = none[k] = (YT)0;
+  }
+  if (act) {
+    sec_load<VT, YT, S>(i, bs, off, col, lv, ad, va, co, vo);
+    d = (YT)diag[i];
+    acc = (YT)r[i];
+    YT xo[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) xo[k] = co[k] >= 0 ? y[co[k]] : (YT)0;
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+      if (co[k] >= 0) acc = fma_t(-(YT)vo[k], xo[k], acc);
+  }
+  if (lane == 0) my[64] = (YT)0;
+  wave_sync();
+  const YT yi = sec_levels<YT, S, false>(nl, acc, d, ad, va, none, none, my, lane, act);
+  if (act) y[i] = yi;
+}
+
+// returns z_i (0 on lanes past the block); with r, also r_i in ri
+template <class VT, class YT, int S>
+__device__ __forceinline__ YT abmc_bwd_sec(int bs, int be, int nl, const int *__restrict__ off,
+                                           const int *__restrict__ col, const VT *__restrict__ uv,
+                                           const VT *__restrict__ diag,
+                                           const YT *__restrict__ y, YT *__restrict__ z,
+                                           const double *__restrict__ r, double &ri, YT *my,
+                                           int lane) {
+  const int i = bs + lane;
+  const bool act = i < be;
+  int ad[S], co[S];
+  YT va[S], xv[S], xo[S];
+  VT vo[S];
+  YT acc = (YT)0, d = (YT)1;
+  ri = 0.0;
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    ad[k] = 64;
+    va[k] = xv[k] = xo[k] = (YT)0;
+  }
+  if (act) {
+    sec_load<VT, YT, S>(i, bs, off, col, uv, ad, va, co, vo);
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+      if (co[k] >= 0) {
+        xo[k] = z[co[k]];
+        xv[k] = (YT)vo[k];
+      }
+    d = (YT)diag[i];
+    acc = y[i];
+    if (r) ri = r[i];
+  }
+  if (lane == 0) my[64] = (YT)0;
+  wave_sync();
+  const YT zi = sec_levels<YT, S, true>(nl, acc, d, ad, va, xv, xo, my, lane, act);
+  if (act) z[i] = zi;
+  return act ? zi : (YT)0;
 }
 
 }  // namespace iccg

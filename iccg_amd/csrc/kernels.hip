@@ -568,6 +568,57 @@ __global__ __launch_bounds__(kWG) void k_abmc_bwd_wave(
   if (RZ) block_partial<1>(h, l, part, gidx0 + blockIdx.x);
 }
 
+// ABMC sweeps over the sectioned factor layout (abmc_fwd_sec /
+// abmc_bwd_sec): one wavefront per block, the block loop wave-uniform so the
+// block bounds come from scalar loads.
+template <class VT, class YT, int SW>
+__global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
+                                                      const int *__restrict__ bstart,
+                                                      const int *__restrict__ nlev,
+                                                      const int *__restrict__ off,
+                                                      const int *__restrict__ col,
+                                                      const VT *__restrict__ lv,
+                                                      const VT *__restrict__ diag,
+                                                      const double *__restrict__ r,
+                                                      YT *__restrict__ y, const Scal *S) {
+  __shared__ YT win[(kWG / 64) * kSecWindow];
+  if (S && S->done) return;
+  const int lane = threadIdx.x & 63;
+  YT *my = win + (threadIdx.x >> 6) * kSecWindow;
+  const WaveRange wr = xcd_waves(kb0, kb1);
+  const int first = __builtin_amdgcn_readfirstlane(wr.first);
+  const int step = __builtin_amdgcn_readfirstlane(wr.step);
+  const int end = __builtin_amdgcn_readfirstlane(wr.end);
+  for (int kb = first; kb < end; kb += step)
+    abmc_fwd_sec<VT, YT, SW>(bstart[kb], bstart[kb + 1], nlev[kb], off, col, lv, diag, r, y, my,
+                             lane);
+}
+
+template <class VT, class YT, int SW, bool RZ>
+__global__ __launch_bounds__(kWG) void k_abmc_bwd_sec(
+    int kb0, int kb1, const int *__restrict__ bstart, const int *__restrict__ nlev,
+    const int *__restrict__ off, const int *__restrict__ col, const VT *__restrict__ uv,
+    const VT *__restrict__ diag, const YT *__restrict__ y, YT *__restrict__ z,
+    const double *__restrict__ r, Scal *S, double *part, int gidx0) {
+  __shared__ YT win[(kWG / 64) * kSecWindow];
+  if (S && S->done) return;
+  const int lane = threadIdx.x & 63;
+  YT *my = win + (threadIdx.x >> 6) * kSecWindow;
+  double h[1] = {0}, l[1] = {0};
+  const WaveRange wr = xcd_waves(kb0, kb1);
+  const int first = __builtin_amdgcn_readfirstlane(wr.first);
+  const int step = __builtin_amdgcn_readfirstlane(wr.step);
+  const int end = __builtin_amdgcn_readfirstlane(wr.end);
+  for (int kb = first; kb < end; kb += step) {
+    const int bs = bstart[kb], be = bstart[kb + 1];
+    double ri;
+    const YT zi = abmc_bwd_sec<VT, YT, SW>(bs, be, nlev[kb], off, col, uv, diag, y, z,
+                                           RZ ? r : nullptr, ri, my, lane);
+    if (RZ && bs + lane < be) dd_fma(h[0], l[0], ri, (double)zi);
+  }
+  if (RZ) block_partial<1>(h, l, part, gidx0 + blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // small elementwise kernels
 // ---------------------------------------------------------------------------
@@ -899,6 +950,57 @@ static void bwd_sliced(const DevSched &sc, const Factor &F, const VT *uv, const 
   if (fuse) hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(kFinWG), 0, st, part, total, S, rz == 2);
 }
 
+template <class VT, class YT, int SW>
+static void fwd_sec(const DevSched &sc, const Factor &F, const VT *lv, const VT *dg,
+                    const double *r, YT *y, const Scal *S, hipStream_t st) {
+  for (int c = 0; c < sc.nc; ++c) {
+    const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
+    if (b1 <= b0) continue;
+    hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW>), dim3(abmc_grid<VT>(b1 - b0)), dim3(kWG), 0,
+                       st, b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.col, lv, dg, r, y, S);
+  }
+}
+
+template <class VT, class YT, int SW>
+static void bwd_sec(const DevSched &sc, const Factor &F, const VT *uv, const VT *dg, const YT *y,
+                    YT *z, const double *r, Scal *S, double *part, int rz, hipStream_t st) {
+  std::vector<int> grids;  // workgroups of every color launch, for the r.z partials
+  int total = 0;
+  for (int t = 0; t < sc.nc; ++t) {
+    const int c = sc.nc - 1 - t;
+    const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
+    grids.push_back(b1 > b0 ? abmc_grid<VT>(b1 - b0) : 0);
+    total += grids.back();
+  }
+  const bool fuse = rz != 0 && r != nullptr && fuse_rz_enabled();
+  int gidx0 = 0;
+  for (int t = 0; t < sc.nc; ++t) {
+    const int c = sc.nc - 1 - t;
+    const int g = grids[t];
+    if (g == 0) continue;
+    const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
+    if (fuse)
+      hipLaunchKernelGGL((k_abmc_bwd_sec<VT, YT, SW, true>), dim3(g), dim3(kWG), 0, st, b0, b1,
+                         sc.d_bstart, sc.d_nlev_b, F.us.off, F.us.col, uv, dg, y, z, r, S, part,
+                         gidx0);
+    else
+      hipLaunchKernelGGL((k_abmc_bwd_sec<VT, YT, SW, false>), dim3(g), dim3(kWG), 0, st, b0, b1,
+                         sc.d_bstart, sc.d_nlev_b, F.us.off, F.us.col, uv, dg, y, z, r, S, part,
+                         gidx0);
+    gidx0 += g;
+  }
+  if (fuse) hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(kFinWG), 0, st, part, total, S, rz == 2);
+}
+
+#define DISPATCH_SEC(sw, FN, VT, YT, ...)        \
+  do {                                           \
+    switch (sw) {                                \
+      case 2: FN<VT, YT, 2>(__VA_ARGS__); break; \
+      case 3: FN<VT, YT, 3>(__VA_ARGS__); break; \
+      default: FN<VT, YT, 4>(__VA_ARGS__); break; \
+    }                                            \
+  } while (0)
+
 #define DISPATCH_SWEEP_WM(wm, FN, VT, YT, ...)   \
   do {                                           \
     switch (wm) {                                \
@@ -910,6 +1012,13 @@ static void bwd_sliced(const DevSched &sc, const Factor &F, const VT *uv, const 
 
 void launch_fwd(const DevSched &sc, const Factor &F, bool mixed, const double *r, void *y,
                 const Scal *S, hipStream_t st) {
+  if (sliced_plan(sc, F) && sc.kind == 2 && F.sec > 0) {
+    if (mixed)
+      DISPATCH_SEC(F.sec, fwd_sec, float, float, sc, F, F.ls.v32, F.d32, r, (float *)y, S, st);
+    else
+      DISPATCH_SEC(F.sec, fwd_sec, double, double, sc, F, F.ls.v64, F.d64, r, (double *)y, S, st);
+    return;
+  }
   if (sliced_plan(sc, F)) {
     const int wm = sweep_wm(F);
     if (mixed)
@@ -931,6 +1040,15 @@ void launch_fwd(const DevSched &sc, const Factor &F, bool mixed, const double *r
 
 void launch_bwd(const DevSched &sc, const Factor &F, bool mixed, const void *y, void *z,
                 const Scal *S, hipStream_t st, const double *r, Scal *Sw, double *part, int rz) {
+  if (sliced_plan(sc, F) && sc.kind == 2 && F.sec > 0) {
+    if (mixed)
+      DISPATCH_SEC(F.sec, bwd_sec, float, float, sc, F, F.us.v32, F.d32, (const float *)y,
+                   (float *)z, r, Sw, part, rz, st);
+    else
+      DISPATCH_SEC(F.sec, bwd_sec, double, double, sc, F, F.us.v64, F.d64, (const double *)y,
+                   (double *)z, r, Sw, part, rz, st);
+    return;
+  }
   if (sliced_plan(sc, F)) {
     const int wm = sweep_wm(F);
     if (mixed)
