@@ -528,6 +528,42 @@ void orc_ic0_apply(const orc_ic0f *f, const orc_sched *s, const double *r,
   orc_ic0_apply(f, NULL, r, z, y);
 }
 
+/* Mixed-precision apply: the FP32 sweeps of the precision split (factor and
+ * y, z in float).  Pinned against an exact-rational float32 restatement in
+ * tests/test_oracle.py. */
+void orc_ic0_apply_f32(const orc_ic0f *f, const double *r, double *z) {
+  const int n = f->n;
+  float *y = (float *)malloc(sizeof(float) * (size_t)(n > 0 ? n : 1));
+  float *zf = (float *)malloc(sizeof(float) * (size_t)(n > 0 ? n : 1));
+  for (int i = 0; i < n; ++i) {
+    float acc = (float)r[i], di = 1.0f;
+    for (int k = f->Lrp[i]; k < f->Lrp[i + 1]; ++k) {
+      int j = f->Lci[k];
+      if (j < i)
+        acc = fmaf(-(float)f->Lv[k], y[j], acc);
+      else if (j == i)
+        di = (float)f->Lv[k];
+      else
+        break;
+    }
+    y[i] = acc / di;
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    float acc = y[i], di = 1.0f;
+    for (int k = f->LTcp[i]; k < f->LTcp[i + 1]; ++k) {
+      int j = f->LTri[k];
+      if (j > i)
+        acc = fmaf(-(float)f->LTv[k], zf[j], acc);
+      else if (j == i)
+        di = (float)f->LTv[k];
+    }
+    zf[i] = acc / di;
+  }
+  for (int i = 0; i < n; ++i) z[i] = (double)zf[i];
+  free(y);
+  free(zf);
+}
+
 /* ------------------------------------------------------------------------ */
 /* L3: Krylov                                                                */
 /* ------------------------------------------------------------------------ */

@@ -100,6 +100,13 @@ typedef struct {
 void orc_ic0_apply(const orc_ic0f *f, const orc_sched *sched, const double *r,
                    double *z, double *y_scratch);
 
+/* The mixed-precision preconditioner apply (no counterpart in the
+ * reference, which solves in FP64 only): the factor's values and diagonal
+ * rounded to float, r rounded to float, the sequential forward and backward
+ * sweeps of ic0.cpp:153-184 in float with fmaf and float division, z
+ * widened to double.  Any valid MC / ABMC order gives the same bits. */
+void orc_ic0_apply_f32(const orc_ic0f *f, const double *r, double *z);
+
 /* ---- L3: Krylov (cg_crs.cpp) -------------------------------------------- */
 
 /* dot with the reference's 80-bit accumulation (cg_crs.cpp:33-40). */

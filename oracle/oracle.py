@@ -280,6 +280,15 @@ def ic0_apply(f: IC0Factor, r, sched: Schedule | None = None):
     return z
 
 
+def ic0_apply_f32(f: IC0Factor, r):
+    """The mixed-precision apply (FP32 sweeps, sequential order)."""
+    r = f64(r)
+    z = np.zeros(f.n)
+    fs = _ic0f_struct(f)
+    lib().orc_ic0_apply_f32(C.byref(fs), _ptr(r), _ptr(z))
+    return z
+
+
 # ---------------------------------------------------------------- L3 Krylov
 def dot_ld(a, b):
     a, b = f64(a), f64(b)

@@ -121,6 +121,59 @@ def test_abmc_grid_boxes():
     assert np.array_equal(M.apply(r, o.sched), O.ic0_apply(f, r, None))
 
 
+@pytest.mark.parametrize("name", ["lap2d_30", "var3d_9", "var2d_17"])
+@pytest.mark.parametrize("order", ["natural", "mc", "abmc", "boxes"])
+def test_ic0_apply_mixed_bitwise(name, order):
+    """FP32 factor and sweeps: the GPU apply equals the oracle's FP32 apply
+    bit for bit in every order (boxes: the sectioned ABMC layout)."""
+    A = _cases()[name]
+    if order == "natural":
+        Ar, s = A, None
+    elif order == "mc":
+        o = P.mc_order(A)
+        Ar, s = o.A, o.sched
+    elif order == "abmc":
+        o = P.abmc_order(A, block=8)
+        Ar, s = o.A, o.sched
+    else:
+        dims = {"lap2d_30": (30, 30, 1), "var3d_9": (9, 8, 7), "var2d_17": (17, 13, 1)}[name]
+        box = (4, 4, 4) if dims[2] > 1 else (8, 8, 1)
+        o = P.abmc_order(A, grid_dims=dims, box=box)
+        Ar, s = o.A, o.sched
+    f = O.ic0_factor(P.orc(Ar), 0.0)
+    M = I.IC0(Ar, 0.0, sched=s, precision="mixed")
+    r = I.gen_normal(Ar.n, 6)
+    assert np.array_equal(M.apply(r, s), O.ic0_apply_f32(f, r))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_sectioned_sweeps_out_of_range_numerators(precision):
+    """Right-hand sides spread over many decades, with +0 and -0 entries:
+    numerators outside the range where the hoisted division reproduces
+    a / d send their blocks through the exact-division pass; the result
+    still equals the sequential oracle bit for bit."""
+    A = P.grid(16, 12, 8, coef=(-2.0, 2.0), shift=1e-6)
+    o = P.abmc_order(A, grid_dims=(16, 12, 8), box=(4, 4, 4))
+    f = O.ic0_factor(P.orc(o.A), 0.0)
+    M = I.IC0(o.A, 0.0, sched=o.sched, precision=precision)
+    rng = np.random.default_rng(11)
+    span = 120 if precision == "fp64" else 18
+    ref = O.ic0_apply if precision == "fp64" else O.ic0_apply_f32
+    r = rng.standard_normal(o.A.n) * 10.0 ** rng.uniform(-span, span, o.A.n)
+    r[rng.choice(o.A.n, 40, replace=False)] = 0.0
+    r[rng.choice(o.A.n, 40, replace=False)] = -0.0
+    z = M.apply(r, o.sched)
+    assert np.array_equal(z, ref(f, r))
+    assert np.array_equal(np.signbit(z), np.signbit(ref(f, r)))
+    # one out-of-range row among ordinary ones
+    r2 = rng.standard_normal(o.A.n)
+    r2[77] = 1e-200 if precision == "fp64" else 1e-30
+    assert np.array_equal(M.apply(r2, o.sched), ref(f, r2))
+    # an all-zero right-hand side gives +0 everywhere
+    z0 = M.apply(np.zeros(o.A.n), o.sched)
+    assert np.array_equal(z0, ref(f, np.zeros(o.A.n))) and not np.signbit(z0).any()
+
+
 def _solve(A, kind, sched=None, b=None, tol=1e-8, max_iter=10000, precision="fp64"):
     if b is None:
         b = np.ones(A.n)

@@ -310,6 +310,64 @@ def test_ic0_apply_vs_triangular_solves():
     assert np.array_equal(O.ic0_apply(fo, r, P.orc_sched(o.sched)), O.ic0_apply(fo, r))
 
 
+def _round_f32(x):
+    """Nearest float32 to the rational x, ties to even (normal range)."""
+    if x == 0:
+        return Fraction(0)
+    a = abs(x)
+    e = a.numerator.bit_length() - a.denominator.bit_length()
+    while a >= Fraction(2) ** (e + 1):
+        e += 1
+    while a < Fraction(2) ** e:
+        e -= 1
+    assert -126 <= e <= 127
+    m = a / Fraction(2) ** (e - 23)  # in [2^23, 2^24)
+    q, rem = divmod(m.numerator, m.denominator)
+    half = Fraction(rem, m.denominator) - Fraction(1, 2)
+    if half > 0 or (half == 0 and q % 2 == 1):
+        q += 1
+    return (1 if x > 0 else -1) * q * Fraction(2) ** (e - 23)
+
+
+def _apply_f32_exact(f, r):
+    """The FP32 sweeps in exact rationals, every operation rounded once to
+    float32: fmaf(a, b, c) = round(a b + c), round(a / d)."""
+    rd = _round_f32
+    y = [Fraction(0)] * f.n
+    for i in range(f.n):
+        acc, di = rd(Fraction(float(r[i]))), Fraction(1)
+        for k in range(f.Lrp[i], f.Lrp[i + 1]):
+            j, v = int(f.Lci[k]), rd(Fraction(float(f.Lv[k])))
+            if j < i:
+                acc = rd(-v * y[j] + acc)
+            elif j == i:
+                di = v
+        y[i] = rd(acc / di)
+    z = [Fraction(0)] * f.n
+    for i in reversed(range(f.n)):
+        acc, di = y[i], Fraction(1)
+        for k in range(f.LTcp[i], f.LTcp[i + 1]):
+            j, v = int(f.LTri[k]), rd(Fraction(float(f.LTv[k])))
+            if j > i:
+                acc = rd(-v * z[j] + acc)
+            elif j == i:
+                di = v
+        z[i] = rd(acc / di)
+    return np.array([float(v) for v in z])
+
+
+def test_ic0_apply_f32_exact_rational():
+    """The mixed-precision apply equals float32 arithmetic done exactly and
+    rounded once per operation; FP64 arithmetic does not."""
+    G = P.grid(6, 5, 4, coef=(-2.0, 2.0), shift=1e-6)
+    f = O.ic0_factor(P.orc(G))
+    r = P.I.gen_normal(G.n, 4)
+    z = O.ic0_apply_f32(f, r)
+    assert np.array_equal(z, _apply_f32_exact(f, r))
+    assert np.all(z.astype(np.float32).astype(np.float64) == z)
+    assert not np.array_equal(z, O.ic0_apply(f, r))
+
+
 def test_pivot_errors():
     A = O.Csr(2, np.array([0, 2, 4], np.int32), np.array([0, 1, 0, 1], np.int32),
               np.array([1.0, 2.0, 2.0, 1.0]))
