@@ -396,6 +396,89 @@ __device__ __forceinline__ void abmc_bwd_group(int bs, int be, int nl, const Sys
   }
 }
 
+// ABMC blocks of a factor in the sectioned layout (sell_layout_sections,
+// section width D0.sec <= SW): the single-system sweeps' mask-free level
+// loop (sec_levels) run for every live member on the shared entries.  my
+// holds one kSecWindow window per member (row slots and the zero slot).
+template <class FT, int SW, int KM, bool FWD>
+__device__ __forceinline__ void abmc_group_sec(int bs, int be, int nl, const SysDesc &D0,
+                                               const GroupDesc &g,
+                                               const SysDesc *__restrict__ desc,
+                                               const bool (&live)[KM], FT *my, int lane) {
+  const int i = bs + lane;
+  const bool act = i < be;
+  const int sec = D0.sec;
+  int ad[SW], co[SW];
+  FT va[SW], vo[SW];
+  FT acc[KM], xo[KM][SW], d = (FT)1;
+#pragma unroll
+  for (int k = 0; k < SW; ++k) {
+    ad[k] = 64;
+    co[k] = -1;
+    va[k] = vo[k] = (FT)0;
+  }
+#pragma unroll
+  for (int j = 0; j < KM; ++j) {
+    acc[j] = (FT)0;
+#pragma unroll
+    for (int k = 0; k < SW; ++k) xo[j][k] = (FT)0;
+  }
+  if (act) {
+    const int *off = FWD ? D0.l_off : D0.u_off;
+    const int *col = FWD ? D0.l_col : D0.u_col;
+    const FT *fv = (const FT *)(FWD ? D0.l_v : D0.u_v);
+    const int s = i >> 6, o0 = off[s];
+    const int w = (off[s + 1] - o0) >> 6;
+    const int base = o0 + (i & 63);
+#pragma unroll
+    for (int k = 0; k < SW; ++k) {
+      if (k < sec) {
+        const int c = col[base + 64 * k];
+        if (c >= 0) {
+          ad[k] = c - bs;
+          va[k] = fv[base + 64 * k];
+        }
+      }
+      if (sec + k < w) {
+        co[k] = col[base + 64 * (sec + k)];
+        vo[k] = fv[base + 64 * (sec + k)];
+      }
+    }
+    d = ((const FT *)D0.d)[i];
+#pragma unroll
+    for (int j = 0; j < KM; ++j) {
+      if (!live[j]) continue;
+      const SysDesc &D = desc[g.slot[j]];
+      if (FWD) {
+        const FT *y = (const FT *)D.y;
+        FT a = (FT)D.r[i];
+        FT t[SW];
+#pragma unroll
+        for (int k = 0; k < SW; ++k) t[k] = co[k] >= 0 ? y[co[k]] : (FT)0;
+#pragma unroll
+        for (int k = 0; k < SW; ++k)
+          if (co[k] >= 0) a = fma_t(-vo[k], t[k], a);
+        acc[j] = a;
+      } else {
+        const FT *z = (const FT *)D.z;
+#pragma unroll
+        for (int k = 0; k < SW; ++k)
+          if (co[k] >= 0) xo[j][k] = z[co[k]];
+        acc[j] = ((const FT *)D.y)[i];
+      }
+    }
+  }
+  if (lane < KM) my[kSecWindow * lane + 64] = (FT)0;
+  wave_sync();
+#pragma unroll
+  for (int j = 0; j < KM; ++j) {
+    if (!live[j]) continue;
+    const FT zi = sec_levels<FT, SW, !FWD>(nl, acc[j], d, ad, va, vo, xo[j],
+                                                 my + kSecWindow * j, lane, act);
+    if (act) ((FT *)(FWD ? desc[g.slot[j]].y : desc[g.slot[j]].z))[i] = zi;
+  }
+}
+
 template <class FT, int WM, int KM, bool FWD>
 __device__ __forceinline__ void abmc_item(const GroupDesc &g, const SysDesc *__restrict__ desc,
                                           const bool (&live)[KM], const ColorItem &c, int lane,
@@ -441,6 +524,38 @@ __global__ __launch_bounds__(kWG) void kb_sweep(const GroupDesc *__restrict__ gr
       mc_item<float, WM, KM, FWD>(g, desc, live, c, lane);
     else
       mc_item<double, WM, KM, FWD>(g, desc, live, c, lane);
+  }
+}
+
+// one wavefront per color item of a sectioned-layout ABMC group
+template <int SW, int KM, bool FWD>
+__global__ __launch_bounds__(kWG) void kb_sweep_sec(const GroupDesc *__restrict__ groups,
+                                                    const SysDesc *__restrict__ desc,
+                                                    const ColorItem *__restrict__ items,
+                                                    int nitems, const SysScal *__restrict__ sc) {
+  // per wavefront: KM windows of kSecWindow (64 rows and the zero slot)
+  __shared__ double lds[(kWG / 64) * kSecWindow * KM];
+  const int wv = blockIdx.x * (kWG >> 6) + (threadIdx.x >> 6);
+  if (wv >= nitems) return;
+  const ColorItem c = items[wv];
+  const GroupDesc g = group_of<KM>(groups, c);
+  bool live[KM], any = false;
+#pragma unroll
+  for (int j = 0; j < KM; ++j) {
+    live[j] = j < g.k && !sc[g.slot[j]].done;
+    any |= live[j];
+  }
+  if (!any) return;
+  const SysDesc &D0 = desc[g.slot[0]];
+  const int lane = threadIdx.x & 63;
+  double *my = lds + (threadIdx.x >> 6) * kSecWindow * KM;
+  for (int kb = c.a0; kb < c.a1; ++kb) {
+    const int bs = D0.bstart[kb], be = D0.bstart[kb + 1];
+    const int nl = FWD ? D0.nlev_f[kb] : D0.nlev_b[kb];
+    if (D0.mixed)
+      abmc_group_sec<float, SW, KM, FWD>(bs, be, nl, D0, g, desc, live, (float *)my, lane);
+    else
+      abmc_group_sec<double, SW, KM, FWD>(bs, be, nl, D0, g, desc, live, my, lane);
   }
 }
 
@@ -556,6 +671,10 @@ inline size_t align_up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
 struct ItemSet {
   int ngroups = 0, nchunks = 0, nfin = 0, ncolors = 0, kmax = 1;
   std::vector<int> color_off;  // ncolors + 1 offsets into d_color
+  // per color: the leading items of sectioned-layout factors (kb_sweep_sec),
+  // and their largest section width
+  std::vector<int> color_sec;
+  int secw = 0;
   std::vector<int> work_of;    // per listed slot (fin order): its group's chunks
   char *buf = nullptr;
   GroupDesc *d_groups = nullptr;
@@ -696,8 +815,8 @@ void build_items(iccg_batch_s &B, const std::vector<int> &ids, ItemSet &I, int c
   }
   std::vector<int> gchunk0, gchunks;
   std::vector<ChunkItem> chunks;
-  std::vector<std::vector<ColorItem>> color;
-  int kmax = 1;
+  std::vector<std::vector<ColorItem>> color, color_sec;
+  int kmax = 1, secw = 0;
   for (int gi = 0; gi < (int)groups.size(); ++gi) {
     const GroupDesc &g = groups[(size_t)gi];
     kmax = std::max(kmax, g.k);
@@ -708,12 +827,18 @@ void build_items(iccg_batch_s &B, const std::vector<int> &ids, ItemSet &I, int c
     gchunks.push_back((int)chunks.size() - gchunk0.back());
     const DevSched &sc = s.M->sc;
     const int nc = sc.nc;
-    if ((int)color.size() < nc) color.resize(nc);
+    if ((int)color.size() < nc) {
+      color.resize(nc);
+      color_sec.resize(nc);
+    }
+    const int sec = sc.kind == 2 ? s.M->F.sec : 0;
+    secw = std::max(secw, sec);
     for (int c = 0; c < nc; ++c) {
       if (sc.kind == 2) {
         const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
         for (int kb = b0; kb < b1; kb += cw)
-          color[c].push_back({gi, kb, std::min(b1, kb + cw), 0, 0, g.slot[0], {0, 0}});
+          (sec > 0 ? color_sec : color)[c].push_back(
+              {gi, kb, std::min(b1, kb + cw), 0, 0, g.slot[0], {0, 0}});
       } else {
         const int lo = sc.h_color_ptr_v[c], hi = sc.h_color_ptr_v[c + 1];
         if (hi <= lo) continue;
@@ -735,8 +860,14 @@ void build_items(iccg_batch_s &B, const std::vector<int> &ids, ItemSet &I, int c
   I.nfin = (int)fin.size();
   I.ncolors = (int)color.size();
   I.kmax = kmax;
+  I.secw = secw;
   I.color_off.assign(1, 0);
-  for (auto &v : color) I.color_off.push_back(I.color_off.back() + (int)v.size());
+  I.color_sec.clear();
+  for (size_t c = 0; c < color.size(); ++c) {
+    I.color_sec.push_back((int)color_sec[c].size());
+    color[c].insert(color[c].begin(), color_sec[c].begin(), color_sec[c].end());
+    I.color_off.push_back(I.color_off.back() + (int)color[c].size());
+  }
   if (I.nfin == 0) return;
   // one device buffer: groups | chunks | fin | color items
   const size_t o_chunks = align_up(groups.size() * sizeof(GroupDesc));
@@ -808,23 +939,46 @@ void launch_resid_items(iccg_batch_s &B, const ItemSet &I, int mode) {
                      B.d_desc, B.d_sc, mode);
 }
 
-void launch_sweeps(iccg_batch_s &B, const ItemSet &I) {
-  dim3 b(kWG);
-  mark(B, T_FWD, true);
-  for (int c = 0; c < I.ncolors; ++c) {
-    const int ni = I.color_off[c + 1] - I.color_off[c];
-    if (ni)
-      BATCH_DISPATCH(B.wm, I.kmax, kb_sweep, FWD_T, dim3((ni + 3) / 4), b, 0, B.st, I.d_groups,
-                     B.d_desc, I.d_color + I.color_off[c], ni, B.d_sc);
+// sectioned-layout items: section width class x group size class
+#define SEC_DISPATCH(sw, km, K, EXTRA, ...)                                                   \
+  do {                                                                                        \
+    const int _s = (sw) <= 2 ? 2 : (sw) == 3 ? 3 : 4, _k = km_class(km);                      \
+    if (_s == 2 && _k == 1) hipLaunchKernelGGL((K<2, 1 EXTRA>), __VA_ARGS__);                 \
+    else if (_s == 2) hipLaunchKernelGGL((K<2, kMaxRhs EXTRA>), __VA_ARGS__);                 \
+    else if (_s == 3 && _k == 1) hipLaunchKernelGGL((K<3, 1 EXTRA>), __VA_ARGS__);            \
+    else if (_s == 3) hipLaunchKernelGGL((K<3, kMaxRhs EXTRA>), __VA_ARGS__);                 \
+    else if (_k == 1) hipLaunchKernelGGL((K<4, 1 EXTRA>), __VA_ARGS__);                       \
+    else hipLaunchKernelGGL((K<4, kMaxRhs EXTRA>), __VA_ARGS__);                              \
+  } while (0)
+
+template <bool FWD>
+static void launch_color(iccg_batch_s &B, const ItemSet &I, int c) {
+  const int n0 = I.color_sec[c], ni = I.color_off[c + 1] - I.color_off[c];
+  const ColorItem *items = I.d_color + I.color_off[c];
+  if (n0 > 0) {
+    if (FWD)
+      SEC_DISPATCH(I.secw, I.kmax, kb_sweep_sec, FWD_T, dim3((n0 + 3) / 4), dim3(kWG), 0, B.st,
+                   I.d_groups, B.d_desc, items, n0, B.d_sc);
+    else
+      SEC_DISPATCH(I.secw, I.kmax, kb_sweep_sec, BWD_T, dim3((n0 + 3) / 4), dim3(kWG), 0, B.st,
+                   I.d_groups, B.d_desc, items, n0, B.d_sc);
   }
+  if (ni > n0) {
+    if (FWD)
+      BATCH_DISPATCH(B.wm, I.kmax, kb_sweep, FWD_T, dim3((ni - n0 + 3) / 4), dim3(kWG), 0, B.st,
+                     I.d_groups, B.d_desc, items + n0, ni - n0, B.d_sc);
+    else
+      BATCH_DISPATCH(B.wm, I.kmax, kb_sweep, BWD_T, dim3((ni - n0 + 3) / 4), dim3(kWG), 0, B.st,
+                     I.d_groups, B.d_desc, items + n0, ni - n0, B.d_sc);
+  }
+}
+
+void launch_sweeps(iccg_batch_s &B, const ItemSet &I) {
+  mark(B, T_FWD, true);
+  for (int c = 0; c < I.ncolors; ++c) launch_color<true>(B, I, c);
   mark(B, T_FWD, false);
   mark(B, T_BWD, true);
-  for (int c = I.ncolors - 1; c >= 0; --c) {
-    const int ni = I.color_off[c + 1] - I.color_off[c];
-    if (ni)
-      BATCH_DISPATCH(B.wm, I.kmax, kb_sweep, BWD_T, dim3((ni + 3) / 4), b, 0, B.st, I.d_groups,
-                     B.d_desc, I.d_color + I.color_off[c], ni, B.d_sc);
-  }
+  for (int c = I.ncolors - 1; c >= 0; --c) launch_color<false>(B, I, c);
   mark(B, T_BWD, false);
 }
 
@@ -1216,6 +1370,7 @@ int iccg_batch_add(iccg_batch_t B, iccg_csr_t A, iccg_precond_t M, const double 
       D.lev_b = M->sc.d_lev_b;
       D.nlev_f = M->sc.d_nlev_f;
       D.nlev_b = M->sc.d_nlev_b;
+      D.sec = M->F.sec;
     }
     D.x = s.x;
     D.r = s.r;

@@ -143,6 +143,7 @@ struct SysDesc {
   // ABMC blocks (nullptr for a multicolor system)
   const int *bstart = nullptr, *nlev_f = nullptr, *nlev_b = nullptr;
   const uint16_t *lev_f = nullptr, *lev_b = nullptr;
+  int sec = 0;  // section width of the factor's sectioned layout, 0 = plain
   double *x = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr;
   const double *b = nullptr;
   void *y = nullptr, *z = nullptr;
