@@ -337,14 +337,15 @@ static void build_schedule(iccg_precond_s &M, const iccg_sched_s *s, const std::
 // row's block bounds and outside counts of L and U.
 static int abmc_sections(const DevSched &sc, int n, const std::vector<int> &lrp,
                          const std::vector<int> &lci, const std::vector<int> &urp,
-                         const std::vector<int> &uci, std::vector<int> &blo, std::vector<int> &bhi,
-                         std::vector<int> &lout, std::vector<int> &uout) {
+                         const std::vector<int> &uci, std::vector<int> &blk, std::vector<int> &lout,
+                         std::vector<int> &uout) {
   if (sc.kind != 2 || sc.d_brows != nullptr || sc.max_block > 64 || n == 0) return 0;
-  blo.assign((size_t)n, 0);
-  bhi.assign((size_t)n, 0);
+  blk.assign((size_t)n, 0);
+  std::vector<int> blo((size_t)n), bhi((size_t)n);
   const int nb = (int)sc.h_bstart.size() - 1;
   for (int kb = 0; kb < nb; ++kb)
     for (int i = sc.h_bstart[kb]; i < sc.h_bstart[kb + 1]; ++i) {
+      blk[(size_t)i] = kb;
       blo[(size_t)i] = sc.h_bstart[kb];
       bhi[(size_t)i] = sc.h_bstart[kb + 1];
     }
@@ -367,15 +368,20 @@ static int abmc_sections(const DevSched &sc, int n, const std::vector<int> &lrp,
   S = std::max(S, 2);
   long plain = 0, sect = 0;
   for (int s0 = 0; s0 < n; s0 += 64) {
-    int wl = 0, wu = 0, ol = 0, ou = 0;
+    int wl = 0, wu = 0;
     for (int i = s0; i < std::min(n, s0 + 64); ++i) {
       wl = std::max(wl, lin[(size_t)i] + lout[(size_t)i]);
       wu = std::max(wu, uin[(size_t)i] + uout[(size_t)i]);
+    }
+    plain += 64L * (wl + wu);
+  }
+  for (int kb = 0; kb < nb; ++kb) {
+    int ol = 0, ou = 0;
+    for (int i = sc.h_bstart[kb]; i < sc.h_bstart[kb + 1]; ++i) {
       ol = std::max(ol, lout[(size_t)i]);
       ou = std::max(ou, uout[(size_t)i]);
     }
-    plain += wl + wu;
-    sect += 2 * S + ol + ou;
+    sect += 64L * (2 * S + ol + ou);
   }
   return 4 * sect <= 5 * plain ? S : 0;
 }
@@ -401,13 +407,13 @@ static void build_sliced_factor(iccg_precond_s &M, const std::vector<int> &lrp,
     b = urp[i] + 1;
     e = urp[i + 1];
   };
-  std::vector<int> blo, bhi, lout, uout;
-  const int S = abmc_sections(M.sc, n, lrp, lci, urp, uci, blo, bhi, lout, uout);
+  std::vector<int> blk, lout, uout;
+  const int S = abmc_sections(M.sc, n, lrp, lci, urp, uci, blk, lout, uout);
   if (S > 0) {
-    sell_layout_sections(n, lrange, lci.data(), blo.data(), bhi.data(), lout, S, loff, lcol, lw,
-                         &l2ls);
-    sell_layout_sections(n, urange, uci.data(), blo.data(), bhi.data(), uout, S, uoff, ucol, uw,
-                         &upos);
+    sell_layout_sections(n, lrange, lci.data(), M.sc.h_bstart, blk.data(), lout, S, loff, lcol,
+                         lw, &l2ls);
+    sell_layout_sections(n, urange, uci.data(), M.sc.h_bstart, blk.data(), uout, S, uoff, ucol,
+                         uw, &upos);
   } else {
     sell_layout(n, lrange, lci.data(), loff, lcol, lw, &l2ls);
     sell_layout(n, urange, uci.data(), uoff, ucol, uw, &upos);

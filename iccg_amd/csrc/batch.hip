@@ -401,7 +401,7 @@ __device__ __forceinline__ void abmc_bwd_group(int bs, int be, int nl, const Sys
 // loop (sec_levels) run for every live member on the shared entries.  my
 // holds one kSecWindow window per member (row slots and the zero slot).
 template <class FT, int SW, int KM, bool FWD>
-__device__ __forceinline__ void abmc_group_sec(int bs, int be, int nl, const SysDesc &D0,
+__device__ __forceinline__ void abmc_group_sec(int kb, int bs, int be, int nl, const SysDesc &D0,
                                                const GroupDesc &g,
                                                const SysDesc *__restrict__ desc,
                                                const bool (&live)[KM], FT *my, int lane) {
@@ -427,9 +427,9 @@ __device__ __forceinline__ void abmc_group_sec(int bs, int be, int nl, const Sys
     const int *off = FWD ? D0.l_off : D0.u_off;
     const int *col = FWD ? D0.l_col : D0.u_col;
     const FT *fv = (const FT *)(FWD ? D0.l_v : D0.u_v);
-    const int s = i >> 6, o0 = off[s];
-    const int w = (off[s + 1] - o0) >> 6;
-    const int base = o0 + (i & 63);
+    const int o0 = off[kb];
+    const int w = (off[kb + 1] - o0) >> 6;
+    const int base = o0 + lane;
 #pragma unroll
     for (int k = 0; k < SW; ++k) {
       if (k < sec) {
@@ -553,9 +553,9 @@ __global__ __launch_bounds__(kWG) void kb_sweep_sec(const GroupDesc *__restrict_
     const int bs = D0.bstart[kb], be = D0.bstart[kb + 1];
     const int nl = FWD ? D0.nlev_f[kb] : D0.nlev_b[kb];
     if (D0.mixed)
-      abmc_group_sec<float, SW, KM, FWD>(bs, be, nl, D0, g, desc, live, (float *)my, lane);
+      abmc_group_sec<float, SW, KM, FWD>(kb, bs, be, nl, D0, g, desc, live, (float *)my, lane);
     else
-      abmc_group_sec<double, SW, KM, FWD>(bs, be, nl, D0, g, desc, live, my, lane);
+      abmc_group_sec<double, SW, KM, FWD>(kb, bs, be, nl, D0, g, desc, live, my, lane);
   }
 }
 

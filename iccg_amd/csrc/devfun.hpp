@@ -483,8 +483,9 @@ __device__ __forceinline__ YT abmc_bwd_block(int bs, int be, int nl,
 }
 
 // ---------------------------------------------------------------------------
-// ABMC sweeps over the sectioned factor layout.  Every 64-row slice holds
-// the block's rows; entry k < S of a row is its k-th in-block entry
+// ABMC sweeps over the sectioned factor layout.  Slice b holds block b's
+// rows (at most 64, lane = row - bstart[b]); entry k < S of a row is its
+// k-th in-block entry
 // (columns in [bs, be), ascending, -1 past the last) and entry S + j its j-th
 // entry outside the block (ascending, -1 past the last): for L the earlier
 // colors' columns (< bs), for U the later colors' (>= be).  A row's chain is
@@ -545,104 +546,69 @@ __device__ __forceinline__ YT sec_levels(int nl, YT acc, YT d, const int (&ad)[S
   return yi;
 }
 
-// one block's entries: in-block slots and coefficients, outside columns and
-// coefficients (the row's slice is i >> 6, lane i & 63)
+// One lane's raw loads for a block, issued without branches so that they
+// stay in flight while the previous block runs: lanes past the block end and
+// absent outside entries load from in-bounds stand-in positions, and
+// sec_take masks them out.
 template <class VT, class YT, int S>
-__device__ __forceinline__ void sec_load(int i, int bs, const int *__restrict__ off,
-                                         const int *__restrict__ col, const VT *__restrict__ v,
-                                         int (&ad)[S], YT (&va)[S], int (&co)[S], VT (&vo)[S]) {
-  const int s = i >> 6, o0 = off[s];
-  const int w = (off[s + 1] - o0) >> 6;
-  const int base = o0 + (i & 63);
+struct SecRaw {
+  int cin[S], cout_[S];
+  VT vin[S], vout[S];
+  VT d;
+  double a;   // r_i (forward) or y_i widened (backward)
+  double ri;  // backward with r
+};
+
+// LOAD_NOW: the caller uses the loads at once, so absent outside entries
+// are skipped by (uniform) branches instead of loaded from stand-ins.
+template <class VT, class YT, int S, bool FWD, bool LOAD_NOW = false>
+__device__ __forceinline__ void sec_fetch(int o0, int w, int bs, int be, int lane,
+                                          const int *__restrict__ col, const VT *__restrict__ v,
+                                          const VT *__restrict__ diag,
+                                          const double *__restrict__ r,
+                                          const YT *__restrict__ yin, SecRaw<VT, YT, S> &e) {
+  const int t = min(lane, be - bs - 1);  // a row of the block
+  const int base = o0 + t;
 #pragma unroll
   for (int k = 0; k < S; ++k) {
-    const int c = col[base + 64 * k];
-    ad[k] = c >= 0 ? c - bs : 64;
-    va[k] = c >= 0 ? (YT)v[base + 64 * k] : (YT)0;
+    e.cin[k] = col[base + 64 * k];
+    e.vin[k] = v[base + 64 * k];
   }
 #pragma unroll
   for (int k = 0; k < S; ++k) {
-    co[k] = -1;
-    vo[k] = (VT)0;
-    if (S + k < w) {
-      co[k] = col[base + 64 * (S + k)];
-      vo[k] = v[base + 64 * (S + k)];
+    if (LOAD_NOW) {
+      e.cout_[k] = -1;
+      e.vout[k] = (VT)0;
+      if (S + k < w) {
+        e.cout_[k] = col[base + 64 * (S + k)];
+        e.vout[k] = v[base + 64 * (S + k)];
+      }
+    } else {
+      const int q = base + 64 * (S + k < w ? S + k : 0);
+      e.cout_[k] = col[q];
+      e.vout[k] = v[q];
     }
   }
+  e.d = diag[bs + t];
+  e.a = FWD ? r[bs + t] : (double)yin[bs + t];
+  e.ri = (!FWD && r) ? r[bs + t] : 0.0;
 }
 
 template <class VT, class YT, int S>
-__device__ __forceinline__ void abmc_fwd_sec(int bs, int be, int nl, const int *__restrict__ off,
-                                             const int *__restrict__ col,
-                                             const VT *__restrict__ lv,
-                                             const VT *__restrict__ diag,
-                                             const double *__restrict__ r, YT *__restrict__ y,
-                                             YT *my, int lane) {
-  const int i = bs + lane;
-  const bool act = i < be;
-  int ad[S], co[S];
-  YT va[S], none[S];
-  VT vo[S];
-  YT acc = (YT)0, d = (YT)1;
+__device__ __forceinline__ void sec_take(const SecRaw<VT, YT, S> &e, int w, int bs, bool act,
+                                         int (&ad)[S], YT (&va)[S], int (&co)[S], YT (&vo)[S],
+                                         YT &d, YT &acc) {
 #pragma unroll
   for (int k = 0; k < S; ++k) {
-    ad[k] = 64;
-    va[k] = none[k] = (YT)0;
+    const bool in = act && e.cin[k] >= 0;
+    ad[k] = in ? e.cin[k] - bs : 64;
+    va[k] = in ? (YT)e.vin[k] : (YT)0;
+    const bool out = act && S + k < w && e.cout_[k] >= 0;
+    co[k] = out ? e.cout_[k] : -1;
+    vo[k] = out ? (YT)e.vout[k] : (YT)0;
   }
-  if (act) {
-    sec_load<VT, YT, S>(i, bs, off, col, lv, ad, va, co, vo);
-    d = (YT)diag[i];
-    acc = (YT)r[i];
-    YT xo[S];
-#pragma unroll
-    for (int k = 0; k < S; ++k) xo[k] = co[k] >= 0 ? y[co[k]] : (YT)0;
-#pragma unroll
-    for (int k = 0; k < S; ++k)
-      if (co[k] >= 0) acc = fma_t(-(YT)vo[k], xo[k], acc);
-  }
-  if (lane == 0) my[64] = (YT)0;
-  wave_sync();
-  const YT yi = sec_levels<YT, S, false>(nl, acc, d, ad, va, none, none, my, lane, act);
-  if (act) y[i] = yi;
-}
-
-// returns z_i (0 on lanes past the block); with r, also r_i in ri
-template <class VT, class YT, int S>
-__device__ __forceinline__ YT abmc_bwd_sec(int bs, int be, int nl, const int *__restrict__ off,
-                                           const int *__restrict__ col, const VT *__restrict__ uv,
-                                           const VT *__restrict__ diag,
-                                           const YT *__restrict__ y, YT *__restrict__ z,
-                                           const double *__restrict__ r, double &ri, YT *my,
-                                           int lane) {
-  const int i = bs + lane;
-  const bool act = i < be;
-  int ad[S], co[S];
-  YT va[S], xv[S], xo[S];
-  VT vo[S];
-  YT acc = (YT)0, d = (YT)1;
-  ri = 0.0;
-#pragma unroll
-  for (int k = 0; k < S; ++k) {
-    ad[k] = 64;
-    va[k] = xv[k] = xo[k] = (YT)0;
-  }
-  if (act) {
-    sec_load<VT, YT, S>(i, bs, off, col, uv, ad, va, co, vo);
-#pragma unroll
-    for (int k = 0; k < S; ++k)
-      if (co[k] >= 0) {
-        xo[k] = z[co[k]];
-        xv[k] = (YT)vo[k];
-      }
-    d = (YT)diag[i];
-    acc = y[i];
-    if (r) ri = r[i];
-  }
-  if (lane == 0) my[64] = (YT)0;
-  wave_sync();
-  const YT zi = sec_levels<YT, S, true>(nl, acc, d, ad, va, xv, xo, my, lane, act);
-  if (act) z[i] = zi;
-  return act ? zi : (YT)0;
+  d = act ? (YT)e.d : (YT)1;
+  acc = act ? (YT)e.a : (YT)0;
 }
 
 }  // namespace iccg

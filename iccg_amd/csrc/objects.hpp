@@ -135,39 +135,40 @@ inline void sell_layout(int n, Range range, const int *ci, std::vector<int> &off
 }
 
 // Sectioned sliced-ELL layout of a triangular factor for the ABMC sweeps:
-// rows are blocked by blk_of (contiguous blocks), and entry k < S of row i
-// is its k-th column inside the row's block, entry S + j its j-th column
-// outside it, both in ascending order, -1 elsewhere.  A slice's width is S
-// plus the largest outside count of its rows.  nin / nout give each row's
-// counts; every count must be <= S.
+// one slice per block (rows [bstart[b], bstart[b+1]), at most 64, lane =
+// row - bstart[b]); entry k < S of a row is its k-th column inside the
+// block, entry S + j its j-th column outside it, both in ascending order, -1
+// elsewhere.  A slice's width is S plus the largest outside count of its
+// rows.  blk[i] is row i's block; every in-block and outside count is <= S.
 template <class Range>
-inline void sell_layout_sections(int n, Range range, const int *ci, const int *blk_lo,
-                                 const int *blk_hi, const std::vector<int> &nout, int S,
-                                 std::vector<int> &off, std::vector<int> &col, int &maxw,
-                                 std::vector<int> *pos) {
-  const int ns = (n + 63) / 64;
-  off.assign((size_t)ns + 1, 0);
+inline void sell_layout_sections(int n, Range range, const int *ci,
+                                 const std::vector<int> &bstart, const int *blk,
+                                 const std::vector<int> &nout, int S, std::vector<int> &off,
+                                 std::vector<int> &col, int &maxw, std::vector<int> *pos) {
+  const int nb = (int)bstart.size() - 1;
+  off.assign((size_t)nb + 1, 0);
   maxw = 0;
   long total = 0;
-  for (int s = 0; s < ns; ++s) {
+  for (int b = 0; b < nb; ++b) {
     int w = 0;
-    for (int i = s * 64; i < std::min(n, s * 64 + 64); ++i) w = std::max(w, nout[(size_t)i]);
+    for (int i = bstart[b]; i < bstart[b + 1]; ++i) w = std::max(w, nout[(size_t)i]);
     w += S;
     maxw = std::max(maxw, w);
     total += 64L * w;
     require(total < 0x7fffffffL, ICCG_ERR_ARG, "matrix too large for the sliced layout");
-    off[s + 1] = (int)total;
+    off[b + 1] = (int)total;
   }
   col.assign((size_t)total, -1);
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i) {
     int b, e;
     range(i, b, e);
-    const int base = off[i >> 6] + (i & 63);
+    const int bl = blk[i], lo = bstart[bl], hi = bstart[bl + 1];
+    const int base = off[bl] + (i - lo);
     int kin = 0, kout = S;
     for (int k = b; k < e; ++k) {
       const int c = ci[k];
-      const int q = base + 64 * (c >= blk_lo[i] && c < blk_hi[i] ? kin++ : kout++);
+      const int q = base + 64 * (c >= lo && c < hi ? kin++ : kout++);
       col[q] = c;
       if (pos) (*pos)[k] = q;
     }
