@@ -16,6 +16,8 @@ ICCG_OK = 0
 ICCG_ERR_ARG, ICCG_ERR_IO, ICCG_ERR_PIVOT, ICCG_ERR_SCHEDULE = 1, 2, 3, 4
 ICCG_ERR_DEVICE, ICCG_ERR_NOMEM, ICCG_ERR_STATE = 5, 6, 7
 PRECOND_IDENTITY, PRECOND_JACOBI, PRECOND_IC0 = 0, 1, 2
+PLAN_NAMES = {0: "none", 1: "levels", 2: "csr", 3: "mc-sliced", 4: "abmc-sliced",
+              5: "abmc-sectioned"}
 FP64, MIXED = 0, 1
 SCHED_MC, SCHED_ABMC = 0, 1
 SOLVE_TRUE_RESID = 1
@@ -94,6 +96,7 @@ _SIGS = {
     "iccg_precond_apply": (_i, [_vp, _vp, _vp]),
     "iccg_precond_factor_nnz": (_i, [_vp, _lp]),
     "iccg_precond_get_factor": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "iccg_precond_get_plan": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
     "iccg_cg_solve": (_i, [_vp, _vp, _vp, _vp, _i, _d, _i, C.POINTER(CGResultC)]),
     "iccg_solver_create": (_i, [_vp, _vp, C.POINTER(_vp)]),
     "iccg_solver_destroy": (_i, [_vp]),

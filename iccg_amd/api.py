@@ -438,6 +438,12 @@ class IC0(_Precond):
         return ("IC preconditioner with multi-coloring" if self._sched is not None
                 else "IC preconditioner")
 
+    def plan(self):
+        """(plan name, section width) of the device triangular solves."""
+        p, w = C.c_int(0), C.c_int(0)
+        check(lib().iccg_precond_get_plan(self.handle, C.byref(p), C.byref(w)))
+        return _capi.PLAN_NAMES[p.value], w.value
+
     def factor(self):
         """(Lrp, Lci, Lv, LTcp, LTri, LTv): IC0's members (ic0.hpp:38-49)."""
         nnz = C.c_long(0)

@@ -912,7 +912,40 @@ int iccg_precond_set_schedule(iccg_precond_t M, iccg_sched_t sched) {
     tmp.st = M->st;
     build_schedule(tmp, sched, lrp, lci, urp, uci);
     std::swap(M->sc, tmp.sc);
+    // an ABMC factor's sliced copies may be laid out by the old schedule's
+    // blocks (sectioned layout): lay them out again for the new one
+    Factor &F = M->F;
+    if (F.sell && (F.sec > 0 || M->sc.kind == 2)) {
+      free_sell(F.ls);
+      free_sell(F.us);
+      dfree(F.d64);
+      dfree(F.d32);
+      F.d64 = nullptr;
+      F.d32 = nullptr;
+      F.sell = false;
+      F.sec = 0;
+      build_sliced_factor(*M, lrp, lci, urp, uci, l2u);
+    }
     M->plan_version++;
+  });
+}
+
+int iccg_precond_get_plan(iccg_precond_t M, int *plan, int *section_width) {
+  return guard([&] {
+    require(M && plan && section_width, ICCG_ERR_ARG, "null argument");
+    *plan = ICCG_PLAN_NONE;
+    *section_width = 0;
+    if (M->kind != ICCG_PRECOND_IC0) return;
+    const DevSched &sc = M->sc;
+    if (!sliced_plan(sc, M->F))
+      *plan = sc.kind == 0 ? ICCG_PLAN_LEVELS : ICCG_PLAN_CSR;
+    else if (sc.kind == 1)
+      *plan = ICCG_PLAN_MC_SLICED;
+    else if (M->F.sec > 0) {
+      *plan = ICCG_PLAN_ABMC_SECTIONED;
+      *section_width = M->F.sec;
+    } else
+      *plan = ICCG_PLAN_ABMC_SLICED;
   });
 }
 

@@ -216,8 +216,21 @@ int iccg_precond_create(iccg_csr_t A, int kind, double shift,
 int iccg_precond_destroy(iccg_precond_t M);
 /* Re-plans the triangular solves of an IC0 preconditioner for another
  * schedule (IC0::apply takes the schedule per call, ic0.hpp:34).  The factor
- * is kept: under every valid schedule it is the same, bit for bit. */
+ * is kept: under every valid schedule it is the same, bit for bit.  Not
+ * while M is in use by a batch or a solver on another thread. */
 int iccg_precond_set_schedule(iccg_precond_t M, iccg_sched_t sched);
+/* Which device plan runs the triangular solves (no counterpart in the
+ * reference; a diagnostic): plan is one of ICCG_PLAN_*, section_width the
+ * in-block section width S of the sectioned ABMC layout, else 0. */
+enum {
+  ICCG_PLAN_NONE = 0,            /* Identity / Jacobi */
+  ICCG_PLAN_LEVELS = 1,          /* natural order, level sets */
+  ICCG_PLAN_CSR = 2,             /* MC or ABMC over the CSR factor */
+  ICCG_PLAN_MC_SLICED = 3,       /* multicolor, sliced ELL */
+  ICCG_PLAN_ABMC_SLICED = 4,     /* ABMC, sliced ELL, masked level loop */
+  ICCG_PLAN_ABMC_SECTIONED = 5   /* ABMC, sectioned layout, mask-free levels */
+};
+int iccg_precond_get_plan(iccg_precond_t M, int *plan, int *section_width);
 /* apply(r, z, sched) (precond.hpp:12,38; ic0.hpp:34) on host buffers. */
 int iccg_precond_apply(iccg_precond_t M, const double *r, double *z);
 /* Copies the FP64 factor out: L (lower+diag CSR) and L^T (CSC of L), in the

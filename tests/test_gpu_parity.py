@@ -116,6 +116,7 @@ def test_abmc_grid_boxes():
     assert o.extra["nc"] == 2
     f = O.ic0_factor(P.orc(o.A), 0.0)
     M = I.IC0(o.A, 0.0, sched=o.sched)
+    assert M.plan() == ("abmc-sectioned", 3)
     _factor_equal(M, f)
     r = I.gen_normal(A.n, 9)
     assert np.array_equal(M.apply(r, o.sched), O.ic0_apply(f, r, None))
@@ -172,6 +173,37 @@ def test_sectioned_sweeps_out_of_range_numerators(precision):
     # an all-zero right-hand side gives +0 everywhere
     z0 = M.apply(np.zeros(o.A.n), o.sched)
     assert np.array_equal(z0, ref(f, np.zeros(o.A.n))) and not np.signbit(z0).any()
+
+
+def _slab_schedule(n, nblocks):
+    """ABMC schedule of a natural-order matrix: nblocks contiguous slabs of
+    rows, slab k in color k (every dependency lies in an earlier slab or in
+    its own)."""
+    bp = np.linspace(0, n, nblocks + 1).astype(np.int32)
+    ar = np.arange(nblocks + 1, dtype=np.int32)
+    return I.ColorSchedule.abmc(nblocks, nblocks, ar, ar[:-1].copy(), bp,
+                                np.arange(n, dtype=np.int32))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_schedule_switch_lays_out_factor_again(precision):
+    """The sliced factor of an ABMC schedule is laid out by its blocks
+    (sectioned layout, or the plain one): switching the schedule of the same
+    preconditioner lays it out again, and every apply stays bitwise equal to
+    the oracle."""
+    A = P.grid(4, 4, 8, coef=(-2.0, 2.0), shift=1e-6)
+    f = O.ic0_factor(P.orc(A), 0.0)
+    ref = O.ic0_apply if precision == "fp64" else O.ic0_apply_f32
+    two, four = _slab_schedule(A.n, 2), _slab_schedule(A.n, 4)
+    M = I.IC0(A, 0.0, sched=two, precision=precision)
+    r = I.gen_normal(A.n, 12)
+    expect = ref(f, r)
+    for sched, plan in [(two, ("abmc-sectioned", 3)), (four, ("abmc-sliced", 0)),
+                        (two, ("abmc-sectioned", 3)), (None, ("levels", 0)),
+                        (four, ("abmc-sliced", 0))]:
+        z = M.apply(r, sched)
+        assert M.plan() == plan
+        assert np.array_equal(z, expect)
 
 
 def _solve(A, kind, sched=None, b=None, tol=1e-8, max_iter=10000, precision="fp64"):
