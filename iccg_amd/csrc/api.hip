@@ -409,11 +409,12 @@ static void build_sliced_factor(iccg_precond_s &M, const std::vector<int> &lrp,
   };
   std::vector<int> blk, lout, uout;
   const int S = abmc_sections(M.sc, n, lrp, lci, urp, uci, blk, lout, uout);
+  std::vector<unsigned> linb, uinb;
   if (S > 0) {
     sell_layout_sections(n, lrange, lci.data(), M.sc.h_bstart, blk.data(), lout, S, loff, lcol,
-                         lw, &l2ls);
+                         linb, lw, &l2ls);
     sell_layout_sections(n, urange, uci.data(), M.sc.h_bstart, blk.data(), uout, S, uoff, ucol,
-                         uw, &upos);
+                         uinb, uw, &upos);
   } else {
     sell_layout(n, lrange, lci.data(), loff, lcol, lw, &l2ls);
     sell_layout(n, urange, uci.data(), uoff, ucol, uw, &upos);
@@ -439,6 +440,10 @@ static void build_sliced_factor(iccg_precond_s &M, const std::vector<int> &lrp,
   };
   up(F.ls, loff, lcol, lw);
   up(F.us, uoff, ucol, uw);
+  if (S > 0) {
+    F.ls.inb = dupload(linb.data(), linb.size(), st);
+    F.us.inb = dupload(uinb.data(), uinb.size(), st);
+  }
   if (M.mixed)
     F.d32 = dalloc<float>((size_t)n);
   else

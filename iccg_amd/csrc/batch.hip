@@ -427,17 +427,15 @@ __device__ __forceinline__ void abmc_group_sec(int kb, int bs, int be, int nl, c
     const int *off = FWD ? D0.l_off : D0.u_off;
     const int *col = FWD ? D0.l_col : D0.u_col;
     const FT *fv = (const FT *)(FWD ? D0.l_v : D0.u_v);
+    const unsigned pk = (FWD ? D0.l_inb : D0.u_inb)[64 * kb + lane];
     const int o0 = off[kb];
     const int w = (off[kb + 1] - o0) >> 6;
     const int base = o0 + lane;
 #pragma unroll
     for (int k = 0; k < SW; ++k) {
       if (k < sec) {
-        const int c = col[base + 64 * k];
-        if (c >= 0) {
-          ad[k] = c - bs;
-          va[k] = fv[base + 64 * k];
-        }
+        ad[k] = (int)((pk >> (8 * k)) & 0xFFu);  // 64 (the zero slot) when absent
+        va[k] = fv[base + 64 * k];               // absent entries hold 0
       }
       if (sec + k < w) {
         co[k] = col[base + 64 * (sec + k)];
@@ -1371,6 +1369,8 @@ int iccg_batch_add(iccg_batch_t B, iccg_csr_t A, iccg_precond_t M, const double 
       D.nlev_f = M->sc.d_nlev_f;
       D.nlev_b = M->sc.d_nlev_b;
       D.sec = M->F.sec;
+      D.l_inb = M->F.ls.inb;
+      D.u_inb = M->F.us.inb;
     }
     D.x = s.x;
     D.r = s.r;

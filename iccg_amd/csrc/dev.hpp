@@ -62,6 +62,9 @@ struct Sell {
   // (t << 6 | column & 63), padding 0xFFFF.  Half the index bytes.
   uint16_t *col16 = nullptr;
   int *tbl = nullptr;
+  // sectioned layout (slice = ABMC block b): inb[64 b + lane], byte k = the
+  // row within the block of the lane's k-th in-block entry, 64 when absent
+  unsigned *inb = nullptr;
 };
 
 struct DevCsr {
@@ -139,6 +142,7 @@ struct SysDesc {
   const double *a_v64 = nullptr;
   const float *a_v32 = nullptr;      // set when every value of A is an exact float
   const int *l_off = nullptr, *l_col = nullptr, *u_off = nullptr, *u_col = nullptr;
+  const unsigned *l_inb = nullptr, *u_inb = nullptr;  // sectioned layout
   const void *l_v = nullptr, *u_v = nullptr, *d = nullptr;
   // ABMC blocks (nullptr for a multicolor system)
   const int *bstart = nullptr, *nlev_f = nullptr, *nlev_b = nullptr;

@@ -552,7 +552,8 @@ __device__ __forceinline__ YT sec_levels(int nl, YT acc, YT d, const int (&ad)[S
 // sec_take masks them out.
 template <class VT, class YT, int S>
 struct SecRaw {
-  int cin[S], cout_[S];
+  unsigned inb;  // packed in-block rows (Sell::inb)
+  int cout_[S];
   VT vin[S], vout[S];
   VT d;
   double a;   // r_i (forward) or y_i widened (backward)
@@ -562,18 +563,17 @@ struct SecRaw {
 // LOAD_NOW: the caller uses the loads at once, so absent outside entries
 // are skipped by (uniform) branches instead of loaded from stand-ins.
 template <class VT, class YT, int S, bool FWD, bool LOAD_NOW = false>
-__device__ __forceinline__ void sec_fetch(int o0, int w, int bs, int be, int lane,
+__device__ __forceinline__ void sec_fetch(int kb, int o0, int w, int bs, int be, int lane,
+                                          const unsigned *__restrict__ inb,
                                           const int *__restrict__ col, const VT *__restrict__ v,
                                           const VT *__restrict__ diag,
                                           const double *__restrict__ r,
                                           const YT *__restrict__ yin, SecRaw<VT, YT, S> &e) {
   const int t = min(lane, be - bs - 1);  // a row of the block
   const int base = o0 + t;
+  e.inb = inb[64 * kb + t];
 #pragma unroll
-  for (int k = 0; k < S; ++k) {
-    e.cin[k] = col[base + 64 * k];
-    e.vin[k] = v[base + 64 * k];
-  }
+  for (int k = 0; k < S; ++k) e.vin[k] = v[base + 64 * k];  // absent entries hold 0
 #pragma unroll
   for (int k = 0; k < S; ++k) {
     if (LOAD_NOW) {
@@ -595,14 +595,13 @@ __device__ __forceinline__ void sec_fetch(int o0, int w, int bs, int be, int lan
 }
 
 template <class VT, class YT, int S>
-__device__ __forceinline__ void sec_take(const SecRaw<VT, YT, S> &e, int w, int bs, bool act,
+__device__ __forceinline__ void sec_take(const SecRaw<VT, YT, S> &e, int w, bool act,
                                          int (&ad)[S], YT (&va)[S], int (&co)[S], YT (&vo)[S],
                                          YT &d, YT &acc) {
 #pragma unroll
   for (int k = 0; k < S; ++k) {
-    const bool in = act && e.cin[k] >= 0;
-    ad[k] = in ? e.cin[k] - bs : 64;
-    va[k] = in ? (YT)e.vin[k] : (YT)0;
+    ad[k] = act ? (int)((e.inb >> (8 * k)) & 0xFFu) : 64;
+    va[k] = act ? (YT)e.vin[k] : (YT)0;
     const bool out = act && S + k < w && e.cout_[k] >= 0;
     co[k] = out ? e.cout_[k] : -1;
     vo[k] = out ? (YT)e.vout[k] : (YT)0;

@@ -582,6 +582,7 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
                                                       const int *__restrict__ bstart,
                                                       const int *__restrict__ nlev,
                                                       const int *__restrict__ off,
+                                                      const unsigned *__restrict__ inb,
                                                       const int *__restrict__ col,
                                                       const VT *__restrict__ lv,
                                                       const VT *__restrict__ diag,
@@ -603,7 +604,7 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
   int bs = bstart[first], be = bstart[first + 1], nl = nlev[first];
   int o0 = off[first], w = (off[first + 1] - o0) >> 6;
   SecRaw<VT, YT, SW> cur;
-  if (PIPE) sec_fetch<VT, YT, SW, true>(o0, w, bs, be, lane, col, lv, diag, r, y, cur);
+  if (PIPE) sec_fetch<VT, YT, SW, true>(first, o0, w, bs, be, lane, inb, col, lv, diag, r, y, cur);
   for (int kb = first; kb < end; kb += step) {
     if (!PIPE) {
       bs = bstart[kb];
@@ -611,12 +612,12 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
       nl = nlev[kb];
       o0 = off[kb];
       w = (off[kb + 1] - o0) >> 6;
-      sec_fetch<VT, YT, SW, true, true>(o0, w, bs, be, lane, col, lv, diag, r, y, cur);
+      sec_fetch<VT, YT, SW, true, true>(kb, o0, w, bs, be, lane, inb, col, lv, diag, r, y, cur);
     }
     const bool act = bs + lane < be;
     int ad[SW], co[SW];
     YT va[SW], vo[SW], d, acc, xo[SW];
-    sec_take<VT, YT, SW>(cur, w, bs, act, ad, va, co, vo, d, acc);
+    sec_take<VT, YT, SW>(cur, w, act, ad, va, co, vo, d, acc);
 #pragma unroll
     for (int k = 0; k < SW; ++k)
       xo[k] = PIPE ? y[co[k] >= 0 ? co[k] : 0] : co[k] >= 0 ? y[co[k]] : (YT)0;
@@ -624,7 +625,8 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
     const int nbs = PIPE ? bstart[kc] : 0, nbe = PIPE ? bstart[kc + 1] : 0;
     const int nnl = PIPE ? nlev[kc] : 0, no0 = PIPE ? off[kc] : 0;
     const int nw = PIPE ? (off[kc + 1] - no0) >> 6 : 0;
-    if (PIPE) sec_fetch<VT, YT, SW, true>(no0, nw, nbs, nbe, lane, col, lv, diag, r, y, cur);
+    if (PIPE)
+      sec_fetch<VT, YT, SW, true>(kc, no0, nw, nbs, nbe, lane, inb, col, lv, diag, r, y, cur);
     // the earlier colors' terms, ascending; absent ones are fma(-0, 0, acc)
 #pragma unroll
     for (int k = 0; k < SW; ++k) acc = fma_t(-vo[k], co[k] >= 0 ? xo[k] : (YT)0, acc);
@@ -645,9 +647,9 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
 template <class VT, class YT, int SW, bool RZ>
 __global__ __launch_bounds__(kWG) void k_abmc_bwd_sec(
     int kb0, int kb1, const int *__restrict__ bstart, const int *__restrict__ nlev,
-    const int *__restrict__ off, const int *__restrict__ col, const VT *__restrict__ uv,
-    const VT *__restrict__ diag, const YT *__restrict__ y, YT *__restrict__ z,
-    const double *__restrict__ r, Scal *S, double *part, int gidx0) {
+    const int *__restrict__ off, const unsigned *__restrict__ inb, const int *__restrict__ col,
+    const VT *__restrict__ uv, const VT *__restrict__ diag, const YT *__restrict__ y,
+    YT *__restrict__ z, const double *__restrict__ r, Scal *S, double *part, int gidx0) {
   constexpr bool PIPE = sizeof(YT) == 4;
   __shared__ YT win[(kWG / 64) * kSecWindow];
   if (S && S->done) return;
@@ -663,7 +665,7 @@ __global__ __launch_bounds__(kWG) void k_abmc_bwd_sec(
   int bs = bstart[f0], be = bstart[f0 + 1], nl = nlev[f0];
   int o0 = off[f0], w = (off[f0 + 1] - o0) >> 6;
   SecRaw<VT, YT, SW> cur;
-  if (PIPE) sec_fetch<VT, YT, SW, false>(o0, w, bs, be, lane, col, uv, diag, rr, y, cur);
+  if (PIPE) sec_fetch<VT, YT, SW, false>(f0, o0, w, bs, be, lane, inb, col, uv, diag, rr, y, cur);
   for (int kb = first; kb < end; kb += step) {
     if (!PIPE) {
       bs = bstart[kb];
@@ -671,12 +673,13 @@ __global__ __launch_bounds__(kWG) void k_abmc_bwd_sec(
       nl = nlev[kb];
       o0 = off[kb];
       w = (off[kb + 1] - o0) >> 6;
-      sec_fetch<VT, YT, SW, false, true>(o0, w, bs, be, lane, col, uv, diag, rr, y, cur);
+      sec_fetch<VT, YT, SW, false, true>(kb, o0, w, bs, be, lane, inb, col, uv, diag, rr, y,
+                                         cur);
     }
     const bool act = bs + lane < be;
     int ad[SW], co[SW];
     YT va[SW], vo[SW], d, acc, xo[SW];
-    sec_take<VT, YT, SW>(cur, w, bs, act, ad, va, co, vo, d, acc);
+    sec_take<VT, YT, SW>(cur, w, act, ad, va, co, vo, d, acc);
     const double ri = cur.ri;
 #pragma unroll
     for (int k = 0; k < SW; ++k)
@@ -685,7 +688,8 @@ __global__ __launch_bounds__(kWG) void k_abmc_bwd_sec(
     const int nbs = PIPE ? bstart[kc] : 0, nbe = PIPE ? bstart[kc + 1] : 0;
     const int nnl = PIPE ? nlev[kc] : 0, no0 = PIPE ? off[kc] : 0;
     const int nw = PIPE ? (off[kc + 1] - no0) >> 6 : 0;
-    if (PIPE) sec_fetch<VT, YT, SW, false>(no0, nw, nbs, nbe, lane, col, uv, diag, rr, y, cur);
+    if (PIPE)
+      sec_fetch<VT, YT, SW, false>(kc, no0, nw, nbs, nbe, lane, inb, col, uv, diag, rr, y, cur);
 #pragma unroll
     for (int k = 0; k < SW; ++k) xo[k] = co[k] >= 0 ? xo[k] : (YT)0;
     if (lane == 0) my[64] = (YT)0;
@@ -1042,7 +1046,7 @@ static void fwd_sec(const DevSched &sc, const Factor &F, const VT *lv, const VT 
     const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
     if (b1 <= b0) continue;
     hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW>), dim3(slice_grid(b1 - b0)), dim3(kWG), 0,
-                       st, b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.col, lv, dg, r, y, S);
+                       st, b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.inb, F.ls.col, lv, dg, r, y, S);
   }
 }
 
@@ -1066,11 +1070,13 @@ static void bwd_sec(const DevSched &sc, const Factor &F, const VT *uv, const VT 
     const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
     if (fuse)
       hipLaunchKernelGGL((k_abmc_bwd_sec<VT, YT, SW, true>), dim3(g), dim3(kWG), 0, st, b0, b1,
-                         sc.d_bstart, sc.d_nlev_b, F.us.off, F.us.col, uv, dg, y, z, r, S, part,
+                         sc.d_bstart, sc.d_nlev_b, F.us.off, F.us.inb, F.us.col, uv, dg, y, z, r, S,
+                         part,
                          gidx0);
     else
       hipLaunchKernelGGL((k_abmc_bwd_sec<VT, YT, SW, false>), dim3(g), dim3(kWG), 0, st, b0, b1,
-                         sc.d_bstart, sc.d_nlev_b, F.us.off, F.us.col, uv, dg, y, z, r, S, part,
+                         sc.d_bstart, sc.d_nlev_b, F.us.off, F.us.inb, F.us.col, uv, dg, y, z, r, S,
+                         part,
                          gidx0);
     gidx0 += g;
   }

@@ -52,6 +52,7 @@ inline void free_sell(Sell &S) {
   dfree(S.v32);
   dfree(S.col16);
   dfree(S.tbl);
+  dfree(S.inb);
   S = Sell();
 }
 
@@ -139,12 +140,16 @@ inline void sell_layout(int n, Range range, const int *ci, std::vector<int> &off
 // row - bstart[b]); entry k < S of a row is its k-th column inside the
 // block, entry S + j its j-th column outside it, both in ascending order, -1
 // elsewhere.  A slice's width is S plus the largest outside count of its
-// rows.  blk[i] is row i's block; every in-block and outside count is <= S.
+// rows.  blk[i] is row i's block; every in-block and outside count is <= S
+// (S <= 4).  inb packs each row's in-block entries as rows within the block,
+// one byte per entry (64 = absent), so the sweeps read one word instead of S
+// column indices.
 template <class Range>
 inline void sell_layout_sections(int n, Range range, const int *ci,
                                  const std::vector<int> &bstart, const int *blk,
                                  const std::vector<int> &nout, int S, std::vector<int> &off,
-                                 std::vector<int> &col, int &maxw, std::vector<int> *pos) {
+                                 std::vector<int> &col, std::vector<unsigned> &inb, int &maxw,
+                                 std::vector<int> *pos) {
   const int nb = (int)bstart.size() - 1;
   off.assign((size_t)nb + 1, 0);
   maxw = 0;
@@ -159,6 +164,7 @@ inline void sell_layout_sections(int n, Range range, const int *ci,
     off[b + 1] = (int)total;
   }
   col.assign((size_t)total, -1);
+  inb.assign((size_t)nb * 64, 0x40404040u);
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i) {
     int b, e;
@@ -166,12 +172,16 @@ inline void sell_layout_sections(int n, Range range, const int *ci,
     const int bl = blk[i], lo = bstart[bl], hi = bstart[bl + 1];
     const int base = off[bl] + (i - lo);
     int kin = 0, kout = S;
+    unsigned packed = 0x40404040u;
     for (int k = b; k < e; ++k) {
       const int c = ci[k];
-      const int q = base + 64 * (c >= lo && c < hi ? kin++ : kout++);
+      const bool in = c >= lo && c < hi;
+      if (in) packed = (packed & ~(0xFFu << (8 * kin))) | ((unsigned)(c - lo) << (8 * kin));
+      const int q = base + 64 * (in ? kin++ : kout++);
       col[q] = c;
       if (pos) (*pos)[k] = q;
     }
+    inb[(size_t)bl * 64 + (i - lo)] = packed;
   }
 }
 
