@@ -484,14 +484,14 @@ __device__ __forceinline__ YT abmc_bwd_block(int bs, int be, int nl,
 
 // ---------------------------------------------------------------------------
 // ABMC sweeps over the sectioned factor layout.  Slice b holds block b's
-// rows (at most 64, lane = row - bstart[b]); entry k < S of a row is its
-// k-th in-block entry
-// (columns in [bs, be), ascending, -1 past the last) and entry S + j its j-th
-// entry outside the block (ascending, -1 past the last): for L the earlier
-// colors' columns (< bs), for U the later colors' (>= be).  A row's chain is
-// the sequential reference's (ic0.cpp:263-339): forward, the outside terms
-// first, then the in-block terms; backward, the in-block terms first, then
-// the outside terms.
+// rows (at most 64, lane = row - bstart[b]).  Entry k < S of a row is its
+// k-th in-block entry (columns in [bs, be), ascending; the kernels read their
+// rows within the block from the packed word Sell::inb, and the value, 0
+// past the last), entry S + j its j-th entry outside the block (ascending,
+// column -1 past the last): for L the earlier colors' columns (< bs), for U
+// the later colors' (>= be).  A row's chain is the sequential reference's
+// (ic0.cpp:263-339): forward, the outside terms first, then the in-block
+// terms; backward, the in-block terms first, then the outside terms.
 //
 // The in-block levels run without masks: at every level every lane
 // recomputes its row from the LDS window and stores it.  A term whose entry
