@@ -584,9 +584,12 @@ __device__ __forceinline__ void sec_fetch(int kb, int o0, int w, int bs, int be,
         e.vout[k] = v[base + 64 * (S + k)];
       }
     } else {
-      const int q = base + 64 * (S + k < w ? S + k : 0);
-      e.cout_[k] = col[q];
-      e.vout[k] = v[q];
+      // absent: stand-ins in lines this fetch reads anyway (the packed word,
+      // the first in-block value), so they cost no extra traffic
+      const bool here = S + k < w;
+      const int *cp = here ? col + base + 64 * (S + k) : (const int *)inb + 64 * kb + t;
+      e.cout_[k] = *cp;
+      e.vout[k] = v[here ? base + 64 * (S + k) : base];
     }
   }
   e.d = diag[bs + t];
