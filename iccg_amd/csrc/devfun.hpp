@@ -597,20 +597,23 @@ __device__ __forceinline__ void sec_fetch(int kb, int o0, int w, int bs, int be,
   e.ri = (!FWD && r) ? r[bs + t] : 0.0;
 }
 
+// Lanes past the block end hold a copy of the block's last row (sec_fetch
+// clamps them): they compute it again into their own window slots, which no
+// row of the block reads, and their results are neither stored nor checked.
 template <class VT, class YT, int S>
-__device__ __forceinline__ void sec_take(const SecRaw<VT, YT, S> &e, int w, bool act,
-                                         int (&ad)[S], YT (&va)[S], int (&co)[S], YT (&vo)[S],
-                                         YT &d, YT &acc) {
+__device__ __forceinline__ void sec_take(const SecRaw<VT, YT, S> &e, int w, int (&ad)[S],
+                                         YT (&va)[S], int (&co)[S], YT (&vo)[S], YT &d,
+                                         YT &acc) {
 #pragma unroll
   for (int k = 0; k < S; ++k) {
-    ad[k] = act ? (int)((e.inb >> (8 * k)) & 0xFFu) : 64;
-    va[k] = act ? (YT)e.vin[k] : (YT)0;
-    const bool out = act && S + k < w && e.cout_[k] >= 0;
+    ad[k] = (int)((e.inb >> (8 * k)) & 0xFFu);
+    va[k] = (YT)e.vin[k];
+    const bool out = S + k < w && e.cout_[k] >= 0;
     co[k] = out ? e.cout_[k] : -1;
     vo[k] = out ? (YT)e.vout[k] : (YT)0;
   }
-  d = act ? (YT)e.d : (YT)1;
-  acc = act ? (YT)e.a : (YT)0;
+  d = (YT)e.d;
+  acc = (YT)e.a;
 }
 
 }  // namespace iccg

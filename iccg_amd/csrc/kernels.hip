@@ -617,7 +617,7 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
     const bool act = bs + lane < be;
     int ad[SW], co[SW];
     YT va[SW], vo[SW], d, acc, xo[SW];
-    sec_take<VT, YT, SW>(cur, w, act, ad, va, co, vo, d, acc);
+    sec_take<VT, YT, SW>(cur, w, ad, va, co, vo, d, acc);
 #pragma unroll
     for (int k = 0; k < SW; ++k)
       xo[k] = PIPE ? y[co[k] >= 0 ? co[k] : 0] : co[k] >= 0 ? y[co[k]] : (YT)0;
@@ -679,7 +679,7 @@ __global__ __launch_bounds__(kWG) void k_abmc_bwd_sec(
     const bool act = bs + lane < be;
     int ad[SW], co[SW];
     YT va[SW], vo[SW], d, acc, xo[SW];
-    sec_take<VT, YT, SW>(cur, w, act, ad, va, co, vo, d, acc);
+    sec_take<VT, YT, SW>(cur, w, ad, va, co, vo, d, acc);
     const double ri = cur.ri;
 #pragma unroll
     for (int k = 0; k < SW; ++k)
