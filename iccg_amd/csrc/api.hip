@@ -332,7 +332,8 @@ static void build_schedule(iccg_precond_s &M, const iccg_sched_s *s, const std::
 
 // Section width for the ABMC sweeps' sectioned layout (sell_layout_sections),
 // or 0 for the plain layout: only for contiguous blocks of <= 64 rows, at
-// most 4 in-block and 4 outside entries per row (sweep instantiations 2..4),
+// most 3 in-block and 3 outside entries per row (sweep instantiations 2, 3:
+// 2D and 3D box blockings of 5- and 7-point stencils),
 // and when the sections pad the factor by at most a quarter.  Fills each
 // row's block bounds and outside counts of L and U.
 static int abmc_sections(const DevSched &sc, int n, const std::vector<int> &lrp,
@@ -364,7 +365,7 @@ static int abmc_sections(const DevSched &sc, int n, const std::vector<int> &lrp,
     }
     S = std::max({S, lin[(size_t)i], lout[(size_t)i], uin[(size_t)i], uout[(size_t)i]});
   }
-  if (S > 4) return 0;
+  if (S > 3) return 0;
   S = std::max(S, 2);
   long plain = 0, sect = 0;
   for (int s0 = 0; s0 < n; s0 += 64) {

@@ -940,13 +940,11 @@ void launch_resid_items(iccg_batch_s &B, const ItemSet &I, int mode) {
 // sectioned-layout items: section width class x group size class
 #define SEC_DISPATCH(sw, km, K, EXTRA, ...)                                                   \
   do {                                                                                        \
-    const int _s = (sw) <= 2 ? 2 : (sw) == 3 ? 3 : 4, _k = km_class(km);                      \
-    if (_s == 2 && _k == 1) hipLaunchKernelGGL((K<2, 1 EXTRA>), __VA_ARGS__);                 \
-    else if (_s == 2) hipLaunchKernelGGL((K<2, kMaxRhs EXTRA>), __VA_ARGS__);                 \
-    else if (_s == 3 && _k == 1) hipLaunchKernelGGL((K<3, 1 EXTRA>), __VA_ARGS__);            \
-    else if (_s == 3) hipLaunchKernelGGL((K<3, kMaxRhs EXTRA>), __VA_ARGS__);                 \
-    else if (_k == 1) hipLaunchKernelGGL((K<4, 1 EXTRA>), __VA_ARGS__);                       \
-    else hipLaunchKernelGGL((K<4, kMaxRhs EXTRA>), __VA_ARGS__);                              \
+    const int _k = km_class(km);                                                              \
+    if ((sw) <= 2 && _k == 1) hipLaunchKernelGGL((K<2, 1 EXTRA>), __VA_ARGS__);               \
+    else if ((sw) <= 2) hipLaunchKernelGGL((K<2, kMaxRhs EXTRA>), __VA_ARGS__);               \
+    else if (_k == 1) hipLaunchKernelGGL((K<3, 1 EXTRA>), __VA_ARGS__);                       \
+    else hipLaunchKernelGGL((K<3, kMaxRhs EXTRA>), __VA_ARGS__);                              \
   } while (0)
 
 template <bool FWD>

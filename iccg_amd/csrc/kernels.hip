@@ -1087,8 +1087,7 @@ static void bwd_sec(const DevSched &sc, const Factor &F, const VT *uv, const VT 
   do {                                           \
     switch (sw) {                                \
       case 2: FN<VT, YT, 2>(__VA_ARGS__); break; \
-      case 3: FN<VT, YT, 3>(__VA_ARGS__); break; \
-      default: FN<VT, YT, 4>(__VA_ARGS__); break; \
+      default: FN<VT, YT, 3>(__VA_ARGS__); break; \
     }                                            \
   } while (0)
 

@@ -141,7 +141,7 @@ inline void sell_layout(int n, Range range, const int *ci, std::vector<int> &off
 // block, entry S + j its j-th column outside it, both in ascending order, -1
 // elsewhere.  A slice's width is S plus the largest outside count of its
 // rows.  blk[i] is row i's block; every in-block and outside count is <= S
-// (S <= 4).  inb packs each row's in-block entries as rows within the block,
+// (S <= 3).  inb packs each row's in-block entries as rows within the block,
 // one byte per entry (64 = absent), so the sweeps read one word instead of S
 // column indices.
 template <class Range>
