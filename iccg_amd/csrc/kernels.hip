@@ -924,10 +924,13 @@ bool sliced_plan(const DevSched &sc, const Factor &F) {
   return false;
 }
 
-// r.z comes from a separate streaming kernel after the backward sweep
-// (measured faster on gfx950 than forming it inside the sweep: the fused
-// sweep lost ~23 us per color launch against 15 us for the extra pass);
-// ICCG_FUSE_RZ=1 selects the fused sweep, which gives the same bits.
+// r.z comes from a separate streaming kernel after the backward sweep.
+// ICCG_FUSE_RZ=1 forms it inside the backward sweep instead (at 192^3 the
+// plain-layout sweep lost ~23 us per color launch to it against 15 us for
+// the extra pass; the sectioned sweep gains 0.8% per iteration).  The fused
+// partials partition the rows differently, so the double-double r.z can
+// round to another double when its exact value lies within ~2^-100
+// (relative) of a rounding boundary; the separate pass is the default.
 static bool fuse_rz_enabled() {
   static const bool on = [] {
     const char *e = std::getenv("ICCG_FUSE_RZ");
