@@ -575,10 +575,17 @@ static void enqueue_iteration(iccg_solver_s &S, int it) {
     launch_spmv_cg(A, S.p, S.Ap, S.S, S.part, S.st, false);
   }
   if (M.kind == ICCG_PRECOND_IC0) {
+    // with r.z in its own pass, the stop test waits for it (one finish
+    // kernel fewer per iteration); with r.z fused into the backward sweep
+    // the test stays before the preconditioner
+    const bool fused = bwd_fuses_rz(M.sc, M.F);
     {
       Timer t(S, 3, it);
       launch_fin_alpha(A, S.S, S.part, S.st);
-      launch_xr(PK_IC, n, S.x, S.p, S.r, S.Ap, nullptr, nullptr, S.S, S.part, S.st);
+      if (fused)
+        launch_xr(PK_IC, n, S.x, S.p, S.r, S.Ap, nullptr, nullptr, S.S, S.part, S.st);
+      else
+        launch_xr_ic(n, S.x, S.p, S.r, S.Ap, S.S, S.part, S.st);
     }
     {
       Timer t(S, 1, it);
@@ -589,7 +596,7 @@ static void enqueue_iteration(iccg_solver_s &S, int it) {
       launch_bwd(M.sc, M.F, M.mixed, S.y, S.z, S.S, S.st, S.r, S.S, S.part, 1);
     }
     Timer t(S, 3, it);
-    if (!bwd_fuses_rz(M.sc, M.F)) launch_rz(n, S.r, S.z, M.mixed, S.S, S.part, 0, S.st);
+    if (!fused) launch_rz_stop(n, S.r, S.z, M.mixed, S.S, S.part, S.st);
     launch_update_p(n, S.z, M.mixed, S.p, S.S, 0, S.st);
   } else {
     const int pk = M.kind == ICCG_PRECOND_JACOBI ? PK_JAC : PK_IDENT;

@@ -246,6 +246,37 @@ __global__ __launch_bounds__(kFinWG) void k_fin_xr(const double *__restrict__ pa
   }
 }
 
+// The IC(0) iteration's stop test (cg_crs.cpp:143-149) taken after the
+// preconditioner instead of before it: the same |r| partials, the same
+// decision, and when the iteration goes on, beta from r.z (152-153).  In the
+// iteration that stops, the preconditioner and r.z ran for nothing and p is
+// left as it was (k_update_p sees done), so x, r, iters and rel are those of
+// the immediate test.
+__global__ __launch_bounds__(kFinWG) void k_fin_xr_rz(const double *__restrict__ part_xr,
+                                                   int count_xr,
+                                                   const double *__restrict__ part_rz,
+                                                   int count_rz, Scal *S) {
+  if (S->done) return;
+  double rr[2], rz[1];
+  const bool t0 = finish_reduce<2>(part_xr, count_xr, rr);
+  finish_reduce<1>(part_rz, count_rz, rz);
+  if (t0) {
+    const double rel = sqrt(rr[0]) / S->normb;
+    const int it = S->iters + 1;
+    S->iters = it;
+    S->rel = rel;
+    if (rel < S->tol) {
+      S->converged = 1;
+      S->done = 1;
+    } else if (it >= S->max_iter) {
+      S->done = 1;
+    } else {
+      S->beta = rz[0] / S->rz_old;
+      S->rz_old = rz[0];
+    }
+  }
+}
+
 // z0 = M^-1 r0, p = z0, rz_old = r.z (cg_crs.cpp:112-115) for Identity/Jacobi
 template <int PK>
 __global__ __launch_bounds__(kWG) void k_precond_diag_init(int n, const double *__restrict__ r,
@@ -824,6 +855,27 @@ void launch_xr(int pk, int n, double *x, const double *p, double *r, const doubl
     hipLaunchKernelGGL(k_xr<PK_IC>, g, b, 0, st, n, x, p, r, Ap, inv_diag, z, S, part);
     hipLaunchKernelGGL(k_fin_xr<PK_IC>, dim3(1), dim3(kFinWG), 0, st, part, (int)g.x, S);
   }
+}
+
+// the |r|^2 partials of the deferred stop test: partial pairs from
+// kMaxRedBlocks on, clear of the r.z partials (pairs below kMaxRedBlocks)
+static double *xr_parts(double *part) { return part + 2 * kMaxRedBlocks; }
+
+void launch_xr_ic(int n, double *x, const double *p, double *r, const double *Ap, Scal *S,
+                  double *part, hipStream_t st) {
+  hipLaunchKernelGGL(k_xr<PK_IC>, dim3(quad_grid(n)), dim3(kWG), 0, st, n, x, p, r, Ap, nullptr,
+                     nullptr, S, xr_parts(part));
+}
+
+void launch_rz_stop(int n, const double *r, const void *z, bool z32, Scal *S, double *part,
+                    hipStream_t st) {
+  dim3 g(quad_grid(n)), b(kWG);
+  if (z32)
+    hipLaunchKernelGGL(k_rz<float>, g, b, 0, st, n, r, (const float *)z, S, part, 0);
+  else
+    hipLaunchKernelGGL(k_rz<double>, g, b, 0, st, n, r, (const double *)z, S, part, 0);
+  hipLaunchKernelGGL(k_fin_xr_rz, dim3(1), dim3(kFinWG), 0, st, xr_parts(part), (int)g.x, part,
+                     (int)g.x, S);
 }
 
 void launch_precond_diag_init(int pk, int n, const double *r, const double *inv_diag, double *z,
