@@ -93,26 +93,36 @@ def time_solves(S, steps, warmup, max_iter, tol, true_res):
 
 def kernel_table(S, n, nnz, nnz_l, mixed, nc, iters):
     """Per kernel class, from HIP events recorded on the solver's stream
-    around every launch group during the timed solves."""
-    names = ["spmv_dots", "fwd_sweep", "bwd_sweep", "vector_ops"]
+    around every launch group during the timed solves.  When the plan runs
+    the last color forward and backward in one launch (abmc-sectioned-fused), the
+    forward group holds that color's backward rows too, so the two sweeps
+    are reported as one class."""
+    import iccg_amd as I
     fwd = trsv_bytes(n, nnz_l, 4, 8, 4) if mixed else trsv_bytes(n, nnz_l, 8, 8, 8)
     bwd = trsv_bytes(n, nnz_l, 4, 4, 4) if mixed else trsv_bytes(n, nnz_l, 8, 8, 8)
-    per_unit = [spmv_bytes(n, nnz), fwd, bwd, 80 * n]
-    units = ["one SpMV launch (k_spmv_cg)",
-             f"one forward sweep ({nc} color launches)",
-             f"one backward sweep ({nc} color launches)",
-             "one iteration's vector and scalar work (alpha finish, x/r update + finish, "
-             "r.z + finish, p update)"]
+    fused = isinstance(S.M, I.IC0) and S.M.plan()[0] == "abmc-sectioned-fused"
+    cls = [("spmv_dots", [0], spmv_bytes(n, nnz), "one SpMV launch (k_spmv_cg)")]
+    if fused:
+        cls.append(("sweeps", [1, 2], fwd + bwd,
+                    f"one forward and one backward sweep ({2 * nc - 1} color launches: "
+                    "the last color forward and backward in one)"))
+    else:
+        cls.append(("fwd_sweep", [1], fwd, f"one forward sweep ({nc} color launches)"))
+        cls.append(("bwd_sweep", [2], bwd, f"one backward sweep ({nc} color launches)"))
+    cls.append(("vector_ops", [3], 80 * n,
+                "one iteration's vector and scalar work (alpha finish, x/r update + finish, "
+                "r.z + finish, p update)"))
     out = {}
-    for c, nm in enumerate(names):
-        cnt, ms = S.timing(c)
+    for nm, groups, byt, unit in cls:
+        cnt, ms = S.timing(groups[0])
         if cnt == 0:
             continue
-        per = cnt if c < 3 else max(iters, 1)
+        ms = sum(S.timing(g)[1] for g in groups)
+        per = cnt if groups[0] < 3 else max(iters, 1)
         avg = ms / per
-        gbs = per_unit[c] / (avg * 1e-3) / 1e9
-        out[nm] = {"count": per, "avg_ms": round(avg, 5), "bytes": int(per_unit[c]),
-                   "gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "unit": units[c]}
+        gbs = byt / (avg * 1e-3) / 1e9
+        out[nm] = {"count": per, "avg_ms": round(avg, 5), "bytes": int(byt),
+                   "gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "unit": unit}
     return out
 
 

@@ -422,6 +422,12 @@ static void build_sliced_factor(iccg_precond_s &M, const std::vector<int> &lrp,
   }
   if (std::max(lw, uw) > 16) return;  // wide rows: CSR sweeps only
   F.sec = S;
+  F.fb = false;
+  if (S > 0 && M.sc.nc > 0) {
+    F.fb = true;
+    for (int kb = M.sc.h_color_blk_v[M.sc.nc - 1]; kb < M.sc.h_color_blk_v[M.sc.nc]; ++kb)
+      if (uoff[(size_t)kb + 1] - uoff[(size_t)kb] != 64 * S) F.fb = false;
+  }
   for (int i = 0; i < n; ++i) l2ls[(size_t)lrp[i + 1] - 1] = -1 - i;
   for (long k = 0; k < nnz; ++k) l2us[k] = upos[(size_t)l2u[k]];
   auto up = [&](Sell &S, std::vector<int> &off, std::vector<int> &col, int w) {
@@ -556,8 +562,9 @@ static void precond_init(iccg_solver_s &S) {
   iccg_precond_s &M = *S.M;
   const int n = S.n;
   if (M.kind == ICCG_PRECOND_IC0) {
-    launch_fwd(M.sc, M.F, M.mixed, S.r, S.y, S.S, S.st);
-    launch_bwd(M.sc, M.F, M.mixed, S.y, S.z, S.S, S.st, S.r, S.S, S.part, 2);
+    const bool fb = sweeps_fused(M.sc, M.F);
+    launch_fwd(M.sc, M.F, M.mixed, S.r, S.y, S.S, S.st, fb ? S.z : nullptr);
+    launch_bwd(M.sc, M.F, M.mixed, S.y, S.z, S.S, S.st, S.r, S.S, S.part, 2, fb);
     if (!bwd_fuses_rz(M.sc, M.F)) launch_rz(n, S.r, S.z, M.mixed, S.S, S.part, 1, S.st);
     launch_update_p(n, S.z, M.mixed, S.p, S.S, 1, S.st);
   } else {
@@ -587,13 +594,16 @@ static void enqueue_iteration(iccg_solver_s &S, int it) {
       else
         launch_xr_ic(n, S.x, S.p, S.r, S.Ap, S.S, S.part, S.st);
     }
+    // fused: the last color's backward rows run in the forward timer's
+    // launch group
+    const bool fb = sweeps_fused(M.sc, M.F);
     {
       Timer t(S, 1, it);
-      launch_fwd(M.sc, M.F, M.mixed, S.r, S.y, S.S, S.st);
+      launch_fwd(M.sc, M.F, M.mixed, S.r, S.y, S.S, S.st, fb ? S.z : nullptr);
     }
     {
       Timer t(S, 2, it);
-      launch_bwd(M.sc, M.F, M.mixed, S.y, S.z, S.S, S.st, S.r, S.S, S.part, 1);
+      launch_bwd(M.sc, M.F, M.mixed, S.y, S.z, S.S, S.st, S.r, S.S, S.part, 1, fb);
     }
     Timer t(S, 3, it);
     if (!fused) launch_rz_stop(n, S.r, S.z, M.mixed, S.S, S.part, S.st);
@@ -937,6 +947,7 @@ int iccg_precond_set_schedule(iccg_precond_t M, iccg_sched_t sched) {
       F.d32 = nullptr;
       F.sell = false;
       F.sec = 0;
+      F.fb = false;
       build_sliced_factor(*M, lrp, lci, urp, uci, l2u);
     }
     M->plan_version++;
@@ -955,7 +966,7 @@ int iccg_precond_get_plan(iccg_precond_t M, int *plan, int *section_width) {
     else if (sc.kind == 1)
       *plan = ICCG_PLAN_MC_SLICED;
     else if (M->F.sec > 0) {
-      *plan = ICCG_PLAN_ABMC_SECTIONED;
+      *plan = sweeps_fused(sc, M->F) ? ICCG_PLAN_ABMC_SECTIONED_FUSED : ICCG_PLAN_ABMC_SECTIONED;
       *section_width = M->F.sec;
     } else
       *plan = ICCG_PLAN_ABMC_SLICED;
@@ -977,14 +988,16 @@ int iccg_precond_apply(iccg_precond_t M, const double *r, double *z) {
       launch_jacobi_apply(n, M->inv_diag, dr, dz, M->st);
     } else if (!M->mixed) {
       double *dy = dalloc<double>((size_t)n);
-      launch_fwd(M->sc, M->F, false, dr, dy, nullptr, M->st);
-      launch_bwd(M->sc, M->F, false, dy, dz, nullptr, M->st);
+      const bool fb = sweeps_fused(M->sc, M->F);
+      launch_fwd(M->sc, M->F, false, dr, dy, nullptr, M->st, fb ? dz : nullptr);
+      launch_bwd(M->sc, M->F, false, dy, dz, nullptr, M->st, nullptr, nullptr, nullptr, 0, fb);
       HIPCHK(hipStreamSynchronize(M->st));
       dfree(dy);
     } else {
       float *dy = dalloc<float>((size_t)n), *dz32 = dalloc<float>((size_t)n);
-      launch_fwd(M->sc, M->F, true, dr, dy, nullptr, M->st);
-      launch_bwd(M->sc, M->F, true, dy, dz32, nullptr, M->st);
+      const bool fb = sweeps_fused(M->sc, M->F);
+      launch_fwd(M->sc, M->F, true, dr, dy, nullptr, M->st, fb ? dz32 : nullptr);
+      launch_bwd(M->sc, M->F, true, dy, dz32, nullptr, M->st, nullptr, nullptr, nullptr, 0, fb);
       launch_cast_f64(dz32, dz, n, M->st);
       HIPCHK(hipStreamSynchronize(M->st));
       dfree(dy);

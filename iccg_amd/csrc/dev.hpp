@@ -126,6 +126,9 @@ struct Factor {
   // ABMC factors: ls / us in the sectioned layout with S in-block and up to
   // S outside entries per row (sell_layout_sections), or 0 for the plain one
   int sec = 0;
+  // sectioned, and every block of the last color has no outside entries in
+  // U (its backward rows have in-block terms only)
+  bool fb = false;
 };
 
 // ---- multi-system engine (batch.hip) --------------------------------------
@@ -218,11 +221,16 @@ void launch_jacobi_setup(const DevCsr &A, double *inv_diag, hipStream_t st);
 // sweeps: skip when S != nullptr && S->done.  With rz != 0 the backward
 // sweep also forms r.z and, in its last workgroup, beta (rz == 2: rz_old at
 // the start of a solve), like launch_rz.
+// Sectioned ABMC factors whose last color's backward rows have in-block
+// terms only can run that color forward and backward in one launch:
+// launch_fwd with z_fused writes z for the last color (and not its y), and
+// launch_bwd with last_done then skips that color.
+bool sweeps_fused(const DevSched &sc, const Factor &F);
 void launch_fwd(const DevSched &sc, const Factor &F, bool mixed, const double *r, void *y,
-                const Scal *S, hipStream_t st);
+                const Scal *S, hipStream_t st, void *z_fused = nullptr);
 void launch_bwd(const DevSched &sc, const Factor &F, bool mixed, const void *y, void *z,
                 const Scal *S, hipStream_t st, const double *r = nullptr, Scal *Sw = nullptr,
-                double *part = nullptr, int rz = 0);
+                double *part = nullptr, int rz = 0, bool last_done = false);
 // true when launch_bwd fuses r.z for this schedule (sliced MC / ABMC plans)
 // the triangular solves run on the sliced-ELL sweeps (contiguous MC colors,
 // or contiguous ABMC blocks of at most 64 rows; rows at most 16 wide)

@@ -608,7 +608,19 @@ __global__ __launch_bounds__(kWG) void k_abmc_bwd_wave(
 // it (measured at 192^3: FP32 forward / backward 88.7 / 85.3 -> 85.5 /
 // 78.5 us pipelined; FP64 121 / 118 us not pipelined vs 132 / 131
 // pipelined, where the second block's registers cost occupancy).
-template <class VT, class YT, int SW>
+//
+// With FB (the last color) each block's backward rows follow its forward
+// rows at once: the last color has no later colors, so its backward rows
+// have in-block terms only, start from the y just computed (kept in
+// registers, never stored) and the same diagonal; U's in-block entries are
+// loaded while the forward levels run.
+struct SecU {  // U's sectioned copy and the backward level counts (FB)
+  const int *nlev, *off;
+  const unsigned *inb;
+  const void *v;
+};
+
+template <class VT, class YT, int SW, bool FB>
 __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
                                                       const int *__restrict__ bstart,
                                                       const int *__restrict__ nlev,
@@ -618,7 +630,8 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
                                                       const VT *__restrict__ lv,
                                                       const VT *__restrict__ diag,
                                                       const double *__restrict__ r,
-                                                      YT *__restrict__ y, const Scal *S) {
+                                                      YT *__restrict__ y, SecU U,
+                                                      YT *__restrict__ z, const Scal *S) {
   constexpr bool PIPE = sizeof(YT) == 4;
   __shared__ YT win[(kWG / 64) * kSecWindow];
   if (S && S->done) return;
@@ -658,13 +671,30 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
     const int nw = PIPE ? (off[kc + 1] - no0) >> 6 : 0;
     if (PIPE)
       sec_fetch<VT, YT, SW, true>(kc, no0, nw, nbs, nbe, lane, inb, col, lv, diag, r, y, cur);
+    int uad[SW];
+    YT uva[SW];
+    if (FB) {  // this block's U entries, in flight during the forward levels
+      const int t = min(lane, be - bs - 1), uo = U.off[kb] + t;
+      const unsigned pk = U.inb[64 * kb + t];
+#pragma unroll
+      for (int k = 0; k < SW; ++k) {
+        uad[k] = (int)((pk >> (8 * k)) & 0xFFu);
+        uva[k] = (YT)((const VT *)U.v)[uo + 64 * k];  // absent entries hold 0
+      }
+    }
     // the earlier colors' terms, ascending; absent ones are fma(-0, 0, acc)
 #pragma unroll
     for (int k = 0; k < SW; ++k) acc = fma_t(-vo[k], co[k] >= 0 ? xo[k] : (YT)0, acc);
     if (lane == 0) my[64] = (YT)0;
     wave_sync();
     const YT yi = sec_levels<YT, SW, false>(nl, acc, d, ad, va, none, none, my, lane, act);
-    if (act) y[bs + lane] = yi;
+    if (FB) {
+      const YT zi = sec_levels<YT, SW, false>(U.nlev[kb], yi, d, uad, uva, none, none, my, lane,
+                                              act);
+      if (act) z[bs + lane] = zi;
+    } else if (act) {
+      y[bs + lane] = yi;
+    }
     if (PIPE) {
       bs = nbs;
       be = nbe;
@@ -1094,26 +1124,38 @@ static void bwd_sliced(const DevSched &sc, const Factor &F, const VT *uv, const 
   if (fuse) hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(kFinWG), 0, st, part, total, S, rz == 2);
 }
 
+// forward colors [0, c_end); with uv, the last color through the fused
+// forward + backward launch as well
 template <class VT, class YT, int SW>
 static void fwd_sec(const DevSched &sc, const Factor &F, const VT *lv, const VT *dg,
-                    const double *r, YT *y, const Scal *S, hipStream_t st) {
+                    const double *r, YT *y, const Scal *S, hipStream_t st, const VT *uv = nullptr,
+                    YT *z = nullptr) {
+  const SecU none{nullptr, nullptr, nullptr, nullptr};
   for (int c = 0; c < sc.nc; ++c) {
     const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
     if (b1 <= b0) continue;
-    hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW>), dim3(slice_grid(b1 - b0)), dim3(kWG), 0,
-                       st, b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.inb, F.ls.col, lv, dg, r, y, S);
+    if (uv && c == sc.nc - 1)
+      hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, true>), dim3(slice_grid(b1 - b0)),
+                         dim3(kWG), 0, st, b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.inb,
+                         F.ls.col, lv, dg, r, y,
+                         SecU{sc.d_nlev_b, F.us.off, F.us.inb, (const void *)uv}, z, S);
+    else
+      hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, false>), dim3(slice_grid(b1 - b0)),
+                         dim3(kWG), 0, st, b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.inb,
+                         F.ls.col, lv, dg, r, y, none, nullptr, S);
   }
 }
 
 template <class VT, class YT, int SW>
 static void bwd_sec(const DevSched &sc, const Factor &F, const VT *uv, const VT *dg, const YT *y,
-                    YT *z, const double *r, Scal *S, double *part, int rz, hipStream_t st) {
+                    YT *z, const double *r, Scal *S, double *part, int rz, hipStream_t st,
+                    bool skip_last = false) {
   std::vector<int> grids;  // workgroups of every color launch, for the r.z partials
   int total = 0;
   for (int t = 0; t < sc.nc; ++t) {
     const int c = sc.nc - 1 - t;
     const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
-    grids.push_back(b1 > b0 ? slice_grid(b1 - b0) : 0);
+    grids.push_back(b1 > b0 && !(skip_last && t == 0) ? slice_grid(b1 - b0) : 0);
     total += grids.back();
   }
   const bool fuse = rz != 0 && r != nullptr && fuse_rz_enabled();
@@ -1155,13 +1197,20 @@ static void bwd_sec(const DevSched &sc, const Factor &F, const VT *uv, const VT 
     }                                            \
   } while (0)
 
+bool sweeps_fused(const DevSched &sc, const Factor &F) {
+  return sliced_plan(sc, F) && sc.kind == 2 && F.sec > 0 && F.fb && !fuse_rz_enabled();
+}
+
 void launch_fwd(const DevSched &sc, const Factor &F, bool mixed, const double *r, void *y,
-                const Scal *S, hipStream_t st) {
+                const Scal *S, hipStream_t st, void *z_fused) {
   if (sliced_plan(sc, F) && sc.kind == 2 && F.sec > 0) {
+    const bool fb = z_fused && sweeps_fused(sc, F);
     if (mixed)
-      DISPATCH_SEC(F.sec, fwd_sec, float, float, sc, F, F.ls.v32, F.d32, r, (float *)y, S, st);
+      DISPATCH_SEC(F.sec, fwd_sec, float, float, sc, F, F.ls.v32, F.d32, r, (float *)y, S, st,
+                   fb ? F.us.v32 : nullptr, (float *)z_fused);
     else
-      DISPATCH_SEC(F.sec, fwd_sec, double, double, sc, F, F.ls.v64, F.d64, r, (double *)y, S, st);
+      DISPATCH_SEC(F.sec, fwd_sec, double, double, sc, F, F.ls.v64, F.d64, r, (double *)y, S, st,
+                   fb ? F.us.v64 : nullptr, (double *)z_fused);
     return;
   }
   if (sliced_plan(sc, F)) {
@@ -1184,14 +1233,16 @@ void launch_fwd(const DevSched &sc, const Factor &F, bool mixed, const double *r
 }
 
 void launch_bwd(const DevSched &sc, const Factor &F, bool mixed, const void *y, void *z,
-                const Scal *S, hipStream_t st, const double *r, Scal *Sw, double *part, int rz) {
+                const Scal *S, hipStream_t st, const double *r, Scal *Sw, double *part, int rz,
+                bool last_done) {
   if (sliced_plan(sc, F) && sc.kind == 2 && F.sec > 0) {
+    const bool skip = last_done && sweeps_fused(sc, F);
     if (mixed)
       DISPATCH_SEC(F.sec, bwd_sec, float, float, sc, F, F.us.v32, F.d32, (const float *)y,
-                   (float *)z, r, Sw, part, rz, st);
+                   (float *)z, r, Sw, part, rz, st, skip);
     else
       DISPATCH_SEC(F.sec, bwd_sec, double, double, sc, F, F.us.v64, F.d64, (const double *)y,
-                   (double *)z, r, Sw, part, rz, st);
+                   (double *)z, r, Sw, part, rz, st, skip);
     return;
   }
   if (sliced_plan(sc, F)) {

@@ -228,7 +228,9 @@ enum {
   ICCG_PLAN_CSR = 2,             /* MC or ABMC over the CSR factor */
   ICCG_PLAN_MC_SLICED = 3,       /* multicolor, sliced ELL */
   ICCG_PLAN_ABMC_SLICED = 4,     /* ABMC, sliced ELL, masked level loop */
-  ICCG_PLAN_ABMC_SECTIONED = 5   /* ABMC, sectioned layout, mask-free levels */
+  ICCG_PLAN_ABMC_SECTIONED = 5,  /* ABMC, sectioned layout, mask-free levels */
+  ICCG_PLAN_ABMC_SECTIONED_FUSED = 6  /* the same, the last color forward and
+                                         backward in one launch */
 };
 int iccg_precond_get_plan(iccg_precond_t M, int *plan, int *section_width);
 /* apply(r, z, sched) (precond.hpp:12,38; ic0.hpp:34) on host buffers. */
