@@ -400,7 +400,11 @@ __device__ __forceinline__ void abmc_bwd_group(int bs, int be, int nl, const Sys
 // section width D0.sec <= SW): the single-system sweeps' mask-free level
 // loop (sec_levels) run for every live member on the shared entries.  my
 // holds one kSecWindow window per member (row slots and the zero slot).
-template <class FT, int SW, int KM, bool FWD>
+// FB (forward, blocks of the factor's last color): each member's backward
+// rows follow its forward rows at once, from the y just computed (not
+// stored); the last color has no later colors, so they have in-block terms
+// only, and the result is the single-system sweeps' z bit for bit.
+template <class FT, int SW, int KM, bool FWD, bool FB = false>
 __device__ __forceinline__ void abmc_group_sec(int kb, int bs, int be, int nl, const SysDesc &D0,
                                                const GroupDesc &g,
                                                const SysDesc *__restrict__ desc,
@@ -408,14 +412,14 @@ __device__ __forceinline__ void abmc_group_sec(int kb, int bs, int be, int nl, c
   const int i = bs + lane;
   const bool act = i < be;
   const int sec = D0.sec;
-  int ad[SW], co[SW];
-  FT va[SW], vo[SW];
+  int ad[SW], co[SW], uad[SW];
+  FT va[SW], vo[SW], uva[SW];
   FT acc[KM], xo[KM][SW], d = (FT)1;
 #pragma unroll
   for (int k = 0; k < SW; ++k) {
-    ad[k] = 64;
+    ad[k] = uad[k] = 64;
     co[k] = -1;
-    va[k] = vo[k] = (FT)0;
+    va[k] = vo[k] = uva[k] = (FT)0;
   }
 #pragma unroll
   for (int j = 0; j < KM; ++j) {
@@ -443,6 +447,16 @@ __device__ __forceinline__ void abmc_group_sec(int kb, int bs, int be, int nl, c
       }
     }
     d = ((const FT *)D0.d)[i];
+    if (FB) {  // U's in-block entries of the row (its only ones)
+      const unsigned pu = D0.u_inb[64 * kb + lane];
+      const int ub = D0.u_off[kb] + lane;
+#pragma unroll
+      for (int k = 0; k < SW; ++k)
+        if (k < sec) {
+          uad[k] = (int)((pu >> (8 * k)) & 0xFFu);
+          uva[k] = ((const FT *)D0.u_v)[ub + 64 * k];
+        }
+    }
 #pragma unroll
     for (int j = 0; j < KM; ++j) {
       if (!live[j]) continue;
@@ -473,7 +487,13 @@ __device__ __forceinline__ void abmc_group_sec(int kb, int bs, int be, int nl, c
     if (!live[j]) continue;
     const FT zi = sec_levels<FT, SW, !FWD>(nl, acc[j], d, ad, va, vo, xo[j],
                                                  my + kSecWindow * j, lane, act);
-    if (act) ((FT *)(FWD ? desc[g.slot[j]].y : desc[g.slot[j]].z))[i] = zi;
+    if (FB) {
+      const FT zb = sec_levels<FT, SW, false>(D0.nlev_b[kb], zi, d, uad, uva, vo, xo[j],
+                                              my + kSecWindow * j, lane, act);
+      if (act) ((FT *)desc[g.slot[j]].z)[i] = zb;
+    } else if (act) {
+      ((FT *)(FWD ? desc[g.slot[j]].y : desc[g.slot[j]].z))[i] = zi;
+    }
   }
 }
 
@@ -525,8 +545,9 @@ __global__ __launch_bounds__(kWG) void kb_sweep(const GroupDesc *__restrict__ gr
   }
 }
 
-// one wavefront per color item of a sectioned-layout ABMC group
-template <int SW, int KM, bool FWD>
+// one wavefront per color item of a sectioned-layout ABMC group; FB: the
+// items of factors whose last color this is, forward and backward
+template <int SW, int KM, bool FWD, bool FB>
 __global__ __launch_bounds__(kWG) void kb_sweep_sec(const GroupDesc *__restrict__ groups,
                                                     const SysDesc *__restrict__ desc,
                                                     const ColorItem *__restrict__ items,
@@ -551,9 +572,10 @@ __global__ __launch_bounds__(kWG) void kb_sweep_sec(const GroupDesc *__restrict_
     const int bs = D0.bstart[kb], be = D0.bstart[kb + 1];
     const int nl = FWD ? D0.nlev_f[kb] : D0.nlev_b[kb];
     if (D0.mixed)
-      abmc_group_sec<float, SW, KM, FWD>(kb, bs, be, nl, D0, g, desc, live, (float *)my, lane);
+      abmc_group_sec<float, SW, KM, FWD, FB>(kb, bs, be, nl, D0, g, desc, live, (float *)my,
+                                             lane);
     else
-      abmc_group_sec<double, SW, KM, FWD>(kb, bs, be, nl, D0, g, desc, live, my, lane);
+      abmc_group_sec<double, SW, KM, FWD, FB>(kb, bs, be, nl, D0, g, desc, live, my, lane);
   }
 }
 
@@ -670,8 +692,10 @@ struct ItemSet {
   int ngroups = 0, nchunks = 0, nfin = 0, ncolors = 0, kmax = 1;
   std::vector<int> color_off;  // ncolors + 1 offsets into d_color
   // per color: the leading items of sectioned-layout factors (kb_sweep_sec),
-  // and their largest section width
-  std::vector<int> color_sec;
+  // and their largest section width; of those, the leading color_fb items
+  // belong to factors whose last color this is, and their forward launch
+  // runs their backward rows as well (the backward launch skips them)
+  std::vector<int> color_sec, color_fb;
   int secw = 0;
   std::vector<int> work_of;    // per listed slot (fin order): its group's chunks
   char *buf = nullptr;
@@ -813,7 +837,7 @@ void build_items(iccg_batch_s &B, const std::vector<int> &ids, ItemSet &I, int c
   }
   std::vector<int> gchunk0, gchunks;
   std::vector<ChunkItem> chunks;
-  std::vector<std::vector<ColorItem>> color, color_sec;
+  std::vector<std::vector<ColorItem>> color, color_sec, color_fb;
   int kmax = 1, secw = 0;
   for (int gi = 0; gi < (int)groups.size(); ++gi) {
     const GroupDesc &g = groups[(size_t)gi];
@@ -828,15 +852,17 @@ void build_items(iccg_batch_s &B, const std::vector<int> &ids, ItemSet &I, int c
     if ((int)color.size() < nc) {
       color.resize(nc);
       color_sec.resize(nc);
+      color_fb.resize(nc);
     }
     const int sec = sc.kind == 2 ? s.M->F.sec : 0;
+    const bool fb = sec > 0 && s.M->F.fb;
     secw = std::max(secw, sec);
     for (int c = 0; c < nc; ++c) {
       if (sc.kind == 2) {
         const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
+        auto &list = sec == 0 ? color : fb && c == nc - 1 ? color_fb : color_sec;
         for (int kb = b0; kb < b1; kb += cw)
-          (sec > 0 ? color_sec : color)[c].push_back(
-              {gi, kb, std::min(b1, kb + cw), 0, 0, g.slot[0], {0, 0}});
+          list[c].push_back({gi, kb, std::min(b1, kb + cw), 0, 0, g.slot[0], {0, 0}});
       } else {
         const int lo = sc.h_color_ptr_v[c], hi = sc.h_color_ptr_v[c + 1];
         if (hi <= lo) continue;
@@ -861,9 +887,12 @@ void build_items(iccg_batch_s &B, const std::vector<int> &ids, ItemSet &I, int c
   I.secw = secw;
   I.color_off.assign(1, 0);
   I.color_sec.clear();
+  I.color_fb.clear();
   for (size_t c = 0; c < color.size(); ++c) {
-    I.color_sec.push_back((int)color_sec[c].size());
+    I.color_fb.push_back((int)color_fb[c].size());
+    I.color_sec.push_back((int)(color_fb[c].size() + color_sec[c].size()));
     color[c].insert(color[c].begin(), color_sec[c].begin(), color_sec[c].end());
+    color[c].insert(color[c].begin(), color_fb[c].begin(), color_fb[c].end());
     I.color_off.push_back(I.color_off.back() + (int)color[c].size());
   }
   if (I.nfin == 0) return;
@@ -919,6 +948,9 @@ int km_class(int k) { return k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : 8; }
 #define NOEXTRA
 #define FWD_T , true
 #define BWD_T , false
+#define SFB_T , true, true
+#define SFWD_T , true, false
+#define SBWD_T , false, false
 
 // kernel classes of the timing mode
 enum { T_SPMV = 0, T_FWD = 1, T_BWD = 2, T_VEC = 3, T_NCLASS = 4 };
@@ -949,15 +981,18 @@ void launch_resid_items(iccg_batch_s &B, const ItemSet &I, int mode) {
 
 template <bool FWD>
 static void launch_color(iccg_batch_s &B, const ItemSet &I, int c) {
-  const int n0 = I.color_sec[c], ni = I.color_off[c + 1] - I.color_off[c];
+  const int nf = I.color_fb[c], n0 = I.color_sec[c], ni = I.color_off[c + 1] - I.color_off[c];
   const ColorItem *items = I.d_color + I.color_off[c];
-  if (n0 > 0) {
+  if (FWD && nf > 0)
+    SEC_DISPATCH(I.secw, I.kmax, kb_sweep_sec, SFB_T, dim3((nf + 3) / 4), dim3(kWG), 0, B.st,
+                 I.d_groups, B.d_desc, items, nf, B.d_sc);
+  if (n0 > nf) {
     if (FWD)
-      SEC_DISPATCH(I.secw, I.kmax, kb_sweep_sec, FWD_T, dim3((n0 + 3) / 4), dim3(kWG), 0, B.st,
-                   I.d_groups, B.d_desc, items, n0, B.d_sc);
+      SEC_DISPATCH(I.secw, I.kmax, kb_sweep_sec, SFWD_T, dim3((n0 - nf + 3) / 4), dim3(kWG), 0,
+                   B.st, I.d_groups, B.d_desc, items + nf, n0 - nf, B.d_sc);
     else
-      SEC_DISPATCH(I.secw, I.kmax, kb_sweep_sec, BWD_T, dim3((n0 + 3) / 4), dim3(kWG), 0, B.st,
-                   I.d_groups, B.d_desc, items, n0, B.d_sc);
+      SEC_DISPATCH(I.secw, I.kmax, kb_sweep_sec, SBWD_T, dim3((n0 - nf + 3) / 4), dim3(kWG), 0,
+                   B.st, I.d_groups, B.d_desc, items + nf, n0 - nf, B.d_sc);
   }
   if (ni > n0) {
     if (FWD)
