@@ -230,6 +230,19 @@ __device__ __forceinline__ void st4(double *p, int i, const double (&o)[4]) {
   *reinterpret_cast<double2 *>(p + i) = make_double2(o[0], o[1]);
   *reinterpret_cast<double2 *>(p + i + 2) = make_double2(o[2], o[3]);
 }
+// The same, nontemporal: for vectors that no later kernel of the iteration
+// reads, so that they do not displace the ones that are read again
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void ld4_nt(const double *p, int i, double (&o)[4]) {
+  const dvec2 u = __builtin_nontemporal_load(reinterpret_cast<const dvec2 *>(p + i));
+  const dvec2 w = __builtin_nontemporal_load(reinterpret_cast<const dvec2 *>(p + i + 2));
+  o[0] = u.x; o[1] = u.y; o[2] = w.x; o[3] = w.y;
+}
+__device__ __forceinline__ void st4_nt(double *p, int i, const double (&o)[4]) {
+  const dvec2 u = {o[0], o[1]}, w = {o[2], o[3]};
+  __builtin_nontemporal_store(u, reinterpret_cast<dvec2 *>(p + i));
+  __builtin_nontemporal_store(w, reinterpret_cast<dvec2 *>(p + i + 2));
+}
 
 // ---------------------------------------------------------------------------
 // Sliced-ELL row chains
@@ -316,6 +329,59 @@ __device__ __forceinline__ AT sell_chain16(const int *__restrict__ off,
         acc = NEG ? fma_t(-a, (AT)x[c], acc) : fma_t(a, (AT)x[c], acc);
       }
     }
+  }
+  return acc;
+}
+
+// The lane's row sum 0 + v_0 x[c_0] + v_1 x[c_1] + ... (ascending stored
+// order, one fma per entry) over slice s of the 16-bit layout, for a
+// wave-uniform s (the caller's loop runs on readfirstlane values, so the
+// slice header comes from scalar loads).  Every load is issued without a
+// branch: the table entry, all WM columns, then all WM values (entries
+// k >= w load entry 0 again, in a line already fetched), so the one wait
+// before the table lookups covers the table and the columns only, and the
+// gathers x[c] (x[0] for padding) go out together.  Padding terms are
+// skipped by a select, as the reference's CSR loop never sees them.
+template <int WM>
+__device__ __forceinline__ double spmv_slice16(const int *__restrict__ off,
+                                               const uint16_t *__restrict__ col16,
+                                               const int *__restrict__ tbl, int *tb,
+                                               const double *__restrict__ v,
+                                               const double *__restrict__ x, int s, int lane) {
+  const int o0 = off[s];
+  const int w = (off[s + 1] - o0) >> 6;
+  if (w == 0) return 0.0;  // a slice of empty rows (uniform): entry 0 may not exist
+  const int base = o0 + lane;
+  const int te = tbl[16 * s + (lane & 15)];
+  unsigned cr[WM];
+  double vv[WM];
+#pragma unroll
+  for (int k = 0; k < WM; ++k) cr[k] = __builtin_nontemporal_load(&col16[k < w ? base + 64 * k : base]);
+#pragma unroll
+  for (int k = 0; k < WM; ++k) vv[k] = __builtin_nontemporal_load(&v[k < w ? base + 64 * k : base]);
+  wave_sync();  // the previous slice's table reads are issued
+  tb[lane & 15] = te;
+  wave_sync();
+  int cc[WM], t[WM];
+#pragma unroll
+  for (int k = 0; k < WM; ++k) {
+    cr[k] = k < w ? cr[k] : 0xFFFFu;
+    t[k] = tb[(cr[k] >> 6) & 15u];
+  }
+  // every lane reads its table entries, all issued before the first is used
+  // (without this the compiler puts a branch and a wait around each read)
+#pragma unroll
+  for (int k = 0; k < WM; ++k) asm volatile("" : "+v"(t[k]));
+#pragma unroll
+  for (int k = 0; k < WM; ++k) cc[k] = cr[k] == 0xFFFFu ? -1 : (t[k] << 6) | (int)(cr[k] & 63u);
+  double xv[WM];
+#pragma unroll
+  for (int k = 0; k < WM; ++k) xv[k] = x[cc[k] >= 0 ? cc[k] : 0];
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < WM; ++k) {
+    const double t = fma(vv[k], xv[k], acc);
+    acc = cc[k] >= 0 ? t : acc;
   }
   return acc;
 }
@@ -571,25 +637,28 @@ __device__ __forceinline__ void sec_fetch(int kb, int o0, int w, int bs, int be,
                                           const YT *__restrict__ yin, SecRaw<VT, YT, S> &e) {
   const int t = min(lane, be - bs - 1);  // a row of the block
   const int base = o0 + t;
-  e.inb = inb[64 * kb + t];
+  // the factor's entries are read once per sweep: nontemporal, so that they
+  // do not push the gathered vector lines out of L2
+  e.inb = __builtin_nontemporal_load(&inb[64 * kb + t]);
 #pragma unroll
-  for (int k = 0; k < S; ++k) e.vin[k] = v[base + 64 * k];  // absent entries hold 0
+  for (int k = 0; k < S; ++k)
+    e.vin[k] = __builtin_nontemporal_load(&v[base + 64 * k]);  // absent entries hold 0
 #pragma unroll
   for (int k = 0; k < S; ++k) {
     if (LOAD_NOW) {
       e.cout_[k] = -1;
       e.vout[k] = (VT)0;
       if (S + k < w) {
-        e.cout_[k] = col[base + 64 * (S + k)];
-        e.vout[k] = v[base + 64 * (S + k)];
+        e.cout_[k] = __builtin_nontemporal_load(&col[base + 64 * (S + k)]);
+        e.vout[k] = __builtin_nontemporal_load(&v[base + 64 * (S + k)]);
       }
     } else {
       // absent: stand-ins in lines this fetch reads anyway (the packed word,
       // the first in-block value), so they cost no extra traffic
       const bool here = S + k < w;
       const int *cp = here ? col + base + 64 * (S + k) : (const int *)inb + 64 * kb + t;
-      e.cout_[k] = *cp;
-      e.vout[k] = v[here ? base + 64 * (S + k) : base];
+      e.cout_[k] = __builtin_nontemporal_load(cp);
+      e.vout[k] = __builtin_nontemporal_load(&v[here ? base + 64 * (S + k) : base]);
     }
   }
   e.d = diag[bs + t];

@@ -41,7 +41,22 @@ __device__ __forceinline__ double row_dot(const int *__restrict__ rp, const int 
 // f(i, Ax_i) for every row i < n; lanes past n get no call.
 template <int WM, class F>
 __device__ __forceinline__ void for_rows_Ax(const DevCsr &A, const double *__restrict__ x, F f) {
-  if (A.sell.off && A.sell.col16) {
+  if (WM > 0 && A.sell.off && A.sell.col16) {
+    constexpr int K = WM > 0 ? WM : 1;
+    __shared__ int tbs[kWG >> 6][16];
+    const int lane = threadIdx.x & 63;
+    int *tb = tbs[threadIdx.x >> 6];
+    const WaveRange wr = xcd_waves(0, A.sell.nslice);
+    const int first = __builtin_amdgcn_readfirstlane(wr.first);
+    const int step = __builtin_amdgcn_readfirstlane(wr.step);
+    const int end = __builtin_amdgcn_readfirstlane(wr.end);
+    for (int s = first; s < end; s += step) {
+      const double y =
+          spmv_slice16<K>(A.sell.off, A.sell.col16, A.sell.tbl, tb, A.sell.v64, x, s, lane);
+      const int i = (s << 6) + lane;
+      if (i < A.n) f(i, y);
+    }
+  } else if (A.sell.off && A.sell.col16) {
     __shared__ int tbs[kWG >> 6][16];
     const int lane = threadIdx.x & 63;
     int *tb = tbs[threadIdx.x >> 6];
@@ -83,7 +98,7 @@ __global__ __launch_bounds__(kWG) void k_spmv_cg(DevCsr A, const double *__restr
   if (S->done) return;
   double h[3] = {0, 0, 0}, l[3] = {0, 0, 0};
   for_rows_Ax<WM>(A, p, [&](int i, double y) {
-    Ap[i] = y;
+    __builtin_nontemporal_store(y, &Ap[i]);
     const double pi = p[i];
     dd_fma(h[0], l[0], pi, y);
     dd_fma(h[1], l[1], pi, pi);
@@ -183,16 +198,16 @@ __global__ __launch_bounds__(kWG) void k_xr(int n, double *__restrict__ x,
   QUAD_LOOP(n) {
     const int i = q_ << 2;
     double xv[4], pv[4], rv[4], av[4];
-    ld4(x, i, xv);
+    ld4_nt(x, i, xv);  // x and Ap: read by no other kernel of the iteration
     ld4(p, i, pv);
     ld4(r, i, rv);
-    ld4(Ap, i, av);
+    ld4_nt(Ap, i, av);
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       xv[e] = fma(alpha, pv[e], xv[e]);
       rv[e] = fma(-alpha, av[e], rv[e]);
     }
-    st4(x, i, xv);
+    st4_nt(x, i, xv);
     st4(r, i, rv);
 #pragma unroll
     for (int e = 0; e < 4; ++e) dd_fma(h[0], l[0], rv[e], rv[e]);
@@ -675,11 +690,12 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
     YT uva[SW];
     if (FB) {  // this block's U entries, in flight during the forward levels
       const int t = min(lane, be - bs - 1), uo = U.off[kb] + t;
-      const unsigned pk = U.inb[64 * kb + t];
+      const unsigned pk = __builtin_nontemporal_load(&U.inb[64 * kb + t]);
 #pragma unroll
       for (int k = 0; k < SW; ++k) {
         uad[k] = (int)((pk >> (8 * k)) & 0xFFu);
-        uva[k] = (YT)((const VT *)U.v)[uo + 64 * k];  // absent entries hold 0
+        // absent entries hold 0
+        uva[k] = (YT)__builtin_nontemporal_load(&((const VT *)U.v)[uo + 64 * k]);
       }
     }
     // the earlier colors' terms, ascending; absent ones are fma(-0, 0, acc)

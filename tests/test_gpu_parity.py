@@ -62,6 +62,20 @@ def test_spmv_empty_rows():
     assert np.array_equal(y, [2.0, 0.0, 7.0])
 
 
+def test_spmv_empty_slices():
+    """Whole 64-row slices without entries, the last one included: their
+    rows give 0 and no entry of the matrix is read for them."""
+    n = 130
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:65] = np.arange(1, 65)
+    rp[65:] = 64
+    A = I.CRS(n, rp, np.arange(64, dtype=np.int32), np.arange(1.0, 65.0))
+    x = I.gen_normal(n, 4)
+    y = I.DeviceCSR(A).spmv(x)
+    assert np.array_equal(y, O.spmv(P.orc(A), x))
+    assert np.array_equal(y[64:], np.zeros(n - 64))
+
+
 @pytest.mark.parametrize("name", ["saad", "var3d_9"])
 def test_jacobi_bitwise(name):
     A = _cases()[name]
