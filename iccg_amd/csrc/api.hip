@@ -428,6 +428,15 @@ static void build_sliced_factor(iccg_precond_s &M, const std::vector<int> &lrp,
     for (int kb = M.sc.h_color_blk_v[M.sc.nc - 1]; kb < M.sc.h_color_blk_v[M.sc.nc]; ++kb)
       if (uoff[(size_t)kb + 1] - uoff[(size_t)kb] != 64 * S) F.fb = false;
   }
+  {
+    // load policy of the sweeps: the factor's slots (value + column, plus
+    // the packed in-block rows) and the vectors an iteration reads again
+    // (p, r; y, z and the diagonal in the sweep precision)
+    const double vs = M.mixed ? 4.0 : 8.0;
+    const double fac = (double)(lcol.size() + ucol.size()) * (vs + 4.0) + (S > 0 ? 8.0 * n : 0.0);
+    const double vec = (double)n * (16.0 + 3.0 * vs);
+    F.nt = fac + vec > kInfinityCacheBytes;
+  }
   for (int i = 0; i < n; ++i) l2ls[(size_t)lrp[i + 1] - 1] = -1 - i;
   for (long k = 0; k < nnz; ++k) l2us[k] = upos[(size_t)l2u[k]];
   auto up = [&](Sell &S, std::vector<int> &off, std::vector<int> &col, int w) {
@@ -948,6 +957,7 @@ int iccg_precond_set_schedule(iccg_precond_t M, iccg_sched_t sched) {
       F.sell = false;
       F.sec = 0;
       F.fb = false;
+      F.nt = false;
       build_sliced_factor(*M, lrp, lci, urp, uci, l2u);
     }
     M->plan_version++;

@@ -431,7 +431,7 @@ __device__ __forceinline__ void abmc_group_sec(int kb, int bs, int be, int nl, c
     const int *off = FWD ? D0.l_off : D0.u_off;
     const int *col = FWD ? D0.l_col : D0.u_col;
     const FT *fv = (const FT *)(FWD ? D0.l_v : D0.u_v);
-    const unsigned pk = (FWD ? D0.l_inb : D0.u_inb)[64 * kb + lane];
+    const unsigned pk = ldm<true>(&(FWD ? D0.l_inb : D0.u_inb)[64 * kb + lane]);
     const int o0 = off[kb];
     const int w = (off[kb + 1] - o0) >> 6;
     const int base = o0 + lane;
@@ -439,22 +439,22 @@ __device__ __forceinline__ void abmc_group_sec(int kb, int bs, int be, int nl, c
     for (int k = 0; k < SW; ++k) {
       if (k < sec) {
         ad[k] = (int)((pk >> (8 * k)) & 0xFFu);  // 64 (the zero slot) when absent
-        va[k] = fv[base + 64 * k];               // absent entries hold 0
+        va[k] = ldm<true>(&fv[base + 64 * k]);   // absent entries hold 0
       }
       if (sec + k < w) {
-        co[k] = col[base + 64 * (sec + k)];
-        vo[k] = fv[base + 64 * (sec + k)];
+        co[k] = ldm<true>(&col[base + 64 * (sec + k)]);
+        vo[k] = ldm<true>(&fv[base + 64 * (sec + k)]);
       }
     }
     d = ((const FT *)D0.d)[i];
     if (FB) {  // U's in-block entries of the row (its only ones)
-      const unsigned pu = D0.u_inb[64 * kb + lane];
+      const unsigned pu = ldm<true>(&D0.u_inb[64 * kb + lane]);
       const int ub = D0.u_off[kb] + lane;
 #pragma unroll
       for (int k = 0; k < SW; ++k)
         if (k < sec) {
           uad[k] = (int)((pu >> (8 * k)) & 0xFFu);
-          uva[k] = ((const FT *)D0.u_v)[ub + 64 * k];
+          uva[k] = ldm<true>(&((const FT *)D0.u_v)[ub + 64 * k]);
         }
     }
 #pragma unroll

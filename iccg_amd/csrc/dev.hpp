@@ -20,6 +20,8 @@ namespace iccg {
 
 constexpr int kWG = 256;          // threads per workgroup of streaming kernels
 constexpr int kMaxRedBlocks = 2048;  // workgroups of reducing kernels (8 per CU)
+// MI355X's Infinity Cache (the memory-side cache in front of HBM): 256 MiB
+constexpr double kInfinityCacheBytes = 256.0 * 1024 * 1024;
 
 // Device-resident solver scalars.  Written only by the one-workgroup finish
 // kernels that fold the reducing kernels' partials; read by the others at
@@ -129,6 +131,10 @@ struct Factor {
   // sectioned, and every block of the last color has no outside entries in
   // U (its backward rows have in-block terms only)
   bool fb = false;
+  // the sweeps read the sliced entries with nontemporal loads (ldm): set
+  // when they and the vectors an iteration reads again exceed the Infinity
+  // Cache, so that the vectors stay in it
+  bool nt = false;
 };
 
 // ---- multi-system engine (batch.hip) --------------------------------------

@@ -216,6 +216,18 @@ __device__ __forceinline__ bool div_steps_exact(T a, T d) {
   return div_range_ok(d) && (div_range_ok(a) || (a == (T)0 && !signbit(a)));
 }
 
+// A load of matrix data, which a kernel reads once.  NT: nontemporal, so
+// that it does not displace from the caches (the Infinity Cache in front of
+// HBM, above all) the vector lines that are gathered or read again; chosen
+// when the matrix and those vectors do not fit in that cache together.
+template <bool NT, class T>
+__device__ __forceinline__ T ldm(const T *p) {
+  if constexpr (NT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+
 // 4-element vector loads/stores (i multiple of 4; HBM buffers 256-B aligned)
 __device__ __forceinline__ void ld4(const double *p, int i, double (&o)[4]) {
   const double2 u = *reinterpret_cast<const double2 *>(p + i);
@@ -273,8 +285,8 @@ __device__ __forceinline__ void sell_load(const int *__restrict__ off, const int
     cc[k] = -1;
     vv[k] = (VT)0;
     if (k < w) {
-      cc[k] = col[base + 64 * k];
-      vv[k] = v[base + 64 * k];
+      cc[k] = ldm<true>(&col[base + 64 * k]);
+      vv[k] = ldm<true>(&v[base + 64 * k]);
     }
   }
 }
@@ -356,9 +368,9 @@ __device__ __forceinline__ double spmv_slice16(const int *__restrict__ off,
   unsigned cr[WM];
   double vv[WM];
 #pragma unroll
-  for (int k = 0; k < WM; ++k) cr[k] = __builtin_nontemporal_load(&col16[k < w ? base + 64 * k : base]);
+  for (int k = 0; k < WM; ++k) cr[k] = ldm<true>(&col16[k < w ? base + 64 * k : base]);
 #pragma unroll
-  for (int k = 0; k < WM; ++k) vv[k] = __builtin_nontemporal_load(&v[k < w ? base + 64 * k : base]);
+  for (int k = 0; k < WM; ++k) vv[k] = ldm<true>(&v[k < w ? base + 64 * k : base]);
   wave_sync();  // the previous slice's table reads are issued
   tb[lane & 15] = te;
   wave_sync();
@@ -388,7 +400,7 @@ __device__ __forceinline__ double spmv_slice16(const int *__restrict__ off,
 
 // acc op= sum_k v_k * x[c_k] over the entries of the lane's row in slice s,
 // ascending k; NEG selects acc - v*x (triangular solves) over acc + v*x.
-template <class VT, class AT, class XT, int WM, bool NEG>
+template <class VT, class AT, class XT, int WM, bool NEG, bool NT = false>
 __device__ __forceinline__ AT sell_chain(const int *__restrict__ off, const int *__restrict__ col,
                                          const VT *__restrict__ v, const XT *__restrict__ x,
                                          int s, int lane, AT acc) {
@@ -403,8 +415,8 @@ __device__ __forceinline__ AT sell_chain(const int *__restrict__ off, const int 
       cc[k] = -1;
       vv[k] = (VT)0;
       if (k < w) {
-        cc[k] = col[base + 64 * k];
-        vv[k] = v[base + 64 * k];
+        cc[k] = ldm<NT>(&col[base + 64 * k]);
+        vv[k] = ldm<NT>(&v[base + 64 * k]);
       }
     }
     AT xv[WM > 0 ? WM : 1];
@@ -415,9 +427,9 @@ __device__ __forceinline__ AT sell_chain(const int *__restrict__ off, const int 
       if (cc[k] >= 0) acc = NEG ? fma_t(-(AT)vv[k], xv[k], acc) : fma_t((AT)vv[k], xv[k], acc);
   } else {
     for (int k = 0; k < w; ++k) {
-      const int c = col[base + 64 * k];
+      const int c = ldm<NT>(&col[base + 64 * k]);
       if (c >= 0) {
-        const AT a = (AT)v[base + 64 * k];
+        const AT a = (AT)ldm<NT>(&v[base + 64 * k]);
         acc = NEG ? fma_t(-a, (AT)x[c], acc) : fma_t(a, (AT)x[c], acc);
       }
     }
@@ -438,7 +450,7 @@ __device__ __forceinline__ AT sell_chain(const int *__restrict__ off, const int 
 // The per-row operation order equals the sequential reference, bit for bit.
 // Both store the block's outputs once, coalesced, after the level loop.
 // ---------------------------------------------------------------------------
-template <class VT, class YT, int WM>
+template <class VT, class YT, int WM, bool NT = false>
 __device__ __forceinline__ void abmc_fwd_block(int bs, int be, int nl,
                                                const uint16_t *__restrict__ lev,
                                                const int *__restrict__ off,
@@ -463,11 +475,11 @@ __device__ __forceinline__ void abmc_fwd_block(int bs, int be, int nl,
       cc[k] = -1;
       vv[k] = (VT)0;
       if (k < wd) {
-        cc[k] = col[base + 64 * k];
-        vv[k] = lv[base + 64 * k];
+        cc[k] = ldm<NT>(&col[base + 64 * k]);
+        vv[k] = ldm<NT>(&lv[base + 64 * k]);
       }
     }
-    mylev = lev[i];
+    mylev = ldm<NT>(&lev[i]);
     d = (YT)diag[i];
     acc = (YT)r[i];
     YT xo[WM];
@@ -492,7 +504,7 @@ __device__ __forceinline__ void abmc_fwd_block(int bs, int be, int nl,
 }
 
 // returns z_i (0 on lanes past the block); with r, also r_i in ri
-template <class VT, class YT, int WM>
+template <class VT, class YT, int WM, bool NT = false>
 __device__ __forceinline__ YT abmc_bwd_block(int bs, int be, int nl,
                                              const uint16_t *__restrict__ lev,
                                              const int *__restrict__ off,
@@ -520,13 +532,13 @@ __device__ __forceinline__ YT abmc_bwd_block(int bs, int be, int nl,
       cc[k] = -1;
       vv[k] = (VT)0;
       if (k < wd) {
-        cc[k] = col[base + 64 * k];
-        vv[k] = uv[base + 64 * k];
+        cc[k] = ldm<NT>(&col[base + 64 * k]);
+        vv[k] = ldm<NT>(&uv[base + 64 * k]);
       }
     }
 #pragma unroll
     for (int k = 0; k < WM; ++k) xo[k] = cc[k] >= be ? z[cc[k]] : (YT)0;
-    mylev = lev[i];
+    mylev = ldm<NT>(&lev[i]);
     d = (YT)diag[i];
     acc = y[i];
     if (r) ri = r[i];
@@ -628,7 +640,7 @@ struct SecRaw {
 
 // LOAD_NOW: the caller uses the loads at once, so absent outside entries
 // are skipped by (uniform) branches instead of loaded from stand-ins.
-template <class VT, class YT, int S, bool FWD, bool LOAD_NOW = false>
+template <class VT, class YT, int S, bool FWD, bool NT, bool LOAD_NOW = false>
 __device__ __forceinline__ void sec_fetch(int kb, int o0, int w, int bs, int be, int lane,
                                           const unsigned *__restrict__ inb,
                                           const int *__restrict__ col, const VT *__restrict__ v,
@@ -637,28 +649,25 @@ __device__ __forceinline__ void sec_fetch(int kb, int o0, int w, int bs, int be,
                                           const YT *__restrict__ yin, SecRaw<VT, YT, S> &e) {
   const int t = min(lane, be - bs - 1);  // a row of the block
   const int base = o0 + t;
-  // the factor's entries are read once per sweep: nontemporal, so that they
-  // do not push the gathered vector lines out of L2
-  e.inb = __builtin_nontemporal_load(&inb[64 * kb + t]);
+  e.inb = ldm<NT>(&inb[64 * kb + t]);
 #pragma unroll
-  for (int k = 0; k < S; ++k)
-    e.vin[k] = __builtin_nontemporal_load(&v[base + 64 * k]);  // absent entries hold 0
+  for (int k = 0; k < S; ++k) e.vin[k] = ldm<NT>(&v[base + 64 * k]);  // absent entries hold 0
 #pragma unroll
   for (int k = 0; k < S; ++k) {
     if (LOAD_NOW) {
       e.cout_[k] = -1;
       e.vout[k] = (VT)0;
       if (S + k < w) {
-        e.cout_[k] = __builtin_nontemporal_load(&col[base + 64 * (S + k)]);
-        e.vout[k] = __builtin_nontemporal_load(&v[base + 64 * (S + k)]);
+        e.cout_[k] = ldm<NT>(&col[base + 64 * (S + k)]);
+        e.vout[k] = ldm<NT>(&v[base + 64 * (S + k)]);
       }
     } else {
       // absent: stand-ins in lines this fetch reads anyway (the packed word,
       // the first in-block value), so they cost no extra traffic
       const bool here = S + k < w;
       const int *cp = here ? col + base + 64 * (S + k) : (const int *)inb + 64 * kb + t;
-      e.cout_[k] = __builtin_nontemporal_load(cp);
-      e.vout[k] = __builtin_nontemporal_load(&v[here ? base + 64 * (S + k) : base]);
+      e.cout_[k] = ldm<NT>(cp);
+      e.vout[k] = ldm<NT>(&v[here ? base + 64 * (S + k) : base]);
     }
   }
   e.d = diag[bs + t];

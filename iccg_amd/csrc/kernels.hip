@@ -72,7 +72,7 @@ __device__ __forceinline__ void for_rows_Ax(const DevCsr &A, const double *__res
     const int lane = threadIdx.x & 63;
     const WaveRange wr = xcd_waves(0, A.sell.nslice);
     for (int s = wr.first; s < wr.end; s += wr.step) {
-      const double y = sell_chain<double, double, double, WM, false>(A.sell.off, A.sell.col,
+      const double y = sell_chain<double, double, double, WM, false, true>(A.sell.off, A.sell.col,
                                                                     A.sell.v64, x, s, lane, 0.0);
       const int i = (s << 6) + lane;
       if (i < A.n) f(i, y);
@@ -530,7 +530,7 @@ __global__ void k_abmc(const int *__restrict__ bstart, const int *__restrict__ b
 // an earlier (forward) / later (backward) color, finished by a previous
 // launch.
 // ---------------------------------------------------------------------------
-template <class VT, class YT, int WM>
+template <class VT, class YT, int WM, bool NT>
 __global__ __launch_bounds__(kWG) void k_mc_fwd(int c0, int c1, Sell L, const VT *__restrict__ lv,
                                                 const VT *__restrict__ diag,
                                                 const double *__restrict__ r,
@@ -542,12 +542,12 @@ __global__ __launch_bounds__(kWG) void k_mc_fwd(int c0, int c1, Sell L, const VT
     const int i = (s << 6) + lane;
     if (i < c0 || i >= c1) continue;
     YT acc = (YT)r[i];
-    acc = sell_chain<VT, YT, YT, WM, true>(L.off, L.col, lv, y, s, lane, acc);
+    acc = sell_chain<VT, YT, YT, WM, true, NT>(L.off, L.col, lv, y, s, lane, acc);
     y[i] = acc / (YT)diag[i];
   }
 }
 
-template <class VT, class YT, int WM, bool RZ>
+template <class VT, class YT, int WM, bool RZ, bool NT>
 __global__ __launch_bounds__(kWG) void k_mc_bwd(int c0, int c1, Sell U, const VT *__restrict__ uv,
                                                 const VT *__restrict__ diag,
                                                 const YT *__restrict__ y, YT *__restrict__ z,
@@ -561,7 +561,7 @@ __global__ __launch_bounds__(kWG) void k_mc_bwd(int c0, int c1, Sell U, const VT
     const int i = (s << 6) + lane;
     if (i < c0 || i >= c1) continue;
     YT acc = y[i];
-    acc = sell_chain<VT, YT, YT, WM, true>(U.off, U.col, uv, z, s, lane, acc);
+    acc = sell_chain<VT, YT, YT, WM, true, NT>(U.off, U.col, uv, z, s, lane, acc);
     const YT zi = acc / (YT)diag[i];
     z[i] = zi;
     if (RZ) dd_fma(h[0], l[0], r[i], (double)zi);
@@ -573,7 +573,7 @@ __global__ __launch_bounds__(kWG) void k_mc_bwd(int c0, int c1, Sell U, const VT
 // ABMC sweeps for blocks of at most 64 rows: one wavefront per block
 // (abmc_fwd_block / abmc_bwd_block), blocks of a color spread over the grid.
 // ---------------------------------------------------------------------------
-template <class VT, class YT, int WM>
+template <class VT, class YT, int WM, bool NT>
 __global__ __launch_bounds__(kWG) void k_abmc_fwd_wave(int kb0, int kb1,
                                                        const int *__restrict__ bstart,
                                                        const uint16_t *__restrict__ lev,
@@ -588,11 +588,11 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_wave(int kb0, int kb1,
   YT *my = ys + (w << 6);
   const WaveRange wr = xcd_waves(kb0, kb1);
   for (int kb = wr.first; kb < wr.end; kb += wr.step)
-    abmc_fwd_block<VT, YT, WM>(bstart[kb], bstart[kb + 1], nlev[kb], lev, L.off, L.col, lv, diag,
+    abmc_fwd_block<VT, YT, WM, NT>(bstart[kb], bstart[kb + 1], nlev[kb], lev, L.off, L.col, lv, diag,
                                r, y, my, lane);
 }
 
-template <class VT, class YT, int WM, bool RZ>
+template <class VT, class YT, int WM, bool RZ, bool NT>
 __global__ __launch_bounds__(kWG) void k_abmc_bwd_wave(
     int kb0, int kb1, const int *__restrict__ bstart, const uint16_t *__restrict__ lev,
     const int *__restrict__ nlev, Sell U, const VT *__restrict__ uv, const VT *__restrict__ diag,
@@ -607,7 +607,7 @@ __global__ __launch_bounds__(kWG) void k_abmc_bwd_wave(
   for (int kb = wr.first; kb < wr.end; kb += wr.step) {
     const int bs = bstart[kb], be = bstart[kb + 1];
     double ri;
-    const YT zi = abmc_bwd_block<VT, YT, WM>(bs, be, nlev[kb], lev, U.off, U.col, uv, diag, y, z,
+    const YT zi = abmc_bwd_block<VT, YT, WM, NT>(bs, be, nlev[kb], lev, U.off, U.col, uv, diag, y, z,
                                              RZ ? r : nullptr, ri, my, lane);
     if (RZ && bs + lane < be) dd_fma(h[0], l[0], ri, (double)zi);
   }
@@ -635,7 +635,7 @@ struct SecU {  // U's sectioned copy and the backward level counts (FB)
   const void *v;
 };
 
-template <class VT, class YT, int SW, bool FB>
+template <class VT, class YT, int SW, bool FB, bool NT>
 __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
                                                       const int *__restrict__ bstart,
                                                       const int *__restrict__ nlev,
@@ -663,7 +663,7 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
   int bs = bstart[first], be = bstart[first + 1], nl = nlev[first];
   int o0 = off[first], w = (off[first + 1] - o0) >> 6;
   SecRaw<VT, YT, SW> cur;
-  if (PIPE) sec_fetch<VT, YT, SW, true>(first, o0, w, bs, be, lane, inb, col, lv, diag, r, y, cur);
+  if (PIPE) sec_fetch<VT, YT, SW, true, NT>(first, o0, w, bs, be, lane, inb, col, lv, diag, r, y, cur);
   for (int kb = first; kb < end; kb += step) {
     if (!PIPE) {
       bs = bstart[kb];
@@ -671,7 +671,7 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
       nl = nlev[kb];
       o0 = off[kb];
       w = (off[kb + 1] - o0) >> 6;
-      sec_fetch<VT, YT, SW, true, true>(kb, o0, w, bs, be, lane, inb, col, lv, diag, r, y, cur);
+      sec_fetch<VT, YT, SW, true, NT, true>(kb, o0, w, bs, be, lane, inb, col, lv, diag, r, y, cur);
     }
     const bool act = bs + lane < be;
     int ad[SW], co[SW];
@@ -685,17 +685,17 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
     const int nnl = PIPE ? nlev[kc] : 0, no0 = PIPE ? off[kc] : 0;
     const int nw = PIPE ? (off[kc + 1] - no0) >> 6 : 0;
     if (PIPE)
-      sec_fetch<VT, YT, SW, true>(kc, no0, nw, nbs, nbe, lane, inb, col, lv, diag, r, y, cur);
+      sec_fetch<VT, YT, SW, true, NT>(kc, no0, nw, nbs, nbe, lane, inb, col, lv, diag, r, y, cur);
     int uad[SW];
     YT uva[SW];
     if (FB) {  // this block's U entries, in flight during the forward levels
       const int t = min(lane, be - bs - 1), uo = U.off[kb] + t;
-      const unsigned pk = __builtin_nontemporal_load(&U.inb[64 * kb + t]);
+      const unsigned pk = ldm<NT>(&U.inb[64 * kb + t]);
 #pragma unroll
       for (int k = 0; k < SW; ++k) {
         uad[k] = (int)((pk >> (8 * k)) & 0xFFu);
         // absent entries hold 0
-        uva[k] = (YT)__builtin_nontemporal_load(&((const VT *)U.v)[uo + 64 * k]);
+        uva[k] = (YT)ldm<NT>(&((const VT *)U.v)[uo + 64 * k]);
       }
     }
     // the earlier colors' terms, ascending; absent ones are fma(-0, 0, acc)
@@ -721,7 +721,7 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
   }
 }
 
-template <class VT, class YT, int SW, bool RZ>
+template <class VT, class YT, int SW, bool RZ, bool NT>
 __global__ __launch_bounds__(kWG) void k_abmc_bwd_sec(
     int kb0, int kb1, const int *__restrict__ bstart, const int *__restrict__ nlev,
     const int *__restrict__ off, const unsigned *__restrict__ inb, const int *__restrict__ col,
@@ -742,7 +742,7 @@ __global__ __launch_bounds__(kWG) void k_abmc_bwd_sec(
   int bs = bstart[f0], be = bstart[f0 + 1], nl = nlev[f0];
   int o0 = off[f0], w = (off[f0 + 1] - o0) >> 6;
   SecRaw<VT, YT, SW> cur;
-  if (PIPE) sec_fetch<VT, YT, SW, false>(f0, o0, w, bs, be, lane, inb, col, uv, diag, rr, y, cur);
+  if (PIPE) sec_fetch<VT, YT, SW, false, NT>(f0, o0, w, bs, be, lane, inb, col, uv, diag, rr, y, cur);
   for (int kb = first; kb < end; kb += step) {
     if (!PIPE) {
       bs = bstart[kb];
@@ -750,7 +750,7 @@ __global__ __launch_bounds__(kWG) void k_abmc_bwd_sec(
       nl = nlev[kb];
       o0 = off[kb];
       w = (off[kb + 1] - o0) >> 6;
-      sec_fetch<VT, YT, SW, false, true>(kb, o0, w, bs, be, lane, inb, col, uv, diag, rr, y,
+      sec_fetch<VT, YT, SW, false, NT, true>(kb, o0, w, bs, be, lane, inb, col, uv, diag, rr, y,
                                          cur);
     }
     const bool act = bs + lane < be;
@@ -766,7 +766,7 @@ __global__ __launch_bounds__(kWG) void k_abmc_bwd_sec(
     const int nnl = PIPE ? nlev[kc] : 0, no0 = PIPE ? off[kc] : 0;
     const int nw = PIPE ? (off[kc + 1] - no0) >> 6 : 0;
     if (PIPE)
-      sec_fetch<VT, YT, SW, false>(kc, no0, nw, nbs, nbe, lane, inb, col, uv, diag, rr, y, cur);
+      sec_fetch<VT, YT, SW, false, NT>(kc, no0, nw, nbs, nbe, lane, inb, col, uv, diag, rr, y, cur);
 #pragma unroll
     for (int k = 0; k < SW; ++k) xo[k] = co[k] >= 0 ? xo[k] : (YT)0;
     if (lane == 0) my[64] = (YT)0;
@@ -1069,7 +1069,7 @@ static int abmc_grid(int nb) {
   return sizeof(VT) == 8 ? (nb + 3) / 4 : slice_grid(nb);
 }
 
-template <class VT, class YT, int WM>
+template <class VT, class YT, int WM, bool NT>
 static void fwd_sliced(const DevSched &sc, const Factor &F, const VT *lv, const VT *dg,
                        const double *r, YT *y, const Scal *S, hipStream_t st) {
   if (sc.kind == 1) {
@@ -1077,20 +1077,20 @@ static void fwd_sliced(const DevSched &sc, const Factor &F, const VT *lv, const 
       const int c0 = sc.h_color_ptr_v[c], c1 = sc.h_color_ptr_v[c + 1];
       if (c1 <= c0) continue;
       const int ns = ((c1 - 1) >> 6) - (c0 >> 6) + 1;
-      hipLaunchKernelGGL((k_mc_fwd<VT, YT, WM>), dim3(slice_grid(ns)), dim3(kWG), 0, st, c0, c1,
+      hipLaunchKernelGGL((k_mc_fwd<VT, YT, WM, NT>), dim3(slice_grid(ns)), dim3(kWG), 0, st, c0, c1,
                          F.ls, lv, dg, r, y, S);
     }
   } else {
     for (int c = 0; c < sc.nc; ++c) {
       const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
       if (b1 <= b0) continue;
-      hipLaunchKernelGGL((k_abmc_fwd_wave<VT, YT, WM>), dim3(abmc_grid<VT>(b1 - b0)), dim3(kWG), 0,
+      hipLaunchKernelGGL((k_abmc_fwd_wave<VT, YT, WM, NT>), dim3(abmc_grid<VT>(b1 - b0)), dim3(kWG), 0,
                          st, b0, b1, sc.d_bstart, sc.d_lev_f, sc.d_nlev_f, F.ls, lv, dg, r, y, S);
     }
   }
 }
 
-template <class VT, class YT, int WM>
+template <class VT, class YT, int WM, bool NT>
 static void bwd_sliced(const DevSched &sc, const Factor &F, const VT *uv, const VT *dg,
                        const YT *y, YT *z, const double *r, Scal *S, double *part, int rz,
                        hipStream_t st) {
@@ -1119,19 +1119,19 @@ static void bwd_sliced(const DevSched &sc, const Factor &F, const VT *uv, const 
     if (sc.kind == 1) {
       const int c0 = sc.h_color_ptr_v[c], c1 = sc.h_color_ptr_v[c + 1];
       if (fuse)
-        hipLaunchKernelGGL((k_mc_bwd<VT, YT, WM, true>), dim3(g), dim3(kWG), 0, st, c0, c1, F.us,
+        hipLaunchKernelGGL((k_mc_bwd<VT, YT, WM, true, NT>), dim3(g), dim3(kWG), 0, st, c0, c1, F.us,
                            uv, dg, y, z, r, S, part, gidx0, total, rz == 2);
       else
-        hipLaunchKernelGGL((k_mc_bwd<VT, YT, WM, false>), dim3(g), dim3(kWG), 0, st, c0, c1,
+        hipLaunchKernelGGL((k_mc_bwd<VT, YT, WM, false, NT>), dim3(g), dim3(kWG), 0, st, c0, c1,
                            F.us, uv, dg, y, z, r, S, part, gidx0, total, 0);
     } else {
       const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
       if (fuse)
-        hipLaunchKernelGGL((k_abmc_bwd_wave<VT, YT, WM, true>), dim3(g), dim3(kWG), 0, st, b0,
+        hipLaunchKernelGGL((k_abmc_bwd_wave<VT, YT, WM, true, NT>), dim3(g), dim3(kWG), 0, st, b0,
                            b1, sc.d_bstart, sc.d_lev_b, sc.d_nlev_b, F.us, uv, dg, y, z, r, S,
                            part, gidx0, total, rz == 2);
       else
-        hipLaunchKernelGGL((k_abmc_bwd_wave<VT, YT, WM, false>), dim3(g), dim3(kWG), 0, st, b0,
+        hipLaunchKernelGGL((k_abmc_bwd_wave<VT, YT, WM, false, NT>), dim3(g), dim3(kWG), 0, st, b0,
                            b1, sc.d_bstart, sc.d_lev_b, sc.d_nlev_b, F.us, uv, dg, y, z, r, S,
                            part, gidx0, total, 0);
     }
@@ -1142,7 +1142,7 @@ static void bwd_sliced(const DevSched &sc, const Factor &F, const VT *uv, const 
 
 // forward colors [0, c_end); with uv, the last color through the fused
 // forward + backward launch as well
-template <class VT, class YT, int SW>
+template <class VT, class YT, int SW, bool NT>
 static void fwd_sec(const DevSched &sc, const Factor &F, const VT *lv, const VT *dg,
                     const double *r, YT *y, const Scal *S, hipStream_t st, const VT *uv = nullptr,
                     YT *z = nullptr) {
@@ -1151,18 +1151,18 @@ static void fwd_sec(const DevSched &sc, const Factor &F, const VT *lv, const VT 
     const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
     if (b1 <= b0) continue;
     if (uv && c == sc.nc - 1)
-      hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, true>), dim3(slice_grid(b1 - b0)),
+      hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, true, NT>), dim3(slice_grid(b1 - b0)),
                          dim3(kWG), 0, st, b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.inb,
                          F.ls.col, lv, dg, r, y,
                          SecU{sc.d_nlev_b, F.us.off, F.us.inb, (const void *)uv}, z, S);
     else
-      hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, false>), dim3(slice_grid(b1 - b0)),
+      hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, false, NT>), dim3(slice_grid(b1 - b0)),
                          dim3(kWG), 0, st, b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.inb,
                          F.ls.col, lv, dg, r, y, none, nullptr, S);
   }
 }
 
-template <class VT, class YT, int SW>
+template <class VT, class YT, int SW, bool NT>
 static void bwd_sec(const DevSched &sc, const Factor &F, const VT *uv, const VT *dg, const YT *y,
                     YT *z, const double *r, Scal *S, double *part, int rz, hipStream_t st,
                     bool skip_last = false) {
@@ -1182,12 +1182,12 @@ static void bwd_sec(const DevSched &sc, const Factor &F, const VT *uv, const VT 
     if (g == 0) continue;
     const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
     if (fuse)
-      hipLaunchKernelGGL((k_abmc_bwd_sec<VT, YT, SW, true>), dim3(g), dim3(kWG), 0, st, b0, b1,
+      hipLaunchKernelGGL((k_abmc_bwd_sec<VT, YT, SW, true, NT>), dim3(g), dim3(kWG), 0, st, b0, b1,
                          sc.d_bstart, sc.d_nlev_b, F.us.off, F.us.inb, F.us.col, uv, dg, y, z, r, S,
                          part,
                          gidx0);
     else
-      hipLaunchKernelGGL((k_abmc_bwd_sec<VT, YT, SW, false>), dim3(g), dim3(kWG), 0, st, b0, b1,
+      hipLaunchKernelGGL((k_abmc_bwd_sec<VT, YT, SW, false, NT>), dim3(g), dim3(kWG), 0, st, b0, b1,
                          sc.d_bstart, sc.d_nlev_b, F.us.off, F.us.inb, F.us.col, uv, dg, y, z, r, S,
                          part,
                          gidx0);
@@ -1196,21 +1196,25 @@ static void bwd_sec(const DevSched &sc, const Factor &F, const VT *uv, const VT 
   if (fuse) hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(kFinWG), 0, st, part, total, S, rz == 2);
 }
 
-#define DISPATCH_SEC(sw, FN, VT, YT, ...)        \
-  do {                                           \
-    switch (sw) {                                \
-      case 2: FN<VT, YT, 2>(__VA_ARGS__); break; \
-      default: FN<VT, YT, 3>(__VA_ARGS__); break; \
-    }                                            \
+// FN<VT, YT, width, nt> over the section / slot widths and the factor's
+// load policy (Factor::nt)
+#define DISPATCH_SEC(sw, nt, FN, VT, YT, ...)                    \
+  do {                                                           \
+    if ((sw) == 2 && (nt)) FN<VT, YT, 2, true>(__VA_ARGS__);     \
+    else if ((sw) == 2) FN<VT, YT, 2, false>(__VA_ARGS__);       \
+    else if (nt) FN<VT, YT, 3, true>(__VA_ARGS__);               \
+    else FN<VT, YT, 3, false>(__VA_ARGS__);                      \
   } while (0)
 
-#define DISPATCH_SWEEP_WM(wm, FN, VT, YT, ...)   \
-  do {                                           \
-    switch (wm) {                                \
-      case 4: FN<VT, YT, 4>(__VA_ARGS__); break; \
-      case 8: FN<VT, YT, 8>(__VA_ARGS__); break; \
-      default: FN<VT, YT, 16>(__VA_ARGS__); break; \
-    }                                            \
+#define DISPATCH_SWEEP_WM(wm, nt, FN, VT, YT, ...)               \
+  do {                                                           \
+    const int _w = (wm) == 4 ? 4 : (wm) == 8 ? 8 : 16;           \
+    if (_w == 4 && (nt)) FN<VT, YT, 4, true>(__VA_ARGS__);       \
+    else if (_w == 4) FN<VT, YT, 4, false>(__VA_ARGS__);         \
+    else if (_w == 8 && (nt)) FN<VT, YT, 8, true>(__VA_ARGS__);  \
+    else if (_w == 8) FN<VT, YT, 8, false>(__VA_ARGS__);         \
+    else if (nt) FN<VT, YT, 16, true>(__VA_ARGS__);              \
+    else FN<VT, YT, 16, false>(__VA_ARGS__);                     \
   } while (0)
 
 bool sweeps_fused(const DevSched &sc, const Factor &F) {
@@ -1222,20 +1226,20 @@ void launch_fwd(const DevSched &sc, const Factor &F, bool mixed, const double *r
   if (sliced_plan(sc, F) && sc.kind == 2 && F.sec > 0) {
     const bool fb = z_fused && sweeps_fused(sc, F);
     if (mixed)
-      DISPATCH_SEC(F.sec, fwd_sec, float, float, sc, F, F.ls.v32, F.d32, r, (float *)y, S, st,
+      DISPATCH_SEC(F.sec, F.nt, fwd_sec, float, float, sc, F, F.ls.v32, F.d32, r, (float *)y, S, st,
                    fb ? F.us.v32 : nullptr, (float *)z_fused);
     else
-      DISPATCH_SEC(F.sec, fwd_sec, double, double, sc, F, F.ls.v64, F.d64, r, (double *)y, S, st,
+      DISPATCH_SEC(F.sec, F.nt, fwd_sec, double, double, sc, F, F.ls.v64, F.d64, r, (double *)y, S, st,
                    fb ? F.us.v64 : nullptr, (double *)z_fused);
     return;
   }
   if (sliced_plan(sc, F)) {
     const int wm = sweep_wm(F);
     if (mixed)
-      DISPATCH_SWEEP_WM(wm, fwd_sliced, float, float, sc, F, F.ls.v32, F.d32, r, (float *)y, S,
+      DISPATCH_SWEEP_WM(wm, F.nt, fwd_sliced, float, float, sc, F, F.ls.v32, F.d32, r, (float *)y, S,
                         st);
     else
-      DISPATCH_SWEEP_WM(wm, fwd_sliced, double, double, sc, F, F.ls.v64, F.d64, r, (double *)y,
+      DISPATCH_SWEEP_WM(wm, F.nt, fwd_sliced, double, double, sc, F, F.ls.v64, F.d64, r, (double *)y,
                         S, st);
     return;
   }
@@ -1254,20 +1258,20 @@ void launch_bwd(const DevSched &sc, const Factor &F, bool mixed, const void *y, 
   if (sliced_plan(sc, F) && sc.kind == 2 && F.sec > 0) {
     const bool skip = last_done && sweeps_fused(sc, F);
     if (mixed)
-      DISPATCH_SEC(F.sec, bwd_sec, float, float, sc, F, F.us.v32, F.d32, (const float *)y,
+      DISPATCH_SEC(F.sec, F.nt, bwd_sec, float, float, sc, F, F.us.v32, F.d32, (const float *)y,
                    (float *)z, r, Sw, part, rz, st, skip);
     else
-      DISPATCH_SEC(F.sec, bwd_sec, double, double, sc, F, F.us.v64, F.d64, (const double *)y,
+      DISPATCH_SEC(F.sec, F.nt, bwd_sec, double, double, sc, F, F.us.v64, F.d64, (const double *)y,
                    (double *)z, r, Sw, part, rz, st, skip);
     return;
   }
   if (sliced_plan(sc, F)) {
     const int wm = sweep_wm(F);
     if (mixed)
-      DISPATCH_SWEEP_WM(wm, bwd_sliced, float, float, sc, F, F.us.v32, F.d32, (const float *)y,
+      DISPATCH_SWEEP_WM(wm, F.nt, bwd_sliced, float, float, sc, F, F.us.v32, F.d32, (const float *)y,
                         (float *)z, r, Sw, part, rz, st);
     else
-      DISPATCH_SWEEP_WM(wm, bwd_sliced, double, double, sc, F, F.us.v64, F.d64,
+      DISPATCH_SWEEP_WM(wm, F.nt, bwd_sliced, double, double, sc, F, F.us.v64, F.d64,
                         (const double *)y, (double *)z, r, Sw, part, rz, st);
     return;
   }
