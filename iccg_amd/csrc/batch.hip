@@ -96,7 +96,7 @@ __device__ __forceinline__ void ax_group(const GroupDesc &g, const SysDesc *__re
         if (mode == 3) {
           const double y = sell_apply<double, double, VT, WM, false>(cc, vv, D.p, 0.0);
           if (i < D0.n) {
-            D.Ap[i] = y;
+            __builtin_nontemporal_store(y, &D.Ap[i]);  // read once, by kb_xr
             const double pi = D.p[i];
             dd_fma(h[q][0], l[q][0], pi, y);
             dd_fma(h[q][1], l[q][1], pi, pi);
@@ -166,9 +166,10 @@ __global__ __launch_bounds__(kWG) void kb_xr(const GroupDesc *__restrict__ group
     const int end = min(D.n, c.s1 << 6);
     double h[1] = {0}, l[1] = {0};
     for (int i = (c.s0 << 6) + threadIdx.x; i < end; i += kWG) {
-      const double xi = fma(alpha, D.p[i], D.x[i]);
-      const double ri = fma(-alpha, D.Ap[i], D.r[i]);
-      D.x[i] = xi;
+      // x and Ap: read by no other kernel of the iteration (nontemporal)
+      const double xi = fma(alpha, D.p[i], ldm<true>(&D.x[i]));
+      const double ri = fma(-alpha, ldm<true>(&D.Ap[i]), D.r[i]);
+      __builtin_nontemporal_store(xi, &D.x[i]);
       D.r[i] = ri;
       dd_fma(h[0], l[0], ri, ri);
     }
