@@ -601,7 +601,7 @@ static void enqueue_iteration(iccg_solver_s &S, int it) {
       if (fused)
         launch_xr(PK_IC, n, S.x, S.p, S.r, S.Ap, nullptr, nullptr, S.S, S.part, S.st);
       else
-        launch_xr_ic(n, S.x, S.p, S.r, S.Ap, S.S, S.part, S.st);
+        launch_r_ic(n, S.r, S.Ap, S.S, S.part, S.st);
     }
     // fused: the last color's backward rows run in the forward timer's
     // launch group
@@ -615,8 +615,12 @@ static void enqueue_iteration(iccg_solver_s &S, int it) {
       launch_bwd(M.sc, M.F, M.mixed, S.y, S.z, S.S, S.st, S.r, S.S, S.part, 1, fb);
     }
     Timer t(S, 3, it);
-    if (!fused) launch_rz_stop(n, S.r, S.z, M.mixed, S.S, S.part, S.st);
-    launch_update_p(n, S.z, M.mixed, S.p, S.S, 0, S.st);
+    if (fused) {
+      launch_update_p(n, S.z, M.mixed, S.p, S.S, 0, S.st);
+    } else {
+      launch_rz_stop(n, S.r, S.z, M.mixed, S.S, S.part, S.st);
+      launch_update_xp(n, S.x, S.z, M.mixed, S.p, S.S, S.st);
+    }
   } else {
     const int pk = M.kind == ICCG_PRECOND_JACOBI ? PK_JAC : PK_IDENT;
     Timer t(S, 3, it);

@@ -209,11 +209,14 @@ void launch_xr(int pk, int n, double *x, const double *p, double *r, const doubl
                const double *inv_diag, double *z, Scal *S, double *part, hipStream_t st);
 void launch_precond_diag_init(int pk, int n, const double *r, const double *inv_diag, double *z,
                               double *p, Scal *S, double *part, hipStream_t st);
-// IC(0) iterations with the stop test deferred: launch_xr_ic leaves the
-// x/r update's |r|^2 partials in part's second region, and launch_rz_stop
-// folds them after the preconditioner, together with r.z
-void launch_xr_ic(int n, double *x, const double *p, double *r, const double *Ap, Scal *S,
-                  double *part, hipStream_t st);
+// IC(0) iterations with the stop test deferred: launch_r_ic (r -= alpha Ap)
+// leaves the |r|^2 partials in part's second region, launch_rz_stop folds
+// them after the preconditioner, together with r.z, and launch_update_xp
+// applies x += alpha p with the p update (x alone in the iteration that
+// stops)
+void launch_r_ic(int n, double *r, const double *Ap, Scal *S, double *part, hipStream_t st);
+void launch_update_xp(int n, double *x, const void *z, bool z32, double *p, const Scal *S,
+                      hipStream_t st);
 void launch_rz_stop(int n, const double *r, const void *z, bool z32, Scal *S, double *part,
                     hipStream_t st);
 void launch_rz(int n, const double *r, const void *z, bool z32, Scal *S, double *part, int init,

@@ -107,10 +107,16 @@ __global__ __launch_bounds__(kWG) void k_spmv_cg(DevCsr A, const double *__restr
   block_partial<3>(h, l, part, blockIdx.x);
 }
 
-// breakdown test (cg_crs.cpp:127-133) and alpha (135)
+// breakdown test (cg_crs.cpp:127-133) and alpha (135).  A solve that
+// stopped before this iteration is marked done = 2, so that the x update of
+// k_update_xp, which runs for the iteration that stops (done = 1 from the
+// stop test), is not repeated by the iterations enqueued after it.
 __global__ __launch_bounds__(kFinWG) void k_fin_alpha(const double *__restrict__ part, int count,
                                                    Scal *S) {
-  if (S->done) return;
+  if (S->done) {
+    if (threadIdx.x == 0) S->done = 2;
+    return;
+  }
   double out[3];
   if (finish_reduce<3>(part, count, out)) {
     const double pAp = out[0];
@@ -237,6 +243,34 @@ __global__ __launch_bounds__(kWG) void k_xr(int n, double *__restrict__ x,
   block_partial<2>(h, l, part, blockIdx.x);
 }
 
+// r -= alpha Ap (cg_crs.cpp:141) with the ||r||^2 partials of the deferred
+// stop test (IC(0) iterations; x += alpha p, line 139, moves to k_update_xp,
+// which reads p anyway)
+__global__ __launch_bounds__(kWG) void k_r_ic(int n, double *__restrict__ r,
+                                              const double *__restrict__ Ap, const Scal *S,
+                                              double *part) {
+  if (S->done) return;
+  const double alpha = S->alpha;
+  double h[1] = {0}, l[1] = {0};
+  QUAD_LOOP(n) {
+    const int i = q_ << 2;
+    double rv[4], av[4];
+    ld4(r, i, rv);
+    ld4_nt(Ap, i, av);  // Ap: read by no other kernel of the iteration
+#pragma unroll
+    for (int e = 0; e < 4; ++e) rv[e] = fma(-alpha, av[e], rv[e]);
+    st4(r, i, rv);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) dd_fma(h[0], l[0], rv[e], rv[e]);
+  }
+  TAIL_LOOP(n) {
+    const double ri = fma(-alpha, Ap[i], r[i]);
+    r[i] = ri;
+    dd_fma(h[0], l[0], ri, ri);
+  }
+  block_partial<1>(h, l, part, blockIdx.x);
+}
+
 // stop test (cg_crs.cpp:143-149); Identity / Jacobi also beta (152-153)
 template <int PK>
 __global__ __launch_bounds__(kFinWG) void k_fin_xr(const double *__restrict__ part, int count,
@@ -272,8 +306,8 @@ __global__ __launch_bounds__(kFinWG) void k_fin_xr_rz(const double *__restrict__
                                                    const double *__restrict__ part_rz,
                                                    int count_rz, Scal *S) {
   if (S->done) return;
-  double rr[2], rz[1];
-  const bool t0 = finish_reduce<2>(part_xr, count_xr, rr);
+  double rr[1], rz[1];  // part_xr: k_r_ic's one-component partials
+  const bool t0 = finish_reduce<1>(part_xr, count_xr, rr);
   finish_reduce<1>(part_rz, count_rz, rz);
   if (t0) {
     const double rel = sqrt(rr[0]) / S->normb;
@@ -370,6 +404,41 @@ __global__ __launch_bounds__(kWG) void k_update_p(int n, const ZT *__restrict__ 
   TAIL_LOOP(n) {
     const double zi = (double)z[i];
     p[i] = init ? zi : fma(beta, p[i], zi);
+  }
+}
+
+// x += alpha p (cg_crs.cpp:139) and p = z + beta p (155-157) of an IC(0)
+// iteration, the x update deferred from k_r_ic: it runs for every iteration
+// whose x/r update ran (not after a breakdown, not once the solve had
+// stopped, done == 2), including the one whose stop test ends the solve
+// (done == 1), which updates no p.
+template <class ZT>
+__global__ __launch_bounds__(kWG) void k_update_xp(int n, double *__restrict__ x,
+                                                   const ZT *__restrict__ z,
+                                                   double *__restrict__ p, const Scal *S) {
+  if (S->done > 1 || S->breakdown) return;
+  const double alpha = S->alpha, beta = S->beta;
+  const bool pu = !S->done;
+  QUAD_LOOP(n) {
+    const int i = q_ << 2;
+    double xv[4], pv[4];
+    ld4_nt(x, i, xv);  // x: read by no other kernel of the iteration
+    ld4(p, i, pv);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) xv[e] = fma(alpha, pv[e], xv[e]);
+    st4_nt(x, i, xv);
+    if (pu) {
+      double zv[4];
+      ld4(z, i, zv);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) pv[e] = fma(beta, pv[e], zv[e]);
+      st4(p, i, pv);
+    }
+  }
+  TAIL_LOOP(n) {
+    const double pi = p[i];
+    x[i] = fma(alpha, pi, x[i]);
+    if (pu) p[i] = fma(beta, pi, (double)z[i]);
   }
 }
 
@@ -907,10 +976,17 @@ void launch_xr(int pk, int n, double *x, const double *p, double *r, const doubl
 // kMaxRedBlocks on, clear of the r.z partials (pairs below kMaxRedBlocks)
 static double *xr_parts(double *part) { return part + 2 * kMaxRedBlocks; }
 
-void launch_xr_ic(int n, double *x, const double *p, double *r, const double *Ap, Scal *S,
-                  double *part, hipStream_t st) {
-  hipLaunchKernelGGL(k_xr<PK_IC>, dim3(quad_grid(n)), dim3(kWG), 0, st, n, x, p, r, Ap, nullptr,
-                     nullptr, S, xr_parts(part));
+void launch_r_ic(int n, double *r, const double *Ap, Scal *S, double *part, hipStream_t st) {
+  hipLaunchKernelGGL(k_r_ic, dim3(quad_grid(n)), dim3(kWG), 0, st, n, r, Ap, S, xr_parts(part));
+}
+
+void launch_update_xp(int n, double *x, const void *z, bool z32, double *p, const Scal *S,
+                      hipStream_t st) {
+  dim3 g(quad_grid(n)), b(kWG);
+  if (z32)
+    hipLaunchKernelGGL(k_update_xp<float>, g, b, 0, st, n, x, (const float *)z, p, S);
+  else
+    hipLaunchKernelGGL(k_update_xp<double>, g, b, 0, st, n, x, (const double *)z, p, S);
 }
 
 void launch_rz_stop(int n, const double *r, const void *z, bool z32, Scal *S, double *part,
