@@ -79,37 +79,60 @@ __device__ void block_partial(double (&h)[NC], double (&l)[NC], double *part, in
   }
 }
 
-// One workgroup folds `count` partials in slot order; true on thread 0,
-// with out[] = the totals rounded once to double.
-template <int NC>
-__device__ bool finish_reduce(const double *__restrict__ part, int count, double (&out)[NC]) {
-  __shared__ double sh[NC][2][16];
+// One workgroup folds the partials of one reduction (NA components per
+// slot at pa, ca slots) or of two (and NB at pb, cb slots).  Every component
+// is folded in the same fixed tree: per thread the slots b = tid, tid +
+// blockDim, ... in order; then the wavefront's shuffle tree; then the
+// wavefronts in order, each component by a thread of its own; then rounded
+// once to double.  True on thread 0, with oa[] and ob[].  It ends with a
+// barrier, so a following call may reuse the shared buffers.
+template <int NA, int NB>
+__device__ bool finish_reduce2(const double *__restrict__ pa, int ca,
+                               const double *__restrict__ pb, int cb, double *oa, double *ob) {
+  constexpr int N = NA + NB;
+  __shared__ double sh[N][2][16];
+  __shared__ double res[N];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nw = blockDim.x >> 6;
-  double h[NC], l[NC];
+  double h[N], l[N];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) h[c] = l[c] = 0.0;
-  for (int b = threadIdx.x; b < count; b += blockDim.x) {
+  for (int c = 0; c < N; ++c) h[c] = l[c] = 0.0;
+  for (int b = threadIdx.x; b < ca; b += blockDim.x) {
 #pragma unroll
-    for (int c = 0; c < NC; ++c) dd_add(h[c], l[c], part[(b * NC + c) * 2], part[(b * NC + c) * 2 + 1]);
+    for (int c = 0; c < NA; ++c) dd_add(h[c], l[c], pa[(b * NA + c) * 2], pa[(b * NA + c) * 2 + 1]);
   }
-  wave_dd_reduce<NC>(h, l);
+  for (int b = threadIdx.x; b < cb; b += blockDim.x) {
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+      dd_add(h[NA + c], l[NA + c], pb[(b * NB + c) * 2], pb[(b * NB + c) * 2 + 1]);
+  }
+  wave_dd_reduce<N>(h, l);
   if (lane == 0) {
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
+    for (int c = 0; c < N; ++c) {
       sh[c][0][w] = h[c];
       sh[c][1][w] = l[c];
     }
   }
   __syncthreads();
-  if (threadIdx.x != 0) return false;
-#pragma unroll
-  for (int c = 0; c < NC; ++c) {
+  if (threadIdx.x < N) {
+    const int c = threadIdx.x;
     double ah = sh[c][0][0], al = sh[c][1][0];
     for (int k = 1; k < nw; ++k) dd_add(ah, al, sh[c][0][k], sh[c][1][k]);
-    out[c] = ah + al;
+    res[c] = ah + al;
   }
+  __syncthreads();
+  if (threadIdx.x != 0) return false;
+#pragma unroll
+  for (int c = 0; c < NA; ++c) oa[c] = res[c];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) ob[c] = res[NA + c];
   return true;
+}
+
+template <int NC>
+__device__ bool finish_reduce(const double *__restrict__ part, int count, double (&out)[NC]) {
+  return finish_reduce2<NC, 0>(part, count, nullptr, 0, out, nullptr);
 }
 
 // Makes this wavefront's LDS writes visible to its own lanes (rocPRIM's

@@ -307,9 +307,7 @@ __global__ __launch_bounds__(kFinWG) void k_fin_xr_rz(const double *__restrict__
                                                    int count_rz, Scal *S) {
   if (S->done) return;
   double rr[1], rz[1];  // part_xr: k_r_ic's one-component partials
-  const bool t0 = finish_reduce<1>(part_xr, count_xr, rr);
-  finish_reduce<1>(part_rz, count_rz, rz);
-  if (t0) {
+  if (finish_reduce2<1, 1>(part_xr, count_xr, part_rz, count_rz, rr, rz)) {
     const double rel = sqrt(rr[0]) / S->normb;
     const int it = S->iters + 1;
     S->iters = it;
