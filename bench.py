@@ -100,18 +100,23 @@ def kernel_table(S, n, nnz, nnz_l, mixed, nc, iters):
     import iccg_amd as I
     fwd = trsv_bytes(n, nnz_l, 4, 8, 4) if mixed else trsv_bytes(n, nnz_l, 8, 8, 8)
     bwd = trsv_bytes(n, nnz_l, 4, 4, 4) if mixed else trsv_bytes(n, nnz_l, 8, 8, 8)
-    fused = isinstance(S.M, I.IC0) and S.M.plan()[0] == "abmc-sectioned-fused"
+    plan = S.M.plan()[0] if isinstance(S.M, I.IC0) else "none"
+    fused = plan in ("abmc-sectioned-fused", "abmc-sectioned-fused-r")
+    # with the r update in the forward sweep, its bytes beyond the sweep's own
+    # read of r (Ap read, r written: 16n) move from the vector work to the sweeps
+    ru = 16 * n if plan == "abmc-sectioned-fused-r" else 0
     cls = [("spmv_dots", [0], spmv_bytes(n, nnz), "one SpMV launch (k_spmv_cg)")]
     if fused:
-        cls.append(("sweeps", [1, 2], fwd + bwd,
+        cls.append(("sweeps", [1, 2], fwd + bwd + ru,
                     f"one forward and one backward sweep ({2 * nc - 1} color launches: "
-                    "the last color forward and backward in one)"))
+                    "the last color forward and backward in one"
+                    + ("; the forward launches also apply r -= alpha Ap)" if ru else ")")))
     else:
         cls.append(("fwd_sweep", [1], fwd, f"one forward sweep ({nc} color launches)"))
         cls.append(("bwd_sweep", [2], bwd, f"one backward sweep ({nc} color launches)"))
-    cls.append(("vector_ops", [3], 80 * n,
-                "one iteration's vector and scalar work (alpha finish, x/r update + finish, "
-                "r.z + finish, p update)"))
+    cls.append(("vector_ops", [3], 80 * n - (24 * n if ru else 0),
+                "one iteration's vector and scalar work (alpha finish, "
+                + ("" if ru else "r update, ") + "r.z + stop-test finish, x and p update)"))
     out = {}
     for nm, groups, byt, unit in cls:
         cnt, ms = S.timing(groups[0])

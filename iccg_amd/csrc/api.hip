@@ -595,20 +595,24 @@ static void enqueue_iteration(iccg_solver_s &S, int it) {
     // kernel fewer per iteration); with r.z fused into the backward sweep
     // the test stays before the preconditioner
     const bool fused = bwd_fuses_rz(M.sc, M.F);
+    // sectioned ABMC factors: the forward sweep applies r -= alpha Ap itself
+    const bool ru = !fused && fwd_updates_r(M.sc, M.F);
     {
       Timer t(S, 3, it);
       launch_fin_alpha(A, S.S, S.part, S.st);
       if (fused)
         launch_xr(PK_IC, n, S.x, S.p, S.r, S.Ap, nullptr, nullptr, S.S, S.part, S.st);
-      else
+      else if (!ru)
         launch_r_ic(n, S.r, S.Ap, S.S, S.part, S.st);
     }
     // fused: the last color's backward rows run in the forward timer's
     // launch group
     const bool fb = sweeps_fused(M.sc, M.F);
+    int xr_count = 0;
     {
       Timer t(S, 1, it);
-      launch_fwd(M.sc, M.F, M.mixed, S.r, S.y, S.S, S.st, fb ? S.z : nullptr);
+      launch_fwd(M.sc, M.F, M.mixed, S.r, S.y, S.S, S.st, fb ? S.z : nullptr,
+                 ru ? S.Ap : nullptr, S.part, &xr_count);
     }
     {
       Timer t(S, 2, it);
@@ -618,7 +622,7 @@ static void enqueue_iteration(iccg_solver_s &S, int it) {
     if (fused) {
       launch_update_p(n, S.z, M.mixed, S.p, S.S, 0, S.st);
     } else {
-      launch_rz_stop(n, S.r, S.z, M.mixed, S.S, S.part, S.st);
+      launch_rz_stop(n, S.r, S.z, M.mixed, S.S, S.part, S.st, xr_count);
       launch_update_xp(n, S.x, S.z, M.mixed, S.p, S.S, S.st);
     }
   } else {
@@ -980,7 +984,9 @@ int iccg_precond_get_plan(iccg_precond_t M, int *plan, int *section_width) {
     else if (sc.kind == 1)
       *plan = ICCG_PLAN_MC_SLICED;
     else if (M->F.sec > 0) {
-      *plan = sweeps_fused(sc, M->F) ? ICCG_PLAN_ABMC_SECTIONED_FUSED : ICCG_PLAN_ABMC_SECTIONED;
+      *plan = !sweeps_fused(sc, M->F)  ? ICCG_PLAN_ABMC_SECTIONED
+              : fwd_updates_r(sc, M->F) ? ICCG_PLAN_ABMC_SECTIONED_FUSED_R
+                                        : ICCG_PLAN_ABMC_SECTIONED_FUSED;
       *section_width = M->F.sec;
     } else
       *plan = ICCG_PLAN_ABMC_SLICED;

@@ -217,8 +217,10 @@ void launch_precond_diag_init(int pk, int n, const double *r, const double *inv_
 void launch_r_ic(int n, double *r, const double *Ap, Scal *S, double *part, hipStream_t st);
 void launch_update_xp(int n, double *x, const void *z, bool z32, double *p, const Scal *S,
                       hipStream_t st);
+// xr_count: the |r|^2 partials the forward sweep left (launch_fwd's
+// ru_count), or 0 for launch_r_ic's
 void launch_rz_stop(int n, const double *r, const void *z, bool z32, Scal *S, double *part,
-                    hipStream_t st);
+                    hipStream_t st, int xr_count = 0);
 void launch_rz(int n, const double *r, const void *z, bool z32, Scal *S, double *part, int init,
                hipStream_t st);
 void launch_update_p(int n, const void *z, bool z32, double *p, const Scal *S, int init,
@@ -235,8 +237,14 @@ void launch_jacobi_setup(const DevCsr &A, double *inv_diag, hipStream_t st);
 // launch_fwd with z_fused writes z for the last color (and not its y), and
 // launch_bwd with last_done then skips that color.
 bool sweeps_fused(const DevSched &sc, const Factor &F);
+// Sectioned ABMC factors (and a partial buffer that holds the sweep's
+// workgroups) can also take the CG's r update into the forward sweep:
+// launch_fwd with Ap applies r -= alpha Ap to every row it starts, writes r
+// and leaves *ru_count |r|^2 partials where launch_r_ic leaves its own.
+bool fwd_updates_r(const DevSched &sc, const Factor &F);
 void launch_fwd(const DevSched &sc, const Factor &F, bool mixed, const double *r, void *y,
-                const Scal *S, hipStream_t st, void *z_fused = nullptr);
+                const Scal *S, hipStream_t st, void *z_fused = nullptr,
+                const double *Ap = nullptr, double *part = nullptr, int *ru_count = nullptr);
 void launch_bwd(const DevSched &sc, const Factor &F, bool mixed, const void *y, void *z,
                 const Scal *S, hipStream_t st, const double *r = nullptr, Scal *Sw = nullptr,
                 double *part = nullptr, int rz = 0, bool last_done = false);

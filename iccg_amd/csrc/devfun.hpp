@@ -659,17 +659,20 @@ struct SecRaw {
   VT d;
   double a;   // r_i (forward) or y_i widened (backward)
   double ri;  // backward with r
+  double ap;  // forward with the r update: (Ap)_i
 };
 
 // LOAD_NOW: the caller uses the loads at once, so absent outside entries
 // are skipped by (uniform) branches instead of loaded from stand-ins.
-template <class VT, class YT, int S, bool FWD, bool NT, bool LOAD_NOW = false>
+// RU (forward): also (Ap)_i, for the r update that the sweep applies.
+template <class VT, class YT, int S, bool FWD, bool NT, bool LOAD_NOW = false, bool RU = false>
 __device__ __forceinline__ void sec_fetch(int kb, int o0, int w, int bs, int be, int lane,
                                           const unsigned *__restrict__ inb,
                                           const int *__restrict__ col, const VT *__restrict__ v,
                                           const VT *__restrict__ diag,
                                           const double *__restrict__ r,
-                                          const YT *__restrict__ yin, SecRaw<VT, YT, S> &e) {
+                                          const YT *__restrict__ yin, SecRaw<VT, YT, S> &e,
+                                          const double *__restrict__ Ap = nullptr) {
   const int t = min(lane, be - bs - 1);  // a row of the block
   const int base = o0 + t;
   e.inb = ldm<NT>(&inb[64 * kb + t]);
@@ -696,6 +699,7 @@ __device__ __forceinline__ void sec_fetch(int kb, int o0, int w, int bs, int be,
   e.d = diag[bs + t];
   e.a = FWD ? r[bs + t] : (double)yin[bs + t];
   e.ri = (!FWD && r) ? r[bs + t] : 0.0;
+  e.ap = RU ? ldm<true>(&Ap[bs + t]) : 0.0;  // Ap: read once
 }
 
 // Lanes past the block end hold a copy of the block's last row (sec_fetch

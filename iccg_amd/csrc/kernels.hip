@@ -702,7 +702,13 @@ struct SecU {  // U's sectioned copy and the backward level counts (FB)
   const void *v;
 };
 
-template <class VT, class YT, int SW, bool FB, bool NT>
+//
+// With RU (IC(0) iterations) the sweep also applies the CG's r update to its
+// rows: r_i -= alpha (Ap)_i (cg_crs.cpp:141, the same fma as k_r_ic), starts
+// the row from the new r_i, stores it, and leaves the |r|^2 partials of the
+// deferred stop test at part[gidx0 + workgroup] (one launch per color, so
+// every row once).
+template <class VT, class YT, int SW, bool FB, bool NT, bool RU>
 __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
                                                       const int *__restrict__ bstart,
                                                       const int *__restrict__ nlev,
@@ -710,10 +716,11 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
                                                       const unsigned *__restrict__ inb,
                                                       const int *__restrict__ col,
                                                       const VT *__restrict__ lv,
-                                                      const VT *__restrict__ diag,
-                                                      const double *__restrict__ r,
+                                                      const VT *__restrict__ diag, double *r,
                                                       YT *__restrict__ y, SecU U,
-                                                      YT *__restrict__ z, const Scal *S) {
+                                                      YT *__restrict__ z, const Scal *S,
+                                                      const double *__restrict__ Ap,
+                                                      double *part, int gidx0) {
   constexpr bool PIPE = sizeof(YT) == 4;
   __shared__ YT win[(kWG / 64) * kSecWindow];
   if (S && S->done) return;
@@ -723,14 +730,19 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
   const int first = __builtin_amdgcn_readfirstlane(wr.first);
   const int step = __builtin_amdgcn_readfirstlane(wr.step);
   const int end = __builtin_amdgcn_readfirstlane(wr.end);
-  if (first >= end) return;
+  if (!RU && first >= end) return;  // RU: every wavefront reaches the partial
+  const double alpha = RU ? S->alpha : 0.0;
+  double h[1] = {0}, l[1] = {0};
   YT none[SW];
 #pragma unroll
   for (int k = 0; k < SW; ++k) none[k] = (YT)0;
-  int bs = bstart[first], be = bstart[first + 1], nl = nlev[first];
-  int o0 = off[first], w = (off[first + 1] - o0) >> 6;
+  const int f0 = min(first, kb1 - 1);
+  int bs = bstart[f0], be = bstart[f0 + 1], nl = nlev[f0];
+  int o0 = off[f0], w = (off[f0 + 1] - o0) >> 6;
   SecRaw<VT, YT, SW> cur;
-  if (PIPE) sec_fetch<VT, YT, SW, true, NT>(first, o0, w, bs, be, lane, inb, col, lv, diag, r, y, cur);
+  if (PIPE)
+    sec_fetch<VT, YT, SW, true, NT, false, RU>(f0, o0, w, bs, be, lane, inb, col, lv, diag, r, y,
+                                               cur, Ap);
   for (int kb = first; kb < end; kb += step) {
     if (!PIPE) {
       bs = bstart[kb];
@@ -738,12 +750,21 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
       nl = nlev[kb];
       o0 = off[kb];
       w = (off[kb + 1] - o0) >> 6;
-      sec_fetch<VT, YT, SW, true, NT, true>(kb, o0, w, bs, be, lane, inb, col, lv, diag, r, y, cur);
+      sec_fetch<VT, YT, SW, true, NT, true, RU>(kb, o0, w, bs, be, lane, inb, col, lv, diag, r, y,
+                                                cur, Ap);
     }
     const bool act = bs + lane < be;
     int ad[SW], co[SW];
     YT va[SW], vo[SW], d, acc, xo[SW];
     sec_take<VT, YT, SW>(cur, w, ad, va, co, vo, d, acc);
+    if (RU) {
+      const double rn = fma(-alpha, cur.ap, cur.a);
+      acc = (YT)rn;
+      if (act) {
+        r[bs + lane] = rn;
+        dd_fma(h[0], l[0], rn, rn);
+      }
+    }
 #pragma unroll
     for (int k = 0; k < SW; ++k)
       xo[k] = PIPE ? y[co[k] >= 0 ? co[k] : 0] : co[k] >= 0 ? y[co[k]] : (YT)0;
@@ -752,7 +773,8 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
     const int nnl = PIPE ? nlev[kc] : 0, no0 = PIPE ? off[kc] : 0;
     const int nw = PIPE ? (off[kc + 1] - no0) >> 6 : 0;
     if (PIPE)
-      sec_fetch<VT, YT, SW, true, NT>(kc, no0, nw, nbs, nbe, lane, inb, col, lv, diag, r, y, cur);
+      sec_fetch<VT, YT, SW, true, NT, false, RU>(kc, no0, nw, nbs, nbe, lane, inb, col, lv, diag,
+                                                 r, y, cur, Ap);
     int uad[SW];
     YT uva[SW];
     if (FB) {  // this block's U entries, in flight during the forward levels
@@ -786,6 +808,7 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
       w = nw;
     }
   }
+  if (RU) block_partial<1>(h, l, part, gidx0 + blockIdx.x);
 }
 
 template <class VT, class YT, int SW, bool RZ, bool NT>
@@ -988,14 +1011,14 @@ void launch_update_xp(int n, double *x, const void *z, bool z32, double *p, cons
 }
 
 void launch_rz_stop(int n, const double *r, const void *z, bool z32, Scal *S, double *part,
-                    hipStream_t st) {
+                    hipStream_t st, int xr_count) {
   dim3 g(quad_grid(n)), b(kWG);
   if (z32)
     hipLaunchKernelGGL(k_rz<float>, g, b, 0, st, n, r, (const float *)z, S, part, 0);
   else
     hipLaunchKernelGGL(k_rz<double>, g, b, 0, st, n, r, (const double *)z, S, part, 0);
-  hipLaunchKernelGGL(k_fin_xr_rz, dim3(1), dim3(kFinWG), 0, st, xr_parts(part), (int)g.x, part,
-                     (int)g.x, S);
+  hipLaunchKernelGGL(k_fin_xr_rz, dim3(1), dim3(kFinWG), 0, st, xr_parts(part),
+                     xr_count > 0 ? xr_count : (int)g.x, part, (int)g.x, S);
 }
 
 void launch_precond_diag_init(int pk, int n, const double *r, const double *inv_diag, double *z,
@@ -1214,26 +1237,58 @@ static void bwd_sliced(const DevSched &sc, const Factor &F, const VT *uv, const 
   if (fuse) hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(kFinWG), 0, st, part, total, S, rz == 2);
 }
 
-// forward colors [0, c_end); with uv, the last color through the fused
-// forward + backward launch as well
+// forward colors; with uv, the last color through the fused forward +
+// backward launch; with Ap, every launch also applies the r update (r is
+// then written: the caller's r is the solver's own vector) and the |r|^2
+// partials go to part[0, *ru_count)
 template <class VT, class YT, int SW, bool NT>
 static void fwd_sec(const DevSched &sc, const Factor &F, const VT *lv, const VT *dg,
-                    const double *r, YT *y, const Scal *S, hipStream_t st, const VT *uv = nullptr,
-                    YT *z = nullptr) {
+                    const double *r, YT *y, const Scal *S, hipStream_t st, const VT *uv,
+                    YT *z, const double *Ap, double *part, int *ru_count) {
   const SecU none{nullptr, nullptr, nullptr, nullptr};
+  const SecU us{sc.d_nlev_b, F.us.off, F.us.inb, (const void *)uv};
+  double *rw = const_cast<double *>(r);
+  int gidx0 = 0;
   for (int c = 0; c < sc.nc; ++c) {
     const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
     if (b1 <= b0) continue;
-    if (uv && c == sc.nc - 1)
-      hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, true, NT>), dim3(slice_grid(b1 - b0)),
-                         dim3(kWG), 0, st, b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.inb,
-                         F.ls.col, lv, dg, r, y,
-                         SecU{sc.d_nlev_b, F.us.off, F.us.inb, (const void *)uv}, z, S);
+    const int g = slice_grid(b1 - b0);
+    const bool fb = uv && c == sc.nc - 1;
+    if (Ap && fb)
+      hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, true, NT, true>), dim3(g), dim3(kWG), 0, st,
+                         b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.inb, F.ls.col, lv, dg,
+                         rw, y, us, z, S, Ap, part, gidx0);
+    else if (Ap)
+      hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, false, NT, true>), dim3(g), dim3(kWG), 0, st,
+                         b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.inb, F.ls.col, lv, dg,
+                         rw, y, none, nullptr, S, Ap, part, gidx0);
+    else if (fb)
+      hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, true, NT, false>), dim3(g), dim3(kWG), 0, st,
+                         b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.inb, F.ls.col, lv, dg,
+                         rw, y, us, z, S, nullptr, nullptr, 0);
     else
-      hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, false, NT>), dim3(slice_grid(b1 - b0)),
-                         dim3(kWG), 0, st, b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.inb,
-                         F.ls.col, lv, dg, r, y, none, nullptr, S);
+      hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, false, NT, false>), dim3(g), dim3(kWG), 0,
+                         st, b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.inb, F.ls.col, lv,
+                         dg, rw, y, none, nullptr, S, nullptr, nullptr, 0);
+    gidx0 += g;
   }
+  if (ru_count) *ru_count = Ap ? gidx0 : 0;
+}
+
+// workgroups of the forward sweep's launches on the sectioned plan: the
+// |r|^2 partials of the sweep's r update
+static int fwd_sec_groups(const DevSched &sc) {
+  int total = 0;
+  for (int c = 0; c < sc.nc; ++c) {
+    const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
+    if (b1 > b0) total += slice_grid(b1 - b0);
+  }
+  return total;
+}
+
+bool fwd_updates_r(const DevSched &sc, const Factor &F) {
+  return sliced_plan(sc, F) && sc.kind == 2 && F.sec > 0 && !fuse_rz_enabled() &&
+         fwd_sec_groups(sc) <= 2 * kMaxRedBlocks;
 }
 
 template <class VT, class YT, int SW, bool NT>
@@ -1296,15 +1351,19 @@ bool sweeps_fused(const DevSched &sc, const Factor &F) {
 }
 
 void launch_fwd(const DevSched &sc, const Factor &F, bool mixed, const double *r, void *y,
-                const Scal *S, hipStream_t st, void *z_fused) {
+                const Scal *S, hipStream_t st, void *z_fused, const double *Ap, double *part,
+                int *ru_count) {
+  if (ru_count) *ru_count = 0;
   if (sliced_plan(sc, F) && sc.kind == 2 && F.sec > 0) {
     const bool fb = z_fused && sweeps_fused(sc, F);
+    const double *ap = Ap && fwd_updates_r(sc, F) ? Ap : nullptr;
+    double *xp = ap ? xr_parts(part) : nullptr;
     if (mixed)
       DISPATCH_SEC(F.sec, F.nt, fwd_sec, float, float, sc, F, F.ls.v32, F.d32, r, (float *)y, S, st,
-                   fb ? F.us.v32 : nullptr, (float *)z_fused);
+                   fb ? F.us.v32 : nullptr, (float *)z_fused, ap, xp, ru_count);
     else
-      DISPATCH_SEC(F.sec, F.nt, fwd_sec, double, double, sc, F, F.ls.v64, F.d64, r, (double *)y, S, st,
-                   fb ? F.us.v64 : nullptr, (double *)z_fused);
+      DISPATCH_SEC(F.sec, F.nt, fwd_sec, double, double, sc, F, F.ls.v64, F.d64, r, (double *)y, S,
+                   st, fb ? F.us.v64 : nullptr, (double *)z_fused, ap, xp, ru_count);
     return;
   }
   if (sliced_plan(sc, F)) {

@@ -130,7 +130,7 @@ def test_abmc_grid_boxes():
     assert o.extra["nc"] == 2
     f = O.ic0_factor(P.orc(o.A), 0.0)
     M = I.IC0(o.A, 0.0, sched=o.sched)
-    assert M.plan() == ("abmc-sectioned-fused", 3)
+    assert M.plan() == ("abmc-sectioned-fused-r", 3)
     _factor_equal(M, f)
     r = I.gen_normal(A.n, 9)
     assert np.array_equal(M.apply(r, o.sched), O.ic0_apply(f, r, None))
@@ -212,8 +212,8 @@ def test_schedule_switch_lays_out_factor_again(precision):
     M = I.IC0(A, 0.0, sched=two, precision=precision)
     r = I.gen_normal(A.n, 12)
     expect = ref(f, r)
-    for sched, plan in [(two, ("abmc-sectioned-fused", 3)), (four, ("abmc-sliced", 0)),
-                        (two, ("abmc-sectioned-fused", 3)), (None, ("levels", 0)),
+    for sched, plan in [(two, ("abmc-sectioned-fused-r", 3)), (four, ("abmc-sliced", 0)),
+                        (two, ("abmc-sectioned-fused-r", 3)), (None, ("levels", 0)),
                         (four, ("abmc-sliced", 0))]:
         z = M.apply(r, sched)
         assert M.plan() == plan
