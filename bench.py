@@ -102,21 +102,23 @@ def kernel_table(S, n, nnz, nnz_l, mixed, nc, iters):
     bwd = trsv_bytes(n, nnz_l, 4, 4, 4) if mixed else trsv_bytes(n, nnz_l, 8, 8, 8)
     plan = S.M.plan()[0] if isinstance(S.M, I.IC0) else "none"
     fused = plan in ("abmc-sectioned-fused", "abmc-sectioned-fused-r")
-    # with the r update in the forward sweep, its bytes beyond the sweep's own
-    # read of r (Ap read, r written: 16n) move from the vector work to the sweeps
-    ru = 16 * n if plan == "abmc-sectioned-fused-r" else 0
+    # with the r update and r.z in the sweeps, their bytes move from the
+    # vector work (r update 24n, r.z 12n) to the sweeps, less the sweep's own
+    # read of r that the r update shares (8n)
+    ru = plan == "abmc-sectioned-fused-r"
     cls = [("spmv_dots", [0], spmv_bytes(n, nnz), "one SpMV launch (k_spmv_cg)")]
     if fused:
-        cls.append(("sweeps", [1, 2], fwd + bwd + ru,
+        cls.append(("sweeps", [1, 2], fwd + bwd + (28 * n if ru else 0),
                     f"one forward and one backward sweep ({2 * nc - 1} color launches: "
                     "the last color forward and backward in one"
-                    + ("; the forward launches also apply r -= alpha Ap)" if ru else ")")))
+                    + ("; the sweeps also apply r -= alpha Ap and form r.z)" if ru else ")")))
     else:
         cls.append(("fwd_sweep", [1], fwd, f"one forward sweep ({nc} color launches)"))
         cls.append(("bwd_sweep", [2], bwd, f"one backward sweep ({nc} color launches)"))
-    cls.append(("vector_ops", [3], 80 * n - (24 * n if ru else 0),
+    cls.append(("vector_ops", [3], 44 * n if ru else 80 * n,
                 "one iteration's vector and scalar work (alpha finish, "
-                + ("" if ru else "r update, ") + "r.z + stop-test finish, x and p update)"))
+                + ("stop-test finish" if ru else "r update, r.z + stop-test finish")
+                + ", x and p update)"))
     out = {}
     for nm, groups, byt, unit in cls:
         cnt, ms = S.timing(groups[0])

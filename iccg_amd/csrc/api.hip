@@ -616,13 +616,17 @@ static void enqueue_iteration(iccg_solver_s &S, int it) {
     }
     {
       Timer t(S, 2, it);
-      launch_bwd(M.sc, M.F, M.mixed, S.y, S.z, S.S, S.st, S.r, S.S, S.part, 1, fb);
+      launch_bwd(M.sc, M.F, M.mixed, S.y, S.z, S.S, S.st, S.r, S.S, S.part, 1, fb, ru);
     }
     Timer t(S, 3, it);
     if (fused) {
       launch_update_p(n, S.z, M.mixed, S.p, S.S, 0, S.st);
     } else {
-      launch_rz_stop(n, S.r, S.z, M.mixed, S.S, S.part, S.st, xr_count);
+      // with the r update in the forward sweep, r.z came out of the sweeps too
+      if (ru)
+        launch_fin_sweep_stop(M.sc, S.S, S.part, S.st);
+      else
+        launch_rz_stop(n, S.r, S.z, M.mixed, S.S, S.part, S.st, xr_count);
       launch_update_xp(n, S.x, S.z, M.mixed, S.p, S.S, S.st);
     }
   } else {

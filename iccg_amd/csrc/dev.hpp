@@ -240,14 +240,19 @@ bool sweeps_fused(const DevSched &sc, const Factor &F);
 // Sectioned ABMC factors (and a partial buffer that holds the sweep's
 // workgroups) can also take the CG's r update into the forward sweep:
 // launch_fwd with Ap applies r -= alpha Ap to every row it starts, writes r
-// and leaves *ru_count |r|^2 partials where launch_r_ic leaves its own.
+// and leaves *ru_count |r|^2 partials at the start of part.
 bool fwd_updates_r(const DevSched &sc, const Factor &F);
+// ... and then the sweeps also leave the r.z partials (the fused last
+// color's launch, and launch_bwd with rz_after_ru): the stop test's finish
+// folds both from the layout they leave in part, with no r.z pass between
+void launch_fin_sweep_stop(const DevSched &sc, Scal *S, double *part, hipStream_t st);
 void launch_fwd(const DevSched &sc, const Factor &F, bool mixed, const double *r, void *y,
                 const Scal *S, hipStream_t st, void *z_fused = nullptr,
                 const double *Ap = nullptr, double *part = nullptr, int *ru_count = nullptr);
 void launch_bwd(const DevSched &sc, const Factor &F, bool mixed, const void *y, void *z,
                 const Scal *S, hipStream_t st, const double *r = nullptr, Scal *Sw = nullptr,
-                double *part = nullptr, int rz = 0, bool last_done = false);
+                double *part = nullptr, int rz = 0, bool last_done = false,
+                bool rz_after_ru = false);
 // true when launch_bwd fuses r.z for this schedule (sliced MC / ABMC plans)
 // the triangular solves run on the sliced-ELL sweeps (contiguous MC colors,
 // or contiguous ABMC blocks of at most 64 rows; rows at most 16 wide)

@@ -707,7 +707,9 @@ struct SecU {  // U's sectioned copy and the backward level counts (FB)
 // rows: r_i -= alpha (Ap)_i (cg_crs.cpp:141, the same fma as k_r_ic), starts
 // the row from the new r_i, stores it, and leaves the |r|^2 partials of the
 // deferred stop test at part[gidx0 + workgroup] (one launch per color, so
-// every row once).
+// every row once).  With FB as well, the last color's z_i is final here, and
+// the r.z partials of its rows (cg_crs.cpp:152, r_i z_i as k_rz forms it)
+// go to part_rz[workgroup].
 template <class VT, class YT, int SW, bool FB, bool NT, bool RU>
 __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
                                                       const int *__restrict__ bstart,
@@ -720,7 +722,8 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
                                                       YT *__restrict__ y, SecU U,
                                                       YT *__restrict__ z, const Scal *S,
                                                       const double *__restrict__ Ap,
-                                                      double *part, int gidx0) {
+                                                      double *part, int gidx0,
+                                                      double *part_rz) {
   constexpr bool PIPE = sizeof(YT) == 4;
   __shared__ YT win[(kWG / 64) * kSecWindow];
   if (S && S->done) return;
@@ -732,7 +735,7 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
   const int end = __builtin_amdgcn_readfirstlane(wr.end);
   if (!RU && first >= end) return;  // RU: every wavefront reaches the partial
   const double alpha = RU ? S->alpha : 0.0;
-  double h[1] = {0}, l[1] = {0};
+  double h[1] = {0}, l[1] = {0}, hz[1] = {0}, lz[1] = {0};
   YT none[SW];
 #pragma unroll
   for (int k = 0; k < SW; ++k) none[k] = (YT)0;
@@ -757,8 +760,9 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
     int ad[SW], co[SW];
     YT va[SW], vo[SW], d, acc, xo[SW];
     sec_take<VT, YT, SW>(cur, w, ad, va, co, vo, d, acc);
+    double rn = 0.0;
     if (RU) {
-      const double rn = fma(-alpha, cur.ap, cur.a);
+      rn = fma(-alpha, cur.ap, cur.a);
       acc = (YT)rn;
       if (act) {
         r[bs + lane] = rn;
@@ -797,6 +801,7 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
       const YT zi = sec_levels<YT, SW, false>(U.nlev[kb], yi, d, uad, uva, none, none, my, lane,
                                               act);
       if (act) z[bs + lane] = zi;
+      if (RU && act) dd_fma(hz[0], lz[0], rn, (double)zi);
     } else if (act) {
       y[bs + lane] = yi;
     }
@@ -809,6 +814,7 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
     }
   }
   if (RU) block_partial<1>(h, l, part, gidx0 + blockIdx.x);
+  if (RU && FB) block_partial<1>(hz, lz, part_rz, blockIdx.x);
 }
 
 template <class VT, class YT, int SW, bool RZ, bool NT>
@@ -1138,6 +1144,17 @@ bool bwd_fuses_rz(const DevSched &sc, const Factor &F) {
   return sliced_plan(sc, F) && fuse_rz_enabled();
 }
 
+// workgroups of the forward sweep's launches on the sectioned plan: as many
+// |r|^2 partials of the sweep's r update, and as many r.z partials
+static int fwd_sec_groups(const DevSched &sc) {
+  int total = 0;
+  for (int c = 0; c < sc.nc; ++c) {
+    const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
+    if (b1 > b0) total += slice_grid(b1 - b0);
+  }
+  return total;
+}
+
 // partial slots of the fused backward sweep (one per workgroup of every
 // color launch)
 int bwd_partial_slots(const DevSched &sc, const Factor &F) {
@@ -1152,6 +1169,7 @@ int bwd_partial_slots(const DevSched &sc, const Factor &F) {
       if (b1 > b0) total += std::max((b1 - b0 + 3) / 4, slice_grid(b1 - b0));  // either grid
     }
   }
+  if (sc.kind == 2 && F.sec > 0) total = std::max(total, 2 * fwd_sec_groups(sc));
   return total;
 }
 
@@ -1239,8 +1257,9 @@ static void bwd_sliced(const DevSched &sc, const Factor &F, const VT *uv, const 
 
 // forward colors; with uv, the last color through the fused forward +
 // backward launch; with Ap, every launch also applies the r update (r is
-// then written: the caller's r is the solver's own vector) and the |r|^2
-// partials go to part[0, *ru_count)
+// then written: the caller's r is the solver's own vector), the |r|^2
+// partials go to part[0, G) and the fused launch's r.z partials to
+// part[G, ...) (G = fwd_sec_groups, *ru_count)
 template <class VT, class YT, int SW, bool NT>
 static void fwd_sec(const DevSched &sc, const Factor &F, const VT *lv, const VT *dg,
                     const double *r, YT *y, const Scal *S, hipStream_t st, const VT *uv,
@@ -1248,6 +1267,7 @@ static void fwd_sec(const DevSched &sc, const Factor &F, const VT *lv, const VT 
   const SecU none{nullptr, nullptr, nullptr, nullptr};
   const SecU us{sc.d_nlev_b, F.us.off, F.us.inb, (const void *)uv};
   double *rw = const_cast<double *>(r);
+  double *prz = part ? part + 2 * (size_t)fwd_sec_groups(sc) : nullptr;
   int gidx0 = 0;
   for (int c = 0; c < sc.nc; ++c) {
     const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
@@ -1257,44 +1277,38 @@ static void fwd_sec(const DevSched &sc, const Factor &F, const VT *lv, const VT 
     if (Ap && fb)
       hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, true, NT, true>), dim3(g), dim3(kWG), 0, st,
                          b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.inb, F.ls.col, lv, dg,
-                         rw, y, us, z, S, Ap, part, gidx0);
+                         rw, y, us, z, S, Ap, part, gidx0, prz);
     else if (Ap)
       hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, false, NT, true>), dim3(g), dim3(kWG), 0, st,
                          b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.inb, F.ls.col, lv, dg,
-                         rw, y, none, nullptr, S, Ap, part, gidx0);
+                         rw, y, none, nullptr, S, Ap, part, gidx0, nullptr);
     else if (fb)
       hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, true, NT, false>), dim3(g), dim3(kWG), 0, st,
                          b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.inb, F.ls.col, lv, dg,
-                         rw, y, us, z, S, nullptr, nullptr, 0);
+                         rw, y, us, z, S, nullptr, nullptr, 0, nullptr);
     else
       hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, false, NT, false>), dim3(g), dim3(kWG), 0,
                          st, b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.inb, F.ls.col, lv,
-                         dg, rw, y, none, nullptr, S, nullptr, nullptr, 0);
+                         dg, rw, y, none, nullptr, S, nullptr, nullptr, 0, nullptr);
     gidx0 += g;
   }
   if (ru_count) *ru_count = Ap ? gidx0 : 0;
 }
 
-// workgroups of the forward sweep's launches on the sectioned plan: the
-// |r|^2 partials of the sweep's r update
-static int fwd_sec_groups(const DevSched &sc) {
-  int total = 0;
-  for (int c = 0; c < sc.nc; ++c) {
-    const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
-    if (b1 > b0) total += slice_grid(b1 - b0);
-  }
-  return total;
+bool fwd_updates_r(const DevSched &sc, const Factor &F) {
+  return sliced_plan(sc, F) && sc.kind == 2 && F.sec > 0 && !fuse_rz_enabled();
 }
 
-bool fwd_updates_r(const DevSched &sc, const Factor &F) {
-  return sliced_plan(sc, F) && sc.kind == 2 && F.sec > 0 && !fuse_rz_enabled() &&
-         fwd_sec_groups(sc) <= 2 * kMaxRedBlocks;
+void launch_fin_sweep_stop(const DevSched &sc, Scal *S, double *part, hipStream_t st) {
+  const int g = fwd_sec_groups(sc);
+  hipLaunchKernelGGL(k_fin_xr_rz, dim3(1), dim3(kFinWG), 0, st, part, g, part + 2 * (size_t)g, g,
+                     S);
 }
 
 template <class VT, class YT, int SW, bool NT>
 static void bwd_sec(const DevSched &sc, const Factor &F, const VT *uv, const VT *dg, const YT *y,
                     YT *z, const double *r, Scal *S, double *part, int rz, hipStream_t st,
-                    bool skip_last = false) {
+                    bool skip_last = false, int rz_slot0 = -1) {
   std::vector<int> grids;  // workgroups of every color launch, for the r.z partials
   int total = 0;
   for (int t = 0; t < sc.nc; ++t) {
@@ -1303,8 +1317,10 @@ static void bwd_sec(const DevSched &sc, const Factor &F, const VT *uv, const VT 
     grids.push_back(b1 > b0 && !(skip_last && t == 0) ? slice_grid(b1 - b0) : 0);
     total += grids.back();
   }
-  const bool fuse = rz != 0 && r != nullptr && fuse_rz_enabled();
-  int gidx0 = 0;
+  // rz_slot0 >= 0: the r.z partials of the sweep's rows from that slot on,
+  // folded later by the stop test's finish (the forward sweep's r update)
+  const bool fuse = rz_slot0 >= 0 || (rz != 0 && r != nullptr && fuse_rz_enabled());
+  int gidx0 = rz_slot0 >= 0 ? rz_slot0 : 0;
   for (int t = 0; t < sc.nc; ++t) {
     const int c = sc.nc - 1 - t;
     const int g = grids[t];
@@ -1322,7 +1338,8 @@ static void bwd_sec(const DevSched &sc, const Factor &F, const VT *uv, const VT 
                          gidx0);
     gidx0 += g;
   }
-  if (fuse) hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(kFinWG), 0, st, part, total, S, rz == 2);
+  if (fuse && rz_slot0 < 0)
+    hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(kFinWG), 0, st, part, total, S, rz == 2);
 }
 
 // FN<VT, YT, width, nt> over the section / slot widths and the factor's
@@ -1357,7 +1374,7 @@ void launch_fwd(const DevSched &sc, const Factor &F, bool mixed, const double *r
   if (sliced_plan(sc, F) && sc.kind == 2 && F.sec > 0) {
     const bool fb = z_fused && sweeps_fused(sc, F);
     const double *ap = Ap && fwd_updates_r(sc, F) ? Ap : nullptr;
-    double *xp = ap ? xr_parts(part) : nullptr;
+    double *xp = ap ? part : nullptr;
     if (mixed)
       DISPATCH_SEC(F.sec, F.nt, fwd_sec, float, float, sc, F, F.ls.v32, F.d32, r, (float *)y, S, st,
                    fb ? F.us.v32 : nullptr, (float *)z_fused, ap, xp, ru_count);
@@ -1387,15 +1404,24 @@ void launch_fwd(const DevSched &sc, const Factor &F, bool mixed, const double *r
 
 void launch_bwd(const DevSched &sc, const Factor &F, bool mixed, const void *y, void *z,
                 const Scal *S, hipStream_t st, const double *r, Scal *Sw, double *part, int rz,
-                bool last_done) {
+                bool last_done, bool rz_after_ru) {
   if (sliced_plan(sc, F) && sc.kind == 2 && F.sec > 0) {
     const bool skip = last_done && sweeps_fused(sc, F);
+    // after the forward sweep's r update: r.z partials behind its |r|^2
+    // ones, after those of the fused last color
+    int slot0 = -1;
+    double *pz = part;
+    if (rz_after_ru && fwd_updates_r(sc, F)) {
+      pz = part + 2 * (size_t)fwd_sec_groups(sc);
+      const int cl = sc.nc - 1;
+      slot0 = skip ? slice_grid(sc.h_color_blk_v[cl + 1] - sc.h_color_blk_v[cl]) : 0;
+    }
     if (mixed)
       DISPATCH_SEC(F.sec, F.nt, bwd_sec, float, float, sc, F, F.us.v32, F.d32, (const float *)y,
-                   (float *)z, r, Sw, part, rz, st, skip);
+                   (float *)z, r, Sw, pz, rz, st, skip, slot0);
     else
       DISPATCH_SEC(F.sec, F.nt, bwd_sec, double, double, sc, F, F.us.v64, F.d64, (const double *)y,
-                   (double *)z, r, Sw, part, rz, st, skip);
+                   (double *)z, r, Sw, pz, rz, st, skip, slot0);
     return;
   }
   if (sliced_plan(sc, F)) {

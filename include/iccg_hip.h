@@ -232,8 +232,8 @@ enum {
   ICCG_PLAN_ABMC_SECTIONED_FUSED = 6,  /* the same, the last color forward
                                           and backward in one launch */
   ICCG_PLAN_ABMC_SECTIONED_FUSED_R = 7 /* and, in the solver's iterations,
-                                          the forward sweep applies the CG's
-                                          r update */
+                                          the sweeps apply the CG's r update
+                                          and form r.z */
 };
 int iccg_precond_get_plan(iccg_precond_t M, int *plan, int *section_width);
 /* apply(r, z, sched) (precond.hpp:12,38; ic0.hpp:34) on host buffers. */
