@@ -293,6 +293,36 @@ def test_cg_colored_matches_oracle(order):
     assert res.converged
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cg_sectioned_three_colors(precision):
+    """Three colors on the sectioned plan (three 64-row slabs, one color
+    each), so the sweeps' r update and r.z leave partials from several color
+    launches: FP64 equals the oracle's double-double solve bitwise; mixed
+    equals the batch engine's solve of the same system (which forms r and
+    r.z in passes of their own) bitwise and meets the tolerance on the true
+    residual."""
+    A = P.grid(4, 4, 12, coef=(-2.0, 2.0), shift=1e-6)
+    sched = _slab_schedule(A.n, 3)
+    M = I.IC0(A, 0.0, sched=sched, precision=precision)
+    assert M.plan() == ("abmc-sectioned-fused-r", 3)
+    b = I.gen_normal(A.n, 21)
+    x = np.zeros(A.n)
+    res = I.conjugate_gradient(A, b, x, M, 10000, 1e-8, sched)
+    assert res.converged
+    if precision == "fp64":
+        dd = _oracle(A, "ic0", sched, b=b, dot_mode=1)
+        assert res.iters == dd.iters and np.array_equal(x, dd.x)
+    else:
+        B = I.Batch(2)
+        slot = B.add(A, M, b)
+        assert B.run() == 0
+        fin, rb = B.result(slot)
+        assert fin and rb.converged
+        assert (rb.iters, rb.restarts) == (res.iters, res.restarts)
+        assert np.array_equal(B.x(slot), x)
+        assert O.true_resid(P.orc(A), x, b) < 1e-8
+
+
 def test_mixed_precision_true_residual():
     A = P.grid(24, 20, 16, coef=(-2.0, 2.0), shift=1e-6)
     o = P.abmc_order(A, grid_dims=(24, 20, 16))
