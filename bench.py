@@ -499,14 +499,21 @@ def main_serve(args):
     served_mode = I.Server("batched", precision="mixed", max_slots=args.slots,
                            workers=args.workers, hbm_budget=int(args.hbm_budget * (1 << 30)))
     out_b, span_b = drive(served_mode, reqs, arrivals)
+    st = served_mode.stats()
+    engine = {k: round(st[k], 1) for k in ("engine_run_ms", "engine_admit_ms",
+                                           "engine_harvest_ms")}
+    log(f"batched engine thread: {engine} (span {span_b * 1e3:.0f} ms)")
     del served_mode
     sb = summary(out_b, span_b, "batched mixed", True)
-    seq = I.Server("sequential", precision="fp64")
-    out_s, span_s = drive(seq, reqs, arrivals)
-    del seq
-    ss = summary(out_s, span_s, "one-at-a-time fp64", False)
-    err = max(float(np.linalg.norm(a[3] - c[3]) / np.linalg.norm(c[3]))
-              for a, c in zip(out_b, out_s))
+    sb["engine_ms"] = engine
+    ss, err = None, None
+    if not args.no_compare:
+        seq = I.Server("sequential", precision="fp64")
+        out_s, span_s = drive(seq, reqs, arrivals)
+        del seq
+        ss = summary(out_s, span_s, "one-at-a-time fp64", False)
+        err = max(float(np.linalg.norm(a[3] - c[3]) / np.linalg.norm(c[3]))
+                  for a, c in zip(out_b, out_s))
     line = {
         "metric": METRIC, "value": sb["solves_per_s"], "unit": "solves/s",
         "n_gpus": 1, "steps": 1, "warmup": 0,
@@ -523,7 +530,8 @@ def main_serve(args):
                    "hbm_budget_gib": args.hbm_budget},
         "p50_latency_ms": sb["p50_latency_ms"], "p99_latency_ms": sb["p99_latency_ms"],
         "batched_mixed": sb, "one_at_a_time_fp64": ss,
-        "speedup_vs_one_at_a_time_fp64": round(sb["solves_per_s"] / ss["solves_per_s"], 3),
+        "speedup_vs_one_at_a_time_fp64": (round(sb["solves_per_s"] / ss["solves_per_s"], 3)
+                                          if ss else None),
         "max_err_vs_fp64": err,
     }
     print(json.dumps(line), flush=True)

@@ -50,7 +50,9 @@ class ServerConfigC(C.Structure):
 class ServerStatsC(C.Structure):
     _fields_ = [("submitted", C.c_longlong), ("finished", C.c_longlong),
                 ("failed", C.c_longlong), ("hbm_in_use", C.c_longlong), ("waiting", C.c_int),
-                ("preparing", C.c_int), ("ready", C.c_int), ("running", C.c_int)]
+                ("preparing", C.c_int), ("ready", C.c_int), ("running", C.c_int),
+                ("engine_run_ms", C.c_double), ("engine_admit_ms", C.c_double),
+                ("engine_harvest_ms", C.c_double)]
 
 
 _llp = C.POINTER(C.c_longlong)

@@ -111,6 +111,7 @@ struct iccg_server_s {
   long long hbm_used = 0;
   long long submitted = 0, finished = 0, failed = 0;
   int in_prep = 0, running_reqs = 0;
+  double run_ms = 0, admit_ms = 0, harvest_ms = 0;  // engine thread (under mu)
   bool stop = false;
   std::vector<std::thread> workers;
   std::thread engine;
@@ -282,6 +283,11 @@ struct iccg_server_s {
           running_reqs++;
         }
       }
+      using clk = std::chrono::steady_clock;
+      auto ms_since = [](clk::time_point a) {
+        return std::chrono::duration<double, std::milli>(clk::now() - a).count();
+      };
+      const auto t_admit = clk::now();
       // a request the batch does not take fails alone
       for (Request *r : take) {
         r->state = Q_RUN;
@@ -308,7 +314,11 @@ struct iccg_server_s {
       }
       try {
         int nrun = 0;
+        const double a_ms = ms_since(t_admit);
+        const auto t_run = clk::now();
         ck(iccg_batch_run(B, 64, &nrun));
+        const double r_ms = ms_since(t_run);
+        const auto t_harvest = clk::now();
         for (int s = 0; s < (int)owner.size(); ++s) {
           Request *r = owner[(size_t)s].first;
           if (!r) continue;
@@ -332,6 +342,11 @@ struct iccg_server_s {
             complete_locked(r, ICCG_OK, "");
           }
         }
+        const double h_ms = ms_since(t_harvest);
+        std::lock_guard<std::mutex> lk(mu);
+        admit_ms += a_ms;
+        run_ms += r_ms;
+        harvest_ms += h_ms;
       } catch (const Error &e) {
         fail_all_running(e.status, e.what());
       } catch (const std::exception &e) {
@@ -554,6 +569,9 @@ int iccg_server_get_stats(iccg_server_t S, iccg_server_stats *st) {
     st->ready = (int)S->ready.size();
     st->running = S->running_reqs;
     st->hbm_in_use = S->hbm_used;
+    st->engine_run_ms = S->run_ms;
+    st->engine_admit_ms = S->admit_ms;
+    st->engine_harvest_ms = S->harvest_ms;
   });
 }
 

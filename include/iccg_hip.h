@@ -364,6 +364,10 @@ typedef struct {
 typedef struct {
   long long submitted, finished, failed, hbm_in_use;
   int waiting, preparing, ready, running;
+  /* engine thread, batched mode (ms): inside iccg_batch_run, admitting
+   * requests into slots, and harvesting finished right-hand sides (copy-out,
+   * un-permutation, freeing) -- the last two with no GPU work queued */
+  double engine_run_ms, engine_admit_ms, engine_harvest_ms;
 } iccg_server_stats;
 typedef struct iccg_server_s *iccg_server_t;
 int iccg_server_create(const iccg_server_config *cfg, iccg_server_t *out);
