@@ -298,9 +298,10 @@ __global__ __launch_bounds__(kFinWG) void k_fin_xr(const double *__restrict__ pa
 // The IC(0) iteration's stop test (cg_crs.cpp:143-149) taken after the
 // preconditioner instead of before it: the same |r| partials, the same
 // decision, and when the iteration goes on, beta from r.z (152-153).  In the
-// iteration that stops, the preconditioner and r.z ran for nothing and p is
-// left as it was (k_update_p sees done), so x, r, iters and rel are those of
-// the immediate test.
+// iteration that stops, the preconditioner and r.z ran for nothing and
+// k_update_xp applies only the x update (done == 1), so x, r, iters and rel
+// are those of the immediate test.  The partials come from k_r_ic and k_rz,
+// or from the sweeps that apply the r update and form r.z.
 __global__ __launch_bounds__(kFinWG) void k_fin_xr_rz(const double *__restrict__ part_xr,
                                                    int count_xr,
                                                    const double *__restrict__ part_rz,
