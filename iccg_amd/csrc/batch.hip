@@ -149,13 +149,14 @@ __global__ __launch_bounds__(kWG) void kb_ax(const GroupDesc *__restrict__ group
     ax_group<double, WM, KM>(g, desc, sc, c, D0.a_v64, part, nch, mode);
 }
 
-// x += alpha p; r -= alpha Ap; r.r partial -- per member
+// r -= alpha Ap (cg_crs.cpp:141); r.r partial -- per member.  x += alpha p
+// (line 139) runs in kb_update_xp, which reads p anyway.
 template <int KM>
-__global__ __launch_bounds__(kWG) void kb_xr(const GroupDesc *__restrict__ groups,
-                                             const SysDesc *__restrict__ desc,
-                                             const ChunkItem *__restrict__ items,
-                                             const SysScal *__restrict__ sc, double *part,
-                                             int nch) {
+__global__ __launch_bounds__(kWG) void kb_r(const GroupDesc *__restrict__ groups,
+                                            const SysDesc *__restrict__ desc,
+                                            const ChunkItem *__restrict__ items,
+                                            const SysScal *__restrict__ sc, double *part,
+                                            int nch) {
   const ChunkItem c = items[blockIdx.x];
   const GroupDesc g = group_of<KM>(groups, c);
   for (int j = 0; j < (KM == 1 ? 1 : g.k); ++j) {
@@ -166,10 +167,8 @@ __global__ __launch_bounds__(kWG) void kb_xr(const GroupDesc *__restrict__ group
     const int end = min(D.n, c.s1 << 6);
     double h[1] = {0}, l[1] = {0};
     for (int i = (c.s0 << 6) + threadIdx.x; i < end; i += kWG) {
-      // x and Ap: read by no other kernel of the iteration (nontemporal)
-      const double xi = fma(alpha, D.p[i], ldm<true>(&D.x[i]));
+      // Ap: read by no other kernel of the iteration (nontemporal)
       const double ri = fma(-alpha, ldm<true>(&D.Ap[i]), D.r[i]);
-      __builtin_nontemporal_store(xi, &D.x[i]);
       D.r[i] = ri;
       dd_fma(h[0], l[0], ri, ri);
     }
@@ -233,6 +232,42 @@ __global__ __launch_bounds__(kWG) void kb_update_p(const GroupDesc *__restrict__
       p_rows<float>(D, beg, end, beta, init);
     else
       p_rows<double>(D, beg, end, beta, init);
+  }
+}
+
+// x += alpha p (cg_crs.cpp:139) and p = z + beta p (155-157) -- per member:
+// for every slot whose x/r update ran this iteration (not after a breakdown,
+// not once it had stopped, done == 2), including the one whose stop test
+// ended it (done == 1), which updates no p
+template <class FT>
+__device__ __forceinline__ void xp_rows(const SysDesc &D, int beg, int end, double alpha,
+                                        double beta, bool pu) {
+  const FT *z = (const FT *)D.z;
+  for (int i = beg; i < end; i += kWG) {
+    const double pi = D.p[i];
+    // x: read by no other kernel of the iteration (nontemporal)
+    __builtin_nontemporal_store(fma(alpha, pi, ldm<true>(&D.x[i])), &D.x[i]);
+    if (pu) D.p[i] = fma(beta, pi, (double)z[i]);
+  }
+}
+
+template <int KM>
+__global__ __launch_bounds__(kWG) void kb_update_xp(const GroupDesc *__restrict__ groups,
+                                                    const SysDesc *__restrict__ desc,
+                                                    const ChunkItem *__restrict__ items,
+                                                    const SysScal *__restrict__ sc) {
+  const ChunkItem c = items[blockIdx.x];
+  const GroupDesc g = group_of<KM>(groups, c);
+  for (int j = 0; j < (KM == 1 ? 1 : g.k); ++j) {
+    const int id = g.slot[j];
+    const SysScal &S = sc[id];
+    if (S.done > 1 || S.breakdown) continue;
+    const SysDesc &D = desc[id];
+    const int beg = (c.s0 << 6) + threadIdx.x, end = min(D.n, c.s1 << 6);
+    if (D.mixed)
+      xp_rows<float>(D, beg, end, S.alpha, S.beta, !S.done);
+    else
+      xp_rows<double>(D, beg, end, S.alpha, S.beta, !S.done);
   }
 }
 
@@ -626,7 +661,10 @@ __global__ __launch_bounds__(kWG) void kbf_alpha(const FinItem *__restrict__ fin
                                                  SysScal *sc) {
   const FinItem f = fin[blockIdx.x];
   SysScal &S = sc[f.slot];
-  if (S.done) return;
+  if (S.done) {  // stopped before this iteration: no x update from now on
+    if (threadIdx.x == 0) S.done = 2;
+    return;
+  }
   double out[3];
   if (!fold_slot<3>(f, part, nch, out)) return;
   const double pAp = out[0];
@@ -640,15 +678,22 @@ __global__ __launch_bounds__(kWG) void kbf_alpha(const FinItem *__restrict__ fin
   }
 }
 
-__global__ __launch_bounds__(kWG) void kbf_xr(const FinItem *__restrict__ fin,
-                                              const double *__restrict__ part, int nch,
-                                              SysScal *sc) {
+// The stop test (cg_crs.cpp:143-149) taken after the preconditioner, and
+// when the iteration goes on, beta from r.z (152-153): kb_r's r.r partials
+// (part) and kb_rz's r.z partials (part_rz) folded in one pass.  In the
+// iteration that stops, the preconditioner and r.z ran for nothing and
+// kb_update_xp applies only the x update.
+__global__ __launch_bounds__(kWG) void kbf_xr_rz(const FinItem *__restrict__ fin,
+                                                 const double *__restrict__ part,
+                                                 const double *__restrict__ part_rz, int nch,
+                                                 SysScal *sc) {
   const FinItem f = fin[blockIdx.x];
   SysScal &S = sc[f.slot];
   if (S.done) return;
-  double out[1];
-  if (!fold_slot<1>(f, part, nch, out)) return;
-  const double rel = sqrt(out[0]) / S.normb;
+  double rr[1], rz[1];
+  const size_t o = ((size_t)f.plane * nch + f.c0) * 2;
+  if (!finish_reduce2<1, 1>(part + o, f.c1 - f.c0, part_rz + o, f.c1 - f.c0, rr, rz)) return;
+  const double rel = sqrt(rr[0]) / S.normb;
   const int it = S.iters + 1;
   S.iters = it;
   S.rel = rel;
@@ -657,6 +702,9 @@ __global__ __launch_bounds__(kWG) void kbf_xr(const FinItem *__restrict__ fin,
     S.done = 1;
   } else if (it >= S.max_iter) {
     S.done = 1;
+  } else {
+    S.beta = rz[0] / S.rz_old;
+    S.rz_old = rz[0];
   }
 }
 
@@ -1039,12 +1087,20 @@ void launch_iteration(iccg_batch_s &B, const ItemSet &I) {
   mark(B, T_SPMV, false);
   mark(B, T_VEC, true);
   hipLaunchKernelGGL(kbf_alpha, dim3(I.nfin), b, 0, B.st, I.d_fin, B.part, I.nchunks, B.d_sc);
-  VEC_LAUNCH(I.kmax, kb_xr, g, b, 0, B.st, I.d_groups, B.d_desc, I.d_chunks, B.d_sc, B.part,
+  VEC_LAUNCH(I.kmax, kb_r, g, b, 0, B.st, I.d_groups, B.d_desc, I.d_chunks, B.d_sc, B.part,
                      I.nchunks);
-  hipLaunchKernelGGL(kbf_xr, dim3(I.nfin), b, 0, B.st, I.d_fin, B.part, I.nchunks, B.d_sc);
   mark(B, T_VEC, false);
   launch_sweeps(B, I);
-  launch_rz_p(B, I, 0);
+  // the stop test after the preconditioner: the r.z partials go behind the
+  // r.r ones (which stay for the fold), then x and p together
+  double *part_rz = B.part + 2 * (size_t)I.nchunks * I.kmax;
+  mark(B, T_VEC, true);
+  VEC_LAUNCH(I.kmax, kb_rz, g, b, 0, B.st, I.d_groups, B.d_desc, I.d_chunks, B.d_sc, part_rz,
+                     I.nchunks);
+  hipLaunchKernelGGL(kbf_xr_rz, dim3(I.nfin), b, 0, B.st, I.d_fin, B.part, part_rz, I.nchunks,
+                     B.d_sc);
+  VEC_LAUNCH(I.kmax, kb_update_xp, g, b, 0, B.st, I.d_groups, B.d_desc, I.d_chunks, B.d_sc);
+  mark(B, T_VEC, false);
 }
 
 void mark(iccg_batch_s &B, int cls, bool begin) {
@@ -1503,7 +1559,7 @@ int iccg_batch_remove(iccg_batch_t B, int slot) {
     s.A = nullptr;
     s.M = nullptr;
     SysScal S{};
-    S.done = 1;
+    S.done = 2;  // stopped for good: no kernel touches it, the x update included
     B->h_sc[slot] = S;
     HIPCHK(hipMemcpyAsync(B->d_sc + slot, B->h_sc + slot, sizeof(S), hipMemcpyHostToDevice,
                           B->st));
