@@ -692,7 +692,14 @@ static void run_solver(iccg_solver_s &S, int max_iter, double tol, int flags,
   precond_init(S);
   HIPCHK(hipGetLastError());
   int issued = 0, restarts = 0;
-  int chunk = S.n < 100000 ? 8 : 16;
+  // chunk lengths double up to 64; once the recurrence residual has fallen
+  // from one chunk to the next, a chunk is no longer than the iterations its
+  // rate of decrease says are left (rounded up to a power of two, so the
+  // captured graphs are reused), so the chunk that reaches the stop runs few
+  // iterations past it
+  int cap = S.n < 100000 ? 8 : 16, chunk = cap;
+  double rel_prev = -1.0;
+  int it_prev = 0;
   for (;;) {
     const int k = std::min(chunk, max_iter - issued);
     enqueue_chunk(S, issued, k);
@@ -710,11 +717,24 @@ static void run_solver(iccg_solver_s &S, int max_iter, double tol, int flags,
         launch_resid(A, S.x, S.b, S.r, S.S, S.part, 2, tol, max_iter, S.st);
         precond_init(S);
         issued = S.hS->iters;
+        rel_prev = -1.0;
         continue;
       }
     }
     if (stop) break;
-    if (chunk < 64) chunk *= 2;
+    if (cap < 64) cap *= 2;
+    chunk = cap;
+    const double rel = S.hS->rel;
+    const int it = S.hS->iters;
+    if (rel_prev > 0.0 && rel > 0.0 && rel < rel_prev && it > it_prev && tol > 0.0) {
+      const double rate = (std::log(rel) - std::log(rel_prev)) / (it - it_prev);  // < 0
+      const double left = (std::log(tol) - std::log(rel)) / rate;
+      int want = 4;
+      while (want < cap && want < left + 1.0) want *= 2;
+      chunk = want;
+    }
+    rel_prev = rel;
+    it_prev = it;
   }
   auto t1 = std::chrono::steady_clock::now();
   res.iters = S.hS->iters;
