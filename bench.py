@@ -441,6 +441,11 @@ def main_mc128(args):
         cpu = cpu_baseline_systems([(Ap, sched, b)], args.tol, args.cpu_budget,
                                    "128^3 red-black system")
     dom = max(k.items(), key=lambda kv: kv[1]["avg_ms"] * kv[1]["count"])
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_mc128_{dom[0]}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
     line = {
         "metric": METRIC, "value": round(ms / 1e3, 4), "unit": "s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
@@ -453,7 +458,7 @@ def main_mc128(args):
         "hbm_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "kernels": k,
         "roofline": {"bound": "hbm", "kernel": dom[0], "unit_of_launch": dom[1]["unit"],
                      "achieved": dom[1]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": dom[1]["frac"], "traffic": None,
+                     "frac": dom[1]["frac"], "traffic": traffic,
                      "algorithmic_bytes_per_launch": dom[1]["bytes"],
                      "avg_ms": dom[1]["avg_ms"]},
         "cpu_baseline": cpu,
