@@ -595,8 +595,10 @@ static void enqueue_iteration(iccg_solver_s &S, int it) {
     // kernel fewer per iteration); with r.z fused into the backward sweep
     // the test stays before the preconditioner
     const bool fused = bwd_fuses_rz(M.sc, M.F);
-    // sectioned ABMC factors: the forward sweep applies r -= alpha Ap itself
+    // sectioned ABMC and multicolor factors: the forward sweep applies
+    // r -= alpha Ap itself; on the sectioned plan the sweeps also form r.z
     const bool ru = !fused && fwd_updates_r(M.sc, M.F);
+    const bool rzs = ru && sweeps_form_rz(M.sc, M.F);
     {
       Timer t(S, 3, it);
       launch_fin_alpha(A, S.S, S.part, S.st);
@@ -616,14 +618,14 @@ static void enqueue_iteration(iccg_solver_s &S, int it) {
     }
     {
       Timer t(S, 2, it);
-      launch_bwd(M.sc, M.F, M.mixed, S.y, S.z, S.S, S.st, S.r, S.S, S.part, 1, fb, ru);
+      launch_bwd(M.sc, M.F, M.mixed, S.y, S.z, S.S, S.st, S.r, S.S, S.part, 1, fb, rzs);
     }
     Timer t(S, 3, it);
     if (fused) {
       launch_update_p(n, S.z, M.mixed, S.p, S.S, 0, S.st);
     } else {
-      // with the r update in the forward sweep, r.z came out of the sweeps too
-      if (ru)
+      // on the sectioned plan r.z came out of the sweeps too
+      if (rzs)
         launch_fin_sweep_stop(M.sc, S.S, S.part, S.st);
       else
         launch_rz_stop(n, S.r, S.z, M.mixed, S.S, S.part, S.st, xr_count);
@@ -1006,7 +1008,7 @@ int iccg_precond_get_plan(iccg_precond_t M, int *plan, int *section_width) {
     if (!sliced_plan(sc, M->F))
       *plan = sc.kind == 0 ? ICCG_PLAN_LEVELS : ICCG_PLAN_CSR;
     else if (sc.kind == 1)
-      *plan = ICCG_PLAN_MC_SLICED;
+      *plan = fwd_updates_r(sc, M->F) ? ICCG_PLAN_MC_SLICED_R : ICCG_PLAN_MC_SLICED;
     else if (M->F.sec > 0) {
       *plan = !sweeps_fused(sc, M->F)  ? ICCG_PLAN_ABMC_SECTIONED
               : fwd_updates_r(sc, M->F) ? ICCG_PLAN_ABMC_SECTIONED_FUSED_R

@@ -242,6 +242,10 @@ bool sweeps_fused(const DevSched &sc, const Factor &F);
 // launch_fwd with Ap applies r -= alpha Ap to every row it starts, writes r
 // and leaves *ru_count |r|^2 partials at the start of part.
 bool fwd_updates_r(const DevSched &sc, const Factor &F);
+// the sectioned plan's sweeps also form r.z; the multicolor plan's take the
+// r update only (its |r|^2 partials where launch_r_ic leaves its own, so
+// launch_rz_stop with xr_count = *ru_count folds them)
+bool sweeps_form_rz(const DevSched &sc, const Factor &F);
 // ... and then the sweeps also leave the r.z partials (the fused last
 // color's launch, and launch_bwd with rz_after_ru): the stop test's finish
 // folds both from the layout they leave in part, with no r.z pass between

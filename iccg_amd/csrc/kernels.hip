@@ -598,21 +598,34 @@ __global__ void k_abmc(const int *__restrict__ bstart, const int *__restrict__ b
 // an earlier (forward) / later (backward) color, finished by a previous
 // launch.
 // ---------------------------------------------------------------------------
-template <class VT, class YT, int WM, bool NT>
+// RU (IC(0) iterations): the launch also applies the CG's r update to its
+// color's rows, r_i -= alpha (Ap)_i (cg_crs.cpp:141, as k_r_ic), starts them
+// from the new r_i and leaves the |r|^2 partials at part[gidx0 + workgroup].
+template <class VT, class YT, int WM, bool NT, bool RU>
 __global__ __launch_bounds__(kWG) void k_mc_fwd(int c0, int c1, Sell L, const VT *__restrict__ lv,
-                                                const VT *__restrict__ diag,
-                                                const double *__restrict__ r,
-                                                YT *__restrict__ y, const Scal *S) {
+                                                const VT *__restrict__ diag, double *r,
+                                                YT *__restrict__ y, const Scal *S,
+                                                const double *__restrict__ Ap, double *part,
+                                                int gidx0) {
   if (S && S->done) return;
+  const double alpha = RU ? S->alpha : 0.0;
+  double h[1] = {0}, l[1] = {0};
   const int lane = threadIdx.x & 63;
   const WaveRange wr = xcd_waves(c0 >> 6, ((c1 - 1) >> 6) + 1);
   for (int s = wr.first; s < wr.end; s += wr.step) {
     const int i = (s << 6) + lane;
     if (i < c0 || i >= c1) continue;
-    YT acc = (YT)r[i];
+    double ri = r[i];
+    if (RU) {
+      ri = fma(-alpha, ldm<true>(&Ap[i]), ri);  // Ap: read once
+      r[i] = ri;
+      dd_fma(h[0], l[0], ri, ri);
+    }
+    YT acc = (YT)ri;
     acc = sell_chain<VT, YT, YT, WM, true, NT>(L.off, L.col, lv, y, s, lane, acc);
     y[i] = acc / (YT)diag[i];
   }
+  if (RU) block_partial<1>(h, l, part, gidx0 + blockIdx.x);
 }
 
 template <class VT, class YT, int WM, bool RZ, bool NT>
@@ -1185,18 +1198,31 @@ static int abmc_grid(int nb) {
   return sizeof(VT) == 8 ? (nb + 3) / 4 : slice_grid(nb);
 }
 
+// with Ap (multicolor plan): every launch also applies the r update (r is
+// then written: the caller's r is the solver's own vector), the |r|^2
+// partials to part[0, *ru_count)
 template <class VT, class YT, int WM, bool NT>
 static void fwd_sliced(const DevSched &sc, const Factor &F, const VT *lv, const VT *dg,
-                       const double *r, YT *y, const Scal *S, hipStream_t st) {
+                       const double *r, YT *y, const Scal *S, hipStream_t st, const double *Ap,
+                       double *part, int *ru_count) {
   if (sc.kind == 1) {
+    double *rw = const_cast<double *>(r);
+    int gidx0 = 0;
     for (int c = 0; c < sc.nc; ++c) {
       const int c0 = sc.h_color_ptr_v[c], c1 = sc.h_color_ptr_v[c + 1];
       if (c1 <= c0) continue;
-      const int ns = ((c1 - 1) >> 6) - (c0 >> 6) + 1;
-      hipLaunchKernelGGL((k_mc_fwd<VT, YT, WM, NT>), dim3(slice_grid(ns)), dim3(kWG), 0, st, c0, c1,
-                         F.ls, lv, dg, r, y, S);
+      const int g = slice_grid(((c1 - 1) >> 6) - (c0 >> 6) + 1);
+      if (Ap)
+        hipLaunchKernelGGL((k_mc_fwd<VT, YT, WM, NT, true>), dim3(g), dim3(kWG), 0, st, c0, c1,
+                           F.ls, lv, dg, rw, y, S, Ap, part, gidx0);
+      else
+        hipLaunchKernelGGL((k_mc_fwd<VT, YT, WM, NT, false>), dim3(g), dim3(kWG), 0, st, c0, c1,
+                           F.ls, lv, dg, rw, y, S, nullptr, nullptr, 0);
+      gidx0 += g;
     }
+    if (ru_count) *ru_count = Ap ? gidx0 : 0;
   } else {
+    if (ru_count) *ru_count = 0;
     for (int c = 0; c < sc.nc; ++c) {
       const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
       if (b1 <= b0) continue;
@@ -1296,8 +1322,25 @@ static void fwd_sec(const DevSched &sc, const Factor &F, const VT *lv, const VT 
   if (ru_count) *ru_count = Ap ? gidx0 : 0;
 }
 
-bool fwd_updates_r(const DevSched &sc, const Factor &F) {
+// multicolor launches' workgroups, for the r update's |r|^2 partials
+static int mc_fwd_groups(const DevSched &sc) {
+  int total = 0;
+  for (int c = 0; c < sc.nc; ++c) {
+    const int c0 = sc.h_color_ptr_v[c], c1 = sc.h_color_ptr_v[c + 1];
+    if (c1 > c0) total += slice_grid(((c1 - 1) >> 6) - (c0 >> 6) + 1);
+  }
+  return total;
+}
+
+bool sweeps_form_rz(const DevSched &sc, const Factor &F) {
   return sliced_plan(sc, F) && sc.kind == 2 && F.sec > 0 && !fuse_rz_enabled();
+}
+
+bool fwd_updates_r(const DevSched &sc, const Factor &F) {
+  // multicolor: the |r|^2 partials go where k_r_ic leaves its own, which
+  // holds 2 kMaxRedBlocks
+  return sweeps_form_rz(sc, F) || (sliced_plan(sc, F) && sc.kind == 1 && !fuse_rz_enabled() &&
+                                   mc_fwd_groups(sc) <= 2 * kMaxRedBlocks);
 }
 
 void launch_fin_sweep_stop(const DevSched &sc, Scal *S, double *part, hipStream_t st) {
@@ -1386,12 +1429,14 @@ void launch_fwd(const DevSched &sc, const Factor &F, bool mixed, const double *r
   }
   if (sliced_plan(sc, F)) {
     const int wm = sweep_wm(F);
+    const double *ap = Ap && sc.kind == 1 && fwd_updates_r(sc, F) ? Ap : nullptr;
+    double *xp = ap ? xr_parts(part) : nullptr;
     if (mixed)
       DISPATCH_SWEEP_WM(wm, F.nt, fwd_sliced, float, float, sc, F, F.ls.v32, F.d32, r, (float *)y, S,
-                        st);
+                        st, ap, xp, ru_count);
     else
       DISPATCH_SWEEP_WM(wm, F.nt, fwd_sliced, double, double, sc, F, F.ls.v64, F.d64, r, (double *)y,
-                        S, st);
+                        S, st, ap, xp, ru_count);
     return;
   }
   if (mixed) {

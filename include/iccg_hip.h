@@ -231,9 +231,12 @@ enum {
   ICCG_PLAN_ABMC_SECTIONED = 5,  /* ABMC, sectioned layout, mask-free levels */
   ICCG_PLAN_ABMC_SECTIONED_FUSED = 6,  /* the same, the last color forward
                                           and backward in one launch */
-  ICCG_PLAN_ABMC_SECTIONED_FUSED_R = 7 /* and, in the solver's iterations,
-                                          the sweeps apply the CG's r update
-                                          and form r.z */
+  ICCG_PLAN_ABMC_SECTIONED_FUSED_R = 7, /* and, in the solver's iterations,
+                                           the sweeps apply the CG's r update
+                                           and form r.z */
+  ICCG_PLAN_MC_SLICED_R = 8            /* multicolor, sliced ELL, and the
+                                          forward sweep applies the r update
+                                          in the solver's iterations */
 };
 int iccg_precond_get_plan(iccg_precond_t M, int *plan, int *section_width);
 /* apply(r, z, sched) (precond.hpp:12,38; ic0.hpp:34) on host buffers. */

@@ -285,6 +285,9 @@ def test_cg_colored_matches_oracle(order):
     else:
         o = P.abmc_order(A, grid_dims=(20, 18, 16), box=(4, 4, 4))
     b = I.gen_normal(A.n, P.SEED + 1)
+    if order != "abmc8":  # the multicolor and box plans take the r update into the sweeps
+        plan = I.IC0(o.A, 0.0, sched=o.sched).plan()[0]
+        assert plan == ("mc-sliced-r" if order == "mc" else "abmc-sectioned-fused-r")
     res, x = _solve(o.A, "ic0", o.sched, b=b)
     dd = _oracle(o.A, "ic0", o.sched, b=b, dot_mode=1)
     ld = _oracle(o.A, "ic0", o.sched, b=b, dot_mode=0)
