@@ -106,8 +106,9 @@ def kernel_table(S, n, nnz, nnz_l, mixed, nc, iters):
     # vector work (r update 24n, r.z 12n) to the sweeps, less the sweep's own
     # read of r that the r update shares (8n)
     ru = plan == "abmc-sectioned-fused-r"
-    ru_mc = plan == "mc-sliced-r"  # the r update only (24n of vector work; 16n beyond
-    # the sweep's own read of r moves into the forward sweep)
+    # multicolor: the r update's 16n beyond the sweep's own read of r moves
+    # into the forward sweep, r.z's read of r (8n) into the backward one
+    ru_mc = plan == "mc-sliced-r"
     cls = [("spmv_dots", [0], spmv_bytes(n, nnz), "one SpMV launch (k_spmv_cg)")]
     if fused:
         cls.append(("sweeps", [1, 2], fwd + bwd + (28 * n if ru else 0),
@@ -118,11 +119,12 @@ def kernel_table(S, n, nnz, nnz_l, mixed, nc, iters):
         cls.append(("fwd_sweep", [1], fwd + (16 * n if ru_mc else 0),
                     f"one forward sweep ({nc} color launches"
                     + ("; with the r update r -= alpha Ap)" if ru_mc else ")")))
-        cls.append(("bwd_sweep", [2], bwd, f"one backward sweep ({nc} color launches)"))
-    cls.append(("vector_ops", [3], 44 * n if ru else 56 * n if ru_mc else 80 * n,
+        cls.append(("bwd_sweep", [2], bwd + (8 * n if ru_mc else 0),
+                    f"one backward sweep ({nc} color launches"
+                    + ("; forming r.z)" if ru_mc else ")")))
+    cls.append(("vector_ops", [3], 44 * n if ru or ru_mc else 80 * n,
                 "one iteration's vector and scalar work (alpha finish, "
-                + ("stop-test finish" if ru else "r.z + stop-test finish" if ru_mc
-                   else "r update, r.z + stop-test finish")
+                + ("stop-test finish" if ru or ru_mc else "r update, r.z + stop-test finish")
                 + ", x and p update)"))
     out = {}
     for nm, groups, byt, unit in cls:

@@ -596,7 +596,7 @@ static void enqueue_iteration(iccg_solver_s &S, int it) {
     // the test stays before the preconditioner
     const bool fused = bwd_fuses_rz(M.sc, M.F);
     // sectioned ABMC and multicolor factors: the forward sweep applies
-    // r -= alpha Ap itself; on the sectioned plan the sweeps also form r.z
+    // r -= alpha Ap itself and the sweeps also form r.z
     const bool ru = !fused && fwd_updates_r(M.sc, M.F);
     const bool rzs = ru && sweeps_form_rz(M.sc, M.F);
     {
@@ -624,7 +624,7 @@ static void enqueue_iteration(iccg_solver_s &S, int it) {
     if (fused) {
       launch_update_p(n, S.z, M.mixed, S.p, S.S, 0, S.st);
     } else {
-      // on the sectioned plan r.z came out of the sweeps too
+      // r.z came out of the sweeps too
       if (rzs)
         launch_fin_sweep_stop(M.sc, S.S, S.part, S.st);
       else

@@ -237,14 +237,14 @@ void launch_jacobi_setup(const DevCsr &A, double *inv_diag, hipStream_t st);
 // launch_fwd with z_fused writes z for the last color (and not its y), and
 // launch_bwd with last_done then skips that color.
 bool sweeps_fused(const DevSched &sc, const Factor &F);
-// Sectioned ABMC factors (and a partial buffer that holds the sweep's
-// workgroups) can also take the CG's r update into the forward sweep:
+// Sectioned ABMC and multicolor sliced factors (and a partial buffer that
+// holds the sweep's workgroups) can also take the CG's r update into the
+// forward sweep:
 // launch_fwd with Ap applies r -= alpha Ap to every row it starts, writes r
 // and leaves *ru_count |r|^2 partials at the start of part.
 bool fwd_updates_r(const DevSched &sc, const Factor &F);
-// the sectioned plan's sweeps also form r.z; the multicolor plan's take the
-// r update only (its |r|^2 partials where launch_r_ic leaves its own, so
-// launch_rz_stop with xr_count = *ru_count folds them)
+// the sweeps that take the r update also form r.z (multicolor: the |r|^2
+// partials where launch_r_ic leaves its own, the r.z ones where k_rz does)
 bool sweeps_form_rz(const DevSched &sc, const Factor &F);
 // ... and then the sweeps also leave the r.z partials (the fused last
 // color's launch, and launch_bwd with rz_after_ru): the stop test's finish

@@ -421,6 +421,40 @@ __device__ __forceinline__ double spmv_slice16(const int *__restrict__ off,
   return acc;
 }
 
+// sell_chain (WM > 0) for a wave-uniform s, with every load issued without a
+// branch: entries k >= w load entry 0 again (a line already fetched) and
+// padding gathers x[0], both dropped by a select, as in spmv_slice16.  The
+// branchy form, once the kernel also carries an r.z partial, compiled to
+// copies of the whole cc/vv arrays on every path (134 VGPRs at WM = 8).
+template <class VT, class AT, class XT, int WM, bool NEG, bool NT>
+__device__ __forceinline__ AT sell_chain_u(const int *__restrict__ off,
+                                           const int *__restrict__ col,
+                                           const VT *__restrict__ v, const XT *__restrict__ x,
+                                           int s, int lane, AT acc) {
+  static_assert(WM > 0, "sell_chain_u needs a fixed slice width bound");
+  const int o0 = off[s];
+  const int w = (off[s + 1] - o0) >> 6;
+  if (w == 0) return acc;  // a slice of rows without entries: entry 0 may not exist
+  const int base = o0 + lane;
+  int cc[WM];
+  VT vv[WM];
+#pragma unroll
+  for (int k = 0; k < WM; ++k) cc[k] = ldm<NT>(&col[k < w ? base + 64 * k : base]);
+#pragma unroll
+  for (int k = 0; k < WM; ++k) vv[k] = ldm<NT>(&v[k < w ? base + 64 * k : base]);
+#pragma unroll
+  for (int k = 0; k < WM; ++k) cc[k] = k < w ? cc[k] : -1;
+  AT xv[WM];
+#pragma unroll
+  for (int k = 0; k < WM; ++k) xv[k] = (AT)x[cc[k] >= 0 ? cc[k] : 0];
+#pragma unroll
+  for (int k = 0; k < WM; ++k) {
+    const AT t = NEG ? fma_t(-(AT)vv[k], xv[k], acc) : fma_t((AT)vv[k], xv[k], acc);
+    acc = cc[k] >= 0 ? t : acc;
+  }
+  return acc;
+}
+
 // acc op= sum_k v_k * x[c_k] over the entries of the lane's row in slice s,
 // ascending k; NEG selects acc - v*x (triangular solves) over acc + v*x.
 template <class VT, class AT, class XT, int WM, bool NEG, bool NT = false>

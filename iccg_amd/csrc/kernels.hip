@@ -612,7 +612,12 @@ __global__ __launch_bounds__(kWG) void k_mc_fwd(int c0, int c1, Sell L, const VT
   double h[1] = {0}, l[1] = {0};
   const int lane = threadIdx.x & 63;
   const WaveRange wr = xcd_waves(c0 >> 6, ((c1 - 1) >> 6) + 1);
-  for (int s = wr.first; s < wr.end; s += wr.step) {
+  // wave-uniform slice loop: the slice offsets load through the scalar unit
+  // and the slice width is a scalar branch
+  const int first = __builtin_amdgcn_readfirstlane(wr.first);
+  const int step = __builtin_amdgcn_readfirstlane(wr.step);
+  const int end = __builtin_amdgcn_readfirstlane(wr.end);
+  for (int s = first; s < end; s += step) {
     const int i = (s << 6) + lane;
     if (i < c0 || i >= c1) continue;
     double ri = r[i];
@@ -622,7 +627,7 @@ __global__ __launch_bounds__(kWG) void k_mc_fwd(int c0, int c1, Sell L, const VT
       dd_fma(h[0], l[0], ri, ri);
     }
     YT acc = (YT)ri;
-    acc = sell_chain<VT, YT, YT, WM, true, NT>(L.off, L.col, lv, y, s, lane, acc);
+    acc = sell_chain_u<VT, YT, YT, WM, true, NT>(L.off, L.col, lv, y, s, lane, acc);
     y[i] = acc / (YT)diag[i];
   }
   if (RU) block_partial<1>(h, l, part, gidx0 + blockIdx.x);
@@ -638,11 +643,14 @@ __global__ __launch_bounds__(kWG) void k_mc_bwd(int c0, int c1, Sell U, const VT
   const int lane = threadIdx.x & 63;
   const WaveRange wr = xcd_waves(c0 >> 6, ((c1 - 1) >> 6) + 1);
   double h[1] = {0}, l[1] = {0};
-  for (int s = wr.first; s < wr.end; s += wr.step) {
+  const int first = __builtin_amdgcn_readfirstlane(wr.first);
+  const int step = __builtin_amdgcn_readfirstlane(wr.step);
+  const int end = __builtin_amdgcn_readfirstlane(wr.end);
+  for (int s = first; s < end; s += step) {
     const int i = (s << 6) + lane;
     if (i < c0 || i >= c1) continue;
     YT acc = y[i];
-    acc = sell_chain<VT, YT, YT, WM, true, NT>(U.off, U.col, uv, z, s, lane, acc);
+    acc = sell_chain_u<VT, YT, YT, WM, true, NT>(U.off, U.col, uv, z, s, lane, acc);
     const YT zi = acc / (YT)diag[i];
     z[i] = zi;
     if (RZ) dd_fma(h[0], l[0], r[i], (double)zi);
@@ -1232,10 +1240,12 @@ static void fwd_sliced(const DevSched &sc, const Factor &F, const VT *lv, const 
   }
 }
 
+// rz_slot0 >= 0 (multicolor, after the forward sweep's r update): the r.z
+// partials of every launch from part[rz_slot0] on, folded by the stop test
 template <class VT, class YT, int WM, bool NT>
 static void bwd_sliced(const DevSched &sc, const Factor &F, const VT *uv, const VT *dg,
                        const YT *y, YT *z, const double *r, Scal *S, double *part, int rz,
-                       hipStream_t st) {
+                       hipStream_t st, int rz_slot0 = -1) {
   // workgroup counts of every color launch, for the cross-launch r.z sum
   std::vector<int> grids;
   for (int t = 0; t < sc.nc; ++t) {
@@ -1252,8 +1262,9 @@ static void bwd_sliced(const DevSched &sc, const Factor &F, const VT *uv, const 
   }
   int total = 0;
   for (int g : grids) total += g;
-  const bool fuse = rz != 0 && r != nullptr && fuse_rz_enabled();
-  int gidx0 = 0;
+  const bool fin = rz != 0 && r != nullptr && fuse_rz_enabled();
+  const bool fuse = fin || (rz_slot0 >= 0 && sc.kind == 1);
+  int gidx0 = fin ? 0 : rz_slot0;
   for (int t = 0; t < sc.nc; ++t) {
     const int c = sc.nc - 1 - t;
     const int g = grids[t];
@@ -1279,7 +1290,7 @@ static void bwd_sliced(const DevSched &sc, const Factor &F, const VT *uv, const 
     }
     gidx0 += g;
   }
-  if (fuse) hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(kFinWG), 0, st, part, total, S, rz == 2);
+  if (fin) hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(kFinWG), 0, st, part, total, S, rz == 2);
 }
 
 // forward colors; with uv, the last color through the fused forward +
@@ -1332,18 +1343,28 @@ static int mc_fwd_groups(const DevSched &sc) {
   return total;
 }
 
-bool sweeps_form_rz(const DevSched &sc, const Factor &F) {
-  return sliced_plan(sc, F) && sc.kind == 2 && F.sec > 0 && !fuse_rz_enabled();
+// multicolor: the |r|^2 partials of the forward launches go where k_r_ic
+// leaves its own, the r.z partials of the backward ones (as many: the same
+// color ranges and grids) where k_rz leaves its own; each region holds
+// 2 kMaxRedBlocks
+static bool mc_sweeps_form_r(const DevSched &sc, const Factor &F) {
+  return sliced_plan(sc, F) && sc.kind == 1 && !fuse_rz_enabled() &&
+         mc_fwd_groups(sc) <= 2 * kMaxRedBlocks;
 }
 
-bool fwd_updates_r(const DevSched &sc, const Factor &F) {
-  // multicolor: the |r|^2 partials go where k_r_ic leaves its own, which
-  // holds 2 kMaxRedBlocks
-  return sweeps_form_rz(sc, F) || (sliced_plan(sc, F) && sc.kind == 1 && !fuse_rz_enabled() &&
-                                   mc_fwd_groups(sc) <= 2 * kMaxRedBlocks);
+bool sweeps_form_rz(const DevSched &sc, const Factor &F) {
+  return (sliced_plan(sc, F) && sc.kind == 2 && F.sec > 0 && !fuse_rz_enabled()) ||
+         mc_sweeps_form_r(sc, F);
 }
+
+bool fwd_updates_r(const DevSched &sc, const Factor &F) { return sweeps_form_rz(sc, F); }
 
 void launch_fin_sweep_stop(const DevSched &sc, Scal *S, double *part, hipStream_t st) {
+  if (sc.kind == 1) {
+    const int g = mc_fwd_groups(sc);
+    hipLaunchKernelGGL(k_fin_xr_rz, dim3(1), dim3(kFinWG), 0, st, xr_parts(part), g, part, g, S);
+    return;
+  }
   const int g = fwd_sec_groups(sc);
   hipLaunchKernelGGL(k_fin_xr_rz, dim3(1), dim3(kFinWG), 0, st, part, g, part + 2 * (size_t)g, g,
                      S);
@@ -1472,12 +1493,14 @@ void launch_bwd(const DevSched &sc, const Factor &F, bool mixed, const void *y, 
   }
   if (sliced_plan(sc, F)) {
     const int wm = sweep_wm(F);
+    // multicolor after the forward sweep's r update: r.z partials from part[0]
+    const int slot0 = rz_after_ru && sc.kind == 1 && fwd_updates_r(sc, F) ? 0 : -1;
     if (mixed)
       DISPATCH_SWEEP_WM(wm, F.nt, bwd_sliced, float, float, sc, F, F.us.v32, F.d32, (const float *)y,
-                        (float *)z, r, Sw, part, rz, st);
+                        (float *)z, r, Sw, part, rz, st, slot0);
     else
       DISPATCH_SWEEP_WM(wm, F.nt, bwd_sliced, double, double, sc, F, F.us.v64, F.d64,
-                        (const double *)y, (double *)z, r, Sw, part, rz, st);
+                        (const double *)y, (double *)z, r, Sw, part, rz, st, slot0);
     return;
   }
   if (mixed) {

@@ -234,9 +234,10 @@ enum {
   ICCG_PLAN_ABMC_SECTIONED_FUSED_R = 7, /* and, in the solver's iterations,
                                            the sweeps apply the CG's r update
                                            and form r.z */
-  ICCG_PLAN_MC_SLICED_R = 8            /* multicolor, sliced ELL, and the
+  ICCG_PLAN_MC_SLICED_R = 8            /* multicolor, sliced ELL, and, in
+                                          the solver's iterations, the
                                           forward sweep applies the r update
-                                          in the solver's iterations */
+                                          and the backward sweep forms r.z */
 };
 int iccg_precond_get_plan(iccg_precond_t M, int *plan, int *section_width);
 /* apply(r, z, sched) (precond.hpp:12,38; ic0.hpp:34) on host buffers. */
