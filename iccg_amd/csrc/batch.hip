@@ -87,14 +87,14 @@ __device__ __forceinline__ void ax_group(const GroupDesc &g, const SysDesc *__re
     for (int s = c.s0 + w; s < c.s1; s += kWG >> 6) {
       int cc[WM];
       VT vv[WM];
-      sell_load<VT, WM>(D0.a_off, D0.a_col, av, s, lane, cc, vv);
+      sell_load<VT, WM, true>(D0.a_off, D0.a_col, av, s, lane, cc, vv);
       const int i = (s << 6) + lane;
 #pragma unroll
       for (int q = 0; q < KS; ++q) {
         if (!live[q]) continue;
         const SysDesc &D = desc[g.slot[j0 + q]];
         if (mode == 3) {
-          const double y = sell_apply<double, double, VT, WM, false>(cc, vv, D.p, 0.0);
+          const double y = sell_apply_sel<double, double, VT, WM, false>(cc, vv, D.p, 0.0);
           if (i < D0.n) {
             __builtin_nontemporal_store(y, &D.Ap[i]);  // read once, by kb_xr
             const double pi = D.p[i];
@@ -103,7 +103,7 @@ __device__ __forceinline__ void ax_group(const GroupDesc &g, const SysDesc *__re
             dd_fma(h[q][2], l[q][2], y, y);
           }
         } else {
-          const double ax = sell_apply<double, double, VT, WM, false>(cc, vv, D.x, 0.0);
+          const double ax = sell_apply_sel<double, double, VT, WM, false>(cc, vv, D.x, 0.0);
           if (i < D0.n) {
             const double bi = D.b[i], ri = bi - ax;
             if (mode != 0) D.r[i] = ri;

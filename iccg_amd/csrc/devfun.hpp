@@ -296,7 +296,10 @@ __device__ __forceinline__ void load_tbl16(const int *__restrict__ tbl, int s, i
 }
 
 // The lane's row of slice s as registers: columns (padding -1) and values.
-template <class VT, int WM>
+// SEL: the loads of a slice with entries are issued without a branch
+// (entries k >= w load entry 0 again, in a line already fetched, and become
+// padding); faster for the batch SpMV, slower for the batch sweeps.
+template <class VT, int WM, bool SEL = false>
 __device__ __forceinline__ void sell_load(const int *__restrict__ off, const int *__restrict__ col,
                                           const VT *__restrict__ v, int s, int lane,
                                           int (&cc)[WM], VT (&vv)[WM]) {
@@ -307,10 +310,18 @@ __device__ __forceinline__ void sell_load(const int *__restrict__ off, const int
   for (int k = 0; k < WM; ++k) {
     cc[k] = -1;
     vv[k] = (VT)0;
-    if (k < w) {
+    if (!SEL && k < w) {
       cc[k] = ldm<true>(&col[base + 64 * k]);
       vv[k] = ldm<true>(&v[base + 64 * k]);
     }
+  }
+  if (SEL && w > 0) {  // a slice of rows without entries: entry 0 may not exist
+#pragma unroll
+    for (int k = 0; k < WM; ++k) cc[k] = ldm<true>(&col[k < w ? base + 64 * k : base]);
+#pragma unroll
+    for (int k = 0; k < WM; ++k) vv[k] = ldm<true>(&v[k < w ? base + 64 * k : base]);
+#pragma unroll
+    for (int k = 0; k < WM; ++k) cc[k] = k < w ? cc[k] : -1;
   }
 }
 
@@ -325,6 +336,26 @@ __device__ __forceinline__ AT sell_apply(const int (&cc)[WM], const VT (&vv)[WM]
 #pragma unroll
   for (int k = 0; k < WM; ++k)
     if (cc[k] >= 0) acc = NEG ? fma_t(-(AT)vv[k], xv[k], acc) : fma_t((AT)vv[k], xv[k], acc);
+  return acc;
+}
+
+// sell_apply with the gathers and the sum issued without a per-entry branch:
+// a row's padding follows its entries, so a row whose first entry is padding
+// has none; the others gather x[0] for padding and drop it by a select.  Same
+// result; fewer VGPRs where the caller also carries dot partials (the batch
+// SpMV: 82 -> 70 at WM = 8), more in the batch sweeps, which keep sell_apply.
+template <class AT, class XT, class VT, int WM, bool NEG>
+__device__ __forceinline__ AT sell_apply_sel(const int (&cc)[WM], const VT (&vv)[WM],
+                                             const XT *__restrict__ x, AT acc) {
+  if (cc[0] < 0) return acc;
+  AT xv[WM];
+#pragma unroll
+  for (int k = 0; k < WM; ++k) xv[k] = (AT)x[cc[k] >= 0 ? cc[k] : 0];
+#pragma unroll
+  for (int k = 0; k < WM; ++k) {
+    const AT t = NEG ? fma_t(-(AT)vv[k], xv[k], acc) : fma_t((AT)vv[k], xv[k], acc);
+    acc = cc[k] >= 0 ? t : acc;
+  }
   return acc;
 }
 
