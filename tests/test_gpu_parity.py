@@ -424,3 +424,56 @@ def test_graph_replay_matches_eager():
     # max_iter not a multiple of the chunk size
     r = S.run(7, 1e-8, zero_x0=True)
     assert r.iters == 7 and not r.converged
+
+
+def _random_lower(n, per_row, band, seed):
+    """Lower-triangular CRS (diagonal last) of a symmetric matrix with
+    per_row random neighbours j in [i - band, i) per row, entries
+    -10^U[-1,1], diagonal 1 + the row's absolute sum over both triangles."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for i in range(n):
+        lo = max(0, i - band)
+        cols = np.unique(rng.integers(lo, i, size=min(per_row, i))) if i > 0 else np.zeros(0, int)
+        rows.append(cols)
+    off = [(-(10.0 ** rng.uniform(-1, 1, len(c)))) for c in rows]
+    absrow = np.zeros(n)
+    for i, (c, v) in enumerate(zip(rows, off)):
+        absrow[i] += np.abs(v).sum()
+        np.add.at(absrow, c, np.abs(v))
+    rp = np.zeros(n + 1, np.int32)
+    ci, va = [], []
+    for i, (c, v) in enumerate(zip(rows, off)):
+        ci.extend(c.tolist() + [i])
+        va.extend(v.tolist() + [1.0 + absrow[i]])
+        rp[i + 1] = len(ci)
+    return I.CRS(n, rp, np.asarray(ci, np.int32), np.asarray(va, np.float64))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cg_multicolor_many_colors(precision):
+    """A multicolor plan with several colors (greedy coloring of a random
+    banded pattern), so the forward sweep's r update and the backward
+    sweep's r.z leave partials from several color launches: FP64 equals the
+    oracle's double-double solve bitwise; mixed equals the batch engine's
+    solve (which forms r and r.z in passes of their own) bitwise."""
+    o = P.mc_order(_random_lower(3000, 3, 48, 5))
+    assert o.extra["nc"] >= 3
+    M = I.IC0(o.A, 0.0, sched=o.sched, precision=precision)
+    assert M.plan()[0] == "mc-sliced-r"
+    b = I.gen_normal(o.A.n, 22)
+    x = np.zeros(o.A.n)
+    res = I.conjugate_gradient(o.A, b, x, M, 10000, 1e-8, o.sched)
+    assert res.converged
+    if precision == "fp64":
+        dd = _oracle(o.A, "ic0", o.sched, b=b, dot_mode=1)
+        assert res.iters == dd.iters and np.array_equal(x, dd.x)
+    else:
+        B = I.Batch(2)
+        slot = B.add(o.A, M, b)
+        assert B.run() == 0
+        fin, rb = B.result(slot)
+        assert fin and rb.converged
+        assert (rb.iters, rb.restarts) == (res.iters, res.restarts)
+        assert np.array_equal(B.x(slot), x)
+        assert O.true_resid(P.orc(o.A), x, b) < 1e-8
