@@ -1021,8 +1021,9 @@ void launch_xr(int pk, int n, double *x, const double *p, double *r, const doubl
   }
 }
 
-// the |r|^2 partials of the deferred stop test: partial pairs from
-// kMaxRedBlocks on, clear of the r.z partials (pairs below kMaxRedBlocks)
+// the |r|^2 partials of the deferred stop test (k_r_ic, at most
+// kMaxRedBlocks workgroups): pairs from kMaxRedBlocks on, clear of k_rz's
+// r.z pairs below kMaxRedBlocks
 static double *xr_parts(double *part) { return part + 2 * kMaxRedBlocks; }
 
 void launch_r_ic(int n, double *r, const double *Ap, Scal *S, double *part, hipStream_t st) {
@@ -1192,6 +1193,9 @@ int bwd_partial_slots(const DevSched &sc, const Factor &F) {
     }
   }
   if (sc.kind == 2 && F.sec > 0) total = std::max(total, 2 * fwd_sec_groups(sc));
+  // multicolor: the backward launches' r.z pairs from slot 0, the forward
+  // launches' |r|^2 pairs behind them (mc_xr_parts)
+  if (sc.kind == 1) total += std::max(kMaxRedBlocks, total);
   return total;
 }
 
@@ -1343,10 +1347,15 @@ static int mc_fwd_groups(const DevSched &sc) {
   return total;
 }
 
-// multicolor: the |r|^2 partials of the forward launches go where k_r_ic
-// leaves its own, the r.z partials of the backward ones (as many: the same
-// color ranges and grids) where k_rz leaves its own; each region holds
-// 2 kMaxRedBlocks
+// multicolor: the r.z partials of the backward launches go where k_rz leaves
+// its own (pairs 0 .. g-1), the |r|^2 partials of the forward ones (as many:
+// the same color ranges and grids) in the g pairs from max(kMaxRedBlocks, g)
+// on, so that neither region overlaps the other for any g; the buffer holds
+// that many pairs (bwd_partial_slots)
+static double *mc_xr_parts(const DevSched &sc, double *part) {
+  return part + 2 * (size_t)std::max(kMaxRedBlocks, mc_fwd_groups(sc));
+}
+
 static bool mc_sweeps_form_r(const DevSched &sc, const Factor &F) {
   return sliced_plan(sc, F) && sc.kind == 1 && !fuse_rz_enabled() &&
          mc_fwd_groups(sc) <= 2 * kMaxRedBlocks;
@@ -1362,7 +1371,8 @@ bool fwd_updates_r(const DevSched &sc, const Factor &F) { return sweeps_form_rz(
 void launch_fin_sweep_stop(const DevSched &sc, Scal *S, double *part, hipStream_t st) {
   if (sc.kind == 1) {
     const int g = mc_fwd_groups(sc);
-    hipLaunchKernelGGL(k_fin_xr_rz, dim3(1), dim3(kFinWG), 0, st, xr_parts(part), g, part, g, S);
+    hipLaunchKernelGGL(k_fin_xr_rz, dim3(1), dim3(kFinWG), 0, st, mc_xr_parts(sc, part), g, part,
+                       g, S);
     return;
   }
   const int g = fwd_sec_groups(sc);
@@ -1451,7 +1461,7 @@ void launch_fwd(const DevSched &sc, const Factor &F, bool mixed, const double *r
   if (sliced_plan(sc, F)) {
     const int wm = sweep_wm(F);
     const double *ap = Ap && sc.kind == 1 && fwd_updates_r(sc, F) ? Ap : nullptr;
-    double *xp = ap ? xr_parts(part) : nullptr;
+    double *xp = ap ? mc_xr_parts(sc, part) : nullptr;
     if (mixed)
       DISPATCH_SWEEP_WM(wm, F.nt, fwd_sliced, float, float, sc, F, F.ls.v32, F.d32, r, (float *)y, S,
                         st, ap, xp, ru_count);

@@ -1,23 +1,32 @@
-"""The two GPU configurations at their full sizes, through properties that
-hold at any size (the oracle's CG would take minutes there):
+"""The two single-system GPU configurations at their full sizes, against the
+reference's own CG on the same permuted matrix and right-hand side:
 
+- oracle/_ref (the reference compiled from its sources, OpenMP on the host
+  cores) runs conjugate_gradient<IC0> (cg_crs.cpp:89-162) to tolerance; the
+  GPU FP64 solve takes the same number of iterations within one, its final
+  recurrence residual is within 1e-12 of the reference's when the counts
+  agree, and ||x - x_ref|| / ||x_ref|| is reported for FP64 and mixed;
+- the GPU's true residual equals the oracle's ||A x - b|| / ||b||
+  (cg_crs.cpp:517-528) on the GPU's x;
+- at 128^3 the FP64 solve equals the oracle's double-double CG bit for bit
+  and the batch engine's solve of the same system (which forms r and r.z in
+  passes of their own) bit for bit;
 - the GPU IC(0) factor satisfies (L L^T)_ij = a_ij on the stored pattern, to
-  rounding (the defining identity of zero-fill IC);
-- a multicolor / ABMC apply equals the natural-order (level-scheduled)
-  apply on the same permuted factor, bit for bit (a valid coloring only
-  reorders independent rows);
-- the solves converge, the FP64 true residual (oracle SpMV) meets the
-  tolerance, and mixed precision agrees with FP64.
+  rounding, and a multicolor / ABMC apply equals the natural-order apply.
 """
+import os
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
 
 import iccg_amd as I
 from oracle import oracle as O
+from oracle import ref as R
 from tests import problems as P
 
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not R.available(), reason="oracle/_ref not built")]
 
 
 def _permuted_rhs(o, seed):
@@ -38,28 +47,99 @@ def _ic0_identity_error(A, M):
     return abs(diff).max() / abs(lower).max()
 
 
+def _reference_cg(A, b, sched):
+    """The reference's IC(0)-CG on the host cores (its OpenMP loops)."""
+    R.set_threads(min(16, os.cpu_count() or 1))
+    try:
+        it, rel, conv, x = R.cg(P.orc(A), b, "ic0", sched=P.orc_sched(sched), tol=1e-8,
+                                max_iter=20000)
+    finally:
+        R.set_threads(1)
+    assert conv
+    return it, rel, x
+
+
+def _assert_true_resid(Ao, x, b, res):
+    want = O.true_resid_dd(Ao, x, b)
+    assert abs(res.true_rel_resid - want) <= 4.5e-16 * want
+    seq = O.true_resid(Ao, x, b)
+    assert abs(res.true_rel_resid - seq) <= 2 * Ao.n * 1.12e-16 * seq
+
+
+def _err(x, xr):
+    return float(np.linalg.norm(x - xr) / np.linalg.norm(xr))
+
+
 def test_config2_128cube_red_black_fp64():
     """3D 7-point Laplacian 128^3 (n = 2,097,152), red-black IC(0), FP64,
-    b ~ N(0,1), tol 1e-8."""
+    b ~ N(0,1), tol 1e-8.  The two colors' sweeps run 4,096 workgroups, so
+    the r.z and |r|^2 partial regions of the fused multicolor iteration both
+    hold more than 2,048 pairs."""
     N = 128
     o = P.mc_order(P.grid(N, N, N))
     assert o.sched.nc == 2 and o.A.nnz == 14_581_760
     M = I.IC0(o.A, 0.0, sched=o.sched)
+    assert M.plan()[0] == "mc-sliced-r"
     assert _ic0_identity_error(o.A, M) < 1e-14
     r = I.gen_normal(o.A.n, 77)
-    z_mc = M.apply(r, o.sched)
-    z_nat = M.apply(r, None)
-    assert np.array_equal(z_mc, z_nat)
+    assert np.array_equal(M.apply(r, o.sched), M.apply(r, None))
     bp = _permuted_rhs(o, P.SEED + 2)
     x = np.zeros(o.A.n)
     res = I.conjugate_gradient(o.A, bp, x, M, 10000, 1e-8, o.sched)
     assert res.converged and res.rel_resid < 1e-8 and not res.breakdown
-    assert O.true_resid(P.orc(o.A), x, bp) < 1.5e-8
+    Ao = P.orc(o.A)
+    _assert_true_resid(Ao, x, bp, res)
+    # the oracle's double-double CG: the same iterates bit for bit
+    dd = O.cg(Ao, bp, "ic0", ic=O.ic0_factor(Ao), sched=P.orc_sched(o.sched), tol=1e-8,
+              dot_mode=1)
+    assert (res.iters, res.rel_resid) == (dd.iters, dd.rel_resid)
+    assert np.array_equal(x, dd.x)
+    # the reference's CG (80-bit dots, OpenMP): +-1 iteration, residual 1e-12
+    it, rel, xr = _reference_cg(o.A, bp, o.sched)
+    assert abs(res.iters - it) <= 1
+    if res.iters == it:
+        assert abs(res.rel_resid - rel) <= 1e-12
+    e = _err(x, xr)
+    print(f"[128^3 red-black] gpu fp64 iters {res.iters} rel {res.rel_resid:.6e}; "
+          f"reference iters {it} rel {rel:.6e}; ||x-x_ref||/||x_ref|| = {e:.3e}")
+    assert e < 1e-6
+    # the batch engine on the same system: bit for bit
+    B = I.Batch(1)
+    slot = B.add(o.A, M, bp)
+    assert B.run() == 0
+    fin, rb = B.result(slot)
+    assert fin and (rb.iters, rb.rel_resid, rb.converged) == (res.iters, res.rel_resid, True)
+    assert np.array_equal(B.x(slot), x)
+
+
+def test_multicolor_three_colors_over_2048_sweep_groups():
+    """84^3 Laplacian with the valid 3-coloring (i + j + k) mod 3: three
+    colors of 197,568 rows, 772 sweep workgroups each (2,316 in all), so the
+    fused multicolor iteration's partial regions exceed 2,048 pairs with
+    more than two color launches.  FP64 equals the oracle's double-double CG
+    bit for bit."""
+    N = 84
+    ii = np.arange(N ** 3)
+    color = ((ii % N) + (ii // N) % N + ii // (N * N)) % 3
+    o = P.mc_order(P.grid(N, N, N, coef=(-1.0, 1.0)), color)
+    assert o.sched.nc == 3
+    M = I.IC0(o.A, 0.0, sched=o.sched)
+    assert M.plan()[0] == "mc-sliced-r"
+    b = I.gen_normal(o.A.n, 31)
+    x = np.zeros(o.A.n)
+    res = I.conjugate_gradient(o.A, b, x, M, 10000, 1e-8, o.sched)
+    Ao = P.orc(o.A)
+    dd = O.cg(Ao, b, "ic0", ic=O.ic0_factor(Ao), sched=P.orc_sched(o.sched), tol=1e-8,
+              dot_mode=1)
+    assert res.converged and (res.iters, res.rel_resid) == (dd.iters, dd.rel_resid)
+    assert np.array_equal(x, dd.x)
+    _assert_true_resid(Ao, x, b, res)
 
 
 def test_config3_192cube_abmc_mixed_and_fp64():
     """3D variable-coefficient 192^3 (n = 7,077,888), ABMC 4x4x4 boxes, FP64
-    and mixed precision, b ~ N(0,1), tol 1e-8 (true residual for mixed)."""
+    and mixed precision, b ~ N(0,1), tol 1e-8 (true residual for mixed),
+    against the reference's abmc_crs CG on the same permuted system."""
     N = 192
     A = P.grid(N, N, N, coef=(-2.0, 2.0), shift=1e-6)
     o = P.abmc_order(A, grid_dims=(N, N, N), box=(4, 4, 4))
@@ -75,9 +155,18 @@ def test_config3_192cube_abmc_mixed_and_fp64():
     M32 = I.IC0(dA, 0.0, sched=o.sched, precision="mixed")
     x32 = np.zeros(o.A.n)
     r32 = I.conjugate_gradient(dA, bp, x32, M32, 20000, 1e-8, o.sched)
+    del M32
     assert r64.converged and r32.converged and r32.true_rel_resid < 1e-8
     Ao = P.orc(o.A)
-    assert O.true_resid(Ao, x32, bp) < 1e-8
-    assert O.true_resid(Ao, x64, bp) < 1.5e-8
-    assert abs(r32.iters - r64.iters) <= max(3, r64.iters // 50)
-    assert np.linalg.norm(x32 - x64) / np.linalg.norm(x64) < 1e-6
+    _assert_true_resid(Ao, x64, bp, r64)
+    _assert_true_resid(Ao, x32, bp, r32)
+    assert r64.true_rel_resid < 1.5e-8
+    it, rel, xr = _reference_cg(o.A, bp, o.sched)
+    assert abs(r64.iters - it) <= 1
+    if r64.iters == it:
+        assert abs(r64.rel_resid - rel) <= 1e-12
+    e64, e32 = _err(x64, xr), _err(x32, xr)
+    print(f"[192^3 ABMC] reference iters {it} rel {rel:.6e}; gpu fp64 iters {r64.iters} "
+          f"rel {r64.rel_resid:.6e} err {e64:.3e}; mixed iters {r32.iters} "
+          f"true {r32.true_rel_resid:.6e} err {e32:.3e}")
+    assert e64 < 1e-6 and e32 < 1e-5
