@@ -19,6 +19,17 @@
  *     scipy permutations/tril, IC(0) == exact Cholesky on tree patterns,
  *     dense triangular solves, numpy.linalg.solve for tiny SPD systems.
  *   orc_dot_dd: against Python fractions (exact rational arithmetic).
+ *   orc_cg_mixed: its CG loop is orc_cg and its apply orc_ic0_apply_f32,
+ *     both pinned above; the restart rule has no reference counterpart (it
+ *     is the precision split's own) and is checked by its invariants: a
+ *     solve that converges with fewer than 100 restarts meets tol on the
+ *     true residual, no restart happens while the true residual meets tol,
+ *     and after each restart the iterates are those of orc_cg started from
+ *     the current x (tests/test_oracle.py).
+ *   orc_true_resid, orc_true_resid_dd: against exact rational sums of the
+ *     same d_i = fl(A x - b)_i (Python fractions), on cases with ||b|| far
+ *     from 1 and residuals far from 0, and against the value the reference
+ *     binaries print.
  */
 #include "iccg_oracle.h"
 
@@ -632,6 +643,22 @@ double orc_true_resid(int n, const int *rp, const int *ci, const double *v,
   return sqrt(nr) / sqrt(nb);
 }
 
+/* The same quotient with both sums accumulated in double-double and each
+ * rounded once to double (the form the GPU computes: r = b - A x, whose
+ * square equals (A x - b)^2).  ||b|| = 0 is taken as 1, as the solver's
+ * normb (cg_crs.cpp:109-110). */
+double orc_true_resid_dd(int n, const int *rp, const int *ci, const double *v,
+                         const double *x, const double *b) {
+  double *Ax = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  orc_spmv(n, rp, ci, v, x, Ax);
+  for (int i = 0; i < n; ++i) Ax[i] = Ax[i] - b[i];
+  const double nr = orc_dot_dd(n, Ax, Ax), nb2 = orc_dot_dd(n, b, b);
+  free(Ax);
+  double nb = sqrt(nb2);
+  if (nb == 0.0) nb = 1.0;
+  return sqrt(nr) / nb;
+}
+
 static void precond_apply(int n, int kind, const double *inv_diag,
                           const orc_ic0f *ic, const orc_sched *sched,
                           const double *r, double *z, double *y) {
@@ -639,8 +666,10 @@ static void precond_apply(int n, int kind, const double *inv_diag,
     memcpy(z, r, sizeof(double) * (size_t)n); /* precond.hpp:12-14 */
   } else if (kind == 1) {
     for (int i = 0; i < n; ++i) z[i] = inv_diag[i] * r[i]; /* 38-42 */
-  } else {
+  } else if (kind == 2) {
     orc_ic0_apply(ic, sched, r, z, y);
+  } else {
+    orc_ic0_apply_f32(ic, r, z); /* any valid MC / ABMC order: the same bits */
   }
 }
 
@@ -708,6 +737,33 @@ done:
   free(Ax);
   free(y);
   return iters;
+}
+
+/* Mixed precision: the FP32 IC(0) apply (orc_ic0_apply_f32) inside the FP64
+ * CG above with double-double dots, stopped on the FP64 true residual.  When
+ * the recurrence converges but ||b - A x|| / ||b|| (orc_true_resid_dd) is
+ * still >= tol, CG starts again from the current x (r = b - A x, z = M r,
+ * p = z), counting iterations on from where it stopped, at most 100 times;
+ * a breakdown or max_iter ends the solve.  A restart is orc_cg entered again
+ * with x0 = x, so the loop below is exactly that. */
+int orc_cg_mixed(int n, const int *rp, const int *ci, const double *v,
+                 const double *b, double *x, const orc_ic0f *ic, int max_iter,
+                 double tol, double *rel_out, int *converged_out,
+                 int *restarts_out, double *true_rel_out) {
+  int total = 0, restarts = 0, conv = 0;
+  double rel = 0.0;
+  for (;;) {
+    total += orc_cg(n, rp, ci, v, b, x, 3, NULL, ic, NULL, max_iter - total,
+                    tol, 1, &rel, &conv, NULL);
+    if (!conv || restarts >= 100) break;
+    if (orc_true_resid_dd(n, rp, ci, v, x, b) < tol || total >= max_iter) break;
+    ++restarts;
+  }
+  *rel_out = rel;
+  *converged_out = conv;
+  *restarts_out = restarts;
+  *true_rel_out = orc_true_resid_dd(n, rp, ci, v, x, b);
+  return total;
 }
 
 /* ------------------------------------------------------------------------ */

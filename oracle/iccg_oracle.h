@@ -121,9 +121,13 @@ void orc_spmv(int n, const int *rp, const int *ci, const double *v,
 /* ||A x - b|| / ||b|| as printed by main (cg_crs.cpp:517-528). */
 double orc_true_resid(int n, const int *rp, const int *ci, const double *v,
                       const double *x, const double *b);
+/* The same with double-double sums, each rounded once; ||b|| = 0 -> 1. */
+double orc_true_resid_dd(int n, const int *rp, const int *ci, const double *v,
+                         const double *x, const double *b);
 
 /* conjugate_gradient<Precond> (cg_crs.cpp:89-162).
- * kind: 0 Identity, 1 Jacobi (inv_diag), 2 IC0 (ic, sched).
+ * kind: 0 Identity, 1 Jacobi (inv_diag), 2 IC0 (ic, sched), 3 IC0 applied in
+ * FP32 (orc_ic0_apply_f32).
  * dot_mode: 0 = 80-bit accumulation (the reference), 1 = double-double.
  * hist (may be NULL, capacity max_iter+1) receives rel_resid after each
  * update, hist[0] = initial.  Returns iters. */
@@ -132,6 +136,15 @@ int orc_cg(int n, const int *rp, const int *ci, const double *v,
            const orc_ic0f *ic, const orc_sched *sched, int max_iter,
            double tol, int dot_mode, double *rel_out, int *converged_out,
            double *hist);
+
+/* Mixed precision CG: kind 3 of orc_cg (the FP32 apply orc_ic0_apply_f32,
+ * double-double dots) restarted from the current x while the FP64 true
+ * residual is >= tol after the recurrence converged (at most 100 restarts).
+ * Returns the total iterations. */
+int orc_cg_mixed(int n, const int *rp, const int *ci, const double *v,
+                 const double *b, double *x, const orc_ic0f *ic, int max_iter,
+                 double tol, double *rel_out, int *converged_out,
+                 int *restarts_out, double *true_rel_out);
 
 /* ---- schedule validators (cg_crs.cpp:171-336) --------------------------- */
 

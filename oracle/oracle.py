@@ -109,12 +109,16 @@ def lib():
         L.orc_dot_ld.restype = C.c_double
         L.orc_dot_dd.restype = C.c_double
         L.orc_true_resid.restype = C.c_double
+        L.orc_true_resid_dd.restype = C.c_double
         L.orc_check_interblock.restype = C.c_longlong
         L.orc_check_hazards.restype = C.c_longlong
         L.orc_ic0_factor.argtypes = [C.c_int, _vp, _vp, _vp, C.c_double] + [_vp] * 9
         L.orc_true_resid.argtypes = [C.c_int, _vp, _vp, _vp, _vp, _vp]
+        L.orc_true_resid_dd.argtypes = [C.c_int, _vp, _vp, _vp, _vp, _vp]
         L.orc_dot_ld.argtypes = [C.c_long, _vp, _vp]
         L.orc_dot_dd.argtypes = [C.c_long, _vp, _vp]
+        L.orc_cg_mixed.argtypes = [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_double,
+                                   _vp, _vp, _vp, _vp]
         L.orc_cg.argtypes = [C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp,
                              C.c_int, C.c_double, C.c_int, _vp, _vp, _vp]
     return _lib
@@ -308,8 +312,16 @@ def spmv(A: Csr, x):
 
 
 def true_resid(A: Csr, x, b):
+    """||A x - b|| / ||b|| summed as main() does (cg_crs.cpp:517-528)."""
     x, b = f64(x), f64(b)
     return lib().orc_true_resid(A.n, _ptr(A.rowptr), _ptr(A.colind), _ptr(A.val), _ptr(x), _ptr(b))
+
+
+def true_resid_dd(A: Csr, x, b):
+    """The same quotient with double-double sums (||b|| = 0 taken as 1)."""
+    x, b = f64(x), f64(b)
+    return lib().orc_true_resid_dd(A.n, _ptr(A.rowptr), _ptr(A.colind), _ptr(A.val), _ptr(x),
+                                   _ptr(b))
 
 
 @dataclass
@@ -323,12 +335,13 @@ class CGOut:
 
 def cg(A: Csr, b, kind, *, ic: IC0Factor | None = None, sched: Schedule | None = None,
        x0=None, max_iter=10000, tol=1e-8, dot_mode=0):
-    """conjugate_gradient<Precond> (cg_crs.cpp:89-162).  kind: 'none', 'jacobi', 'ic0'."""
-    k = {"none": 0, "jacobi": 1, "ic0": 2}[kind]
+    """conjugate_gradient<Precond> (cg_crs.cpp:89-162).  kind: 'none', 'jacobi', 'ic0',
+    'ic0_f32' (the IC(0) apply in FP32, orc_ic0_apply_f32)."""
+    k = {"none": 0, "jacobi": 1, "ic0": 2, "ic0_f32": 3}[kind]
     b = f64(b)
     x = np.zeros(A.n) if x0 is None else f64(x0).copy()
     inv = jacobi_inv_diag(A) if k == 1 else None
-    fs = _ic0f_struct(ic) if k == 2 else None
+    fs = _ic0f_struct(ic) if k >= 2 else None
     ss = sched.c_struct() if (sched is not None and k == 2) else None
     rel = C.c_double(0.0)
     conv = C.c_int(0)
@@ -338,6 +351,30 @@ def cg(A: Csr, b, kind, *, ic: IC0Factor | None = None, sched: Schedule | None =
                       C.byref(ss) if ss is not None else None, max_iter, tol, dot_mode,
                       C.byref(rel), C.byref(conv), _ptr(hist))
     return CGOut(it, rel.value, bool(conv.value), x, hist[: it + 1].copy())
+
+
+@dataclass
+class CGMixedOut:
+    iters: int
+    rel_resid: float
+    converged: bool
+    restarts: int
+    true_rel_resid: float
+    x: np.ndarray
+
+
+def cg_mixed(A: Csr, b, ic: IC0Factor, *, x0=None, max_iter=10000, tol=1e-8):
+    """FP64 CG with the FP32 IC(0) apply and double-double dots, stopped on
+    the FP64 true residual with restarts from the current x."""
+    b = f64(b)
+    x = np.zeros(A.n) if x0 is None else f64(x0).copy()
+    fs = _ic0f_struct(ic)
+    rel, tr = C.c_double(0.0), C.c_double(0.0)
+    conv, rs = C.c_int(0), C.c_int(0)
+    it = lib().orc_cg_mixed(A.n, _ptr(A.rowptr), _ptr(A.colind), _ptr(A.val), _ptr(b), _ptr(x),
+                            C.byref(fs), max_iter, tol, C.byref(rel), C.byref(conv),
+                            C.byref(rs), C.byref(tr))
+    return CGMixedOut(it, rel.value, bool(conv.value), rs.value, tr.value, x)
 
 
 # ---------------------------------------------------------------- validators

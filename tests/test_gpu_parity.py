@@ -124,6 +124,40 @@ def test_abmc_block_sizes(B):
     assert np.array_equal(M.apply(r, o.sched), O.ic0_apply(f, r, None))
 
 
+@pytest.mark.parametrize("B", [128, 512, 2047])
+@pytest.mark.parametrize("dims", [(24, 22, 20), (100, 100, 1)])
+def test_abmc_large_blocks(B, dims):
+    """Breadth-first blocks of more than 64 rows, the reference's protocol
+    sizes (run_abmc.sh:15: B from 32 to 2047), run one workgroup per block
+    with a level scan (the CSR plan; more rows than threads for B > 256):
+    the factor, the FP64 and FP32 applies, and FP64 and mixed CG, each
+    bitwise equal to the oracle."""
+    A = P.grid(*dims, coef=(-2.0, 2.0), shift=1e-6)
+    o = P.abmc_order(A, block=B)
+    assert int(np.diff(o.sched.block_ptr).max()) > 64
+    Ao = P.orc(o.A)
+    f = O.ic0_factor(Ao, 0.0)
+    M = I.IC0(o.A, 0.0, sched=o.sched)
+    assert M.plan()[0] == "csr"
+    _factor_equal(M, f)
+    r = I.gen_normal(A.n, 8)
+    assert np.array_equal(M.apply(r, o.sched), O.ic0_apply(f, r, None))
+    b = I.gen_normal(A.n, 9)
+    x = np.zeros(A.n)
+    res = I.conjugate_gradient(o.A, b, x, M, 10000, 1e-8, o.sched)
+    dd = O.cg(Ao, b, "ic0", ic=f, sched=P.orc_sched(o.sched), tol=1e-8, dot_mode=1)
+    assert res.converged and (res.iters, res.rel_resid) == (dd.iters, dd.rel_resid)
+    assert np.array_equal(x, dd.x)
+    _assert_true_resid(o.A, x, b, res)
+    M32 = I.IC0(o.A, 0.0, sched=o.sched, precision="mixed")
+    assert np.array_equal(M32.apply(r, o.sched), O.ic0_apply_f32(f, r))
+    x32 = np.zeros(A.n)
+    r32 = I.conjugate_gradient(o.A, b, x32, M32, 10000, 1e-8, o.sched)
+    m = O.cg_mixed(Ao, b, f, tol=1e-8)
+    assert (r32.iters, r32.restarts, r32.rel_resid) == (m.iters, m.restarts, m.rel_resid)
+    assert np.array_equal(x32, m.x)
+
+
 def test_abmc_grid_boxes():
     A = P.grid(16, 12, 8, coef=(-2.0, 2.0), shift=1e-6)
     o = P.abmc_order(A, grid_dims=(16, 12, 8), box=(4, 4, 4))
@@ -234,6 +268,19 @@ def _solve(A, kind, sched=None, b=None, tol=1e-8, max_iter=10000, precision="fp6
     return res, x
 
 
+def _assert_true_resid(A, x, b, res):
+    """The solver's true_rel_resid (k_resid + k_fin_resid) against the
+    oracle's ||A x - b|| / ||b||: within 2 ulps of the double-double form
+    (the GPU's SpMV is the oracle's bit for bit, and both sums are rounded
+    once), and within the recursive-summation bound 2 n u of the reference's
+    sequential form (cg_crs.cpp:517-528)."""
+    Ao = P.orc(A)
+    want = O.true_resid_dd(Ao, x, b)
+    assert abs(res.true_rel_resid - want) <= 4.5e-16 * want
+    seq = O.true_resid(Ao, x, b)
+    assert abs(res.true_rel_resid - seq) <= 2 * A.n * 1.12e-16 * seq
+
+
 def _oracle(A, kind, sched=None, b=None, tol=1e-8, max_iter=10000, dot_mode=0):
     if b is None:
         b = np.ones(A.n)
@@ -254,6 +301,7 @@ def test_cg_matches_oracle(kind, name):
     assert res.iters == dd.iters and res.converged == dd.converged
     assert res.rel_resid == dd.rel_resid
     assert np.array_equal(x, dd.x)
+    _assert_true_resid(A, x, np.ones(A.n), res)
     # against the reference's 80-bit dots: +-1 iteration
     assert abs(res.iters - ld.iters) <= 1
     if res.iters == ld.iters:
@@ -294,6 +342,7 @@ def test_cg_colored_matches_oracle(order):
     assert res.iters == dd.iters and np.array_equal(x, dd.x)
     assert abs(res.iters - ld.iters) <= 1
     assert res.converged
+    _assert_true_resid(o.A, x, b, res)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
@@ -315,6 +364,7 @@ def test_cg_sectioned_three_colors(precision):
     if precision == "fp64":
         dd = _oracle(A, "ic0", sched, b=b, dot_mode=1)
         assert res.iters == dd.iters and np.array_equal(x, dd.x)
+        _assert_true_resid(A, x, b, res)
     else:
         B = I.Batch(2)
         slot = B.add(A, M, b)
@@ -323,7 +373,8 @@ def test_cg_sectioned_three_colors(precision):
         assert fin and rb.converged
         assert (rb.iters, rb.restarts) == (res.iters, res.restarts)
         assert np.array_equal(B.x(slot), x)
-        assert O.true_resid(P.orc(A), x, b) < 1e-8
+        _assert_true_resid(A, x, b, res)
+        assert res.true_rel_resid < 1e-8
 
 
 def test_mixed_precision_true_residual():
@@ -333,9 +384,42 @@ def test_mixed_precision_true_residual():
     res, x = _solve(o.A, "ic0", o.sched, b=b, precision="mixed")
     assert res.converged
     assert res.true_rel_resid < 1e-8
-    assert O.true_resid(P.orc(o.A), x, b) < 1e-8
+    _assert_true_resid(o.A, x, b, res)
     ld = _oracle(o.A, "ic0", o.sched, b=b)
     assert np.linalg.norm(x - ld.x) / np.linalg.norm(ld.x) < 1e-5
+
+
+@pytest.mark.parametrize("order,dims,c,tol,restarts", [
+    ("abmcbox", (24, 20, 16), 2.0, 1e-8, 0), ("abmcbox", (24, 20, 16), 2.0, 1e-14, 1),
+    ("mc", (12, 12, 12), 3.0, 5e-15, 3), ("abmc8", (12, 12, 12), 3.0, 5e-15, 1),
+    ("natural", (40, 40, 1), 2.0, 3e-14, 1), ("abmcbox", (40, 40, 1), 2.0, 3e-14, 1)])
+def test_mixed_cg_matches_oracle(order, dims, c, tol, restarts):
+    """Mixed precision (FP32 factor and sweeps, FP64 CG, stop on the FP64 true
+    residual with restarts from the current x) against the oracle's
+    orc_cg_mixed bit for bit: iterations, restarts, recurrence residual and
+    x; the true residual within 2 ulps.  The tight tolerances make the
+    recurrence stop while the true residual still misses tol (1 to 3
+    restarts), on every plan: sectioned ABMC, plain ABMC, multicolor,
+    natural-order levels."""
+    A = P.grid(*dims, coef=(-c, c), shift=1e-6)
+    if order == "mc":
+        o = P.mc_order(A)
+    elif order == "abmc8":
+        o = P.abmc_order(A, block=8)
+    elif order == "abmcbox":
+        o = P.abmc_order(A, grid_dims=dims, box=(4, 4, 4) if dims[2] > 1 else (8, 8, 1))
+    else:
+        o = None
+    As, sched = (o.A, o.sched) if o else (A, None)
+    b = I.gen_normal(As.n, 4)
+    res, x = _solve(As, "ic0", sched, b=b, tol=tol, max_iter=3000, precision="mixed")
+    Ao = P.orc(As)
+    m = O.cg_mixed(Ao, b, O.ic0_factor(Ao), tol=tol, max_iter=3000)
+    assert m.restarts == restarts
+    assert (res.iters, res.restarts, res.converged) == (m.iters, m.restarts, m.converged)
+    assert res.rel_resid == m.rel_resid
+    assert np.array_equal(x, m.x)
+    _assert_true_resid(As, x, b, res)
 
 
 def test_solver_resident_repeat():

@@ -401,3 +401,101 @@ def test_spmv_and_cg_textbook():
         o = O.cg(Go, b, kind, ic=O.ic0_factor(Go) if kind == "ic0" else None, tol=1e-12)
         assert o.converged and np.allclose(o.x, xs, rtol=1e-9, atol=1e-11)
         assert O.true_resid(Go, o.x, b) < 1e-11
+
+
+# ------------------------------------------------------------------ true residual
+def _exact_true_resid(Ao, x, b):
+    """sqrt(sum d_i^2) / sqrt(sum b_i^2) for d = fl(A x - b), both sums exact
+    (rational), each rounded once, then the IEEE square roots and quotient."""
+    import math
+    d = O.spmv(Ao, x) - b
+    nr = sum(Fraction(float(v)) ** 2 for v in d)
+    nb = sum(Fraction(float(v)) ** 2 for v in b)
+    return math.sqrt(float(nr)) / math.sqrt(float(nb))
+
+
+@pytest.mark.parametrize("scale,noise", [(1e3, 1e-6), (1e-4, 1e-2), (1.0, 0.5)])
+def test_true_resid_exact_rational(scale, noise):
+    """||A x - b|| / ||b|| (cg_crs.cpp:517-528) against exact rational sums of
+    the same d_i, with ||b|| far from 1 and the residual far from 0, so a
+    missing division by ||b||, a missing square root or a wrong sign fails."""
+    G = P.grid(6, 5, 4, coef=(-1.0, 1.0))
+    Go = P.orc(G)
+    b = scale * P.I.gen_normal(G.n, 21)
+    xs = np.linalg.solve(Go.to_scipy().toarray(), b)
+    x = xs * (1.0 + noise * P.I.gen_normal(G.n, 22))
+    want = _exact_true_resid(Go, x, b)
+    assert 1e-9 < want < 10.0 and abs(np.linalg.norm(b) - 1.0) > 0.5
+    assert abs(O.true_resid_dd(Go, x, b) - want) <= 4.5e-16 * want
+    assert abs(O.true_resid(Go, x, b) - want) <= 1e-13 * want
+    # the same quotient from numpy norms of the same d, a formula of its own
+    d = O.spmv(Go, x) - b
+    assert abs(np.linalg.norm(d) / np.linalg.norm(b) - want) <= 1e-13 * want
+
+
+def test_true_resid_zero_rhs():
+    """||b|| = 0: the double-double form divides by 1, as the solver's normb
+    does (cg_crs.cpp:109-110)."""
+    G = P.grid(5, 4)
+    Go = P.orc(G)
+    x = P.I.gen_normal(G.n, 23)
+    assert O.true_resid_dd(Go, x, np.zeros(G.n)) == pytest.approx(
+        np.linalg.norm(O.spmv(Go, x)), rel=1e-15)
+
+
+@needs_ref
+@pytest.mark.parametrize("prog,tol,maxit", [("iccg_crs", "1e-14", "60"),
+                                            ("cg_crs", "1e-14", "200"),
+                                            ("iccg_crs", "1e-3", "10000")])
+def test_true_resid_matches_reference_binary(tmp_path, prog, tol, maxit):
+    """The reference binary's printed ||Ax-b||/||b|| (cg_crs.cpp:517-528) on
+    a 30x30 Laplacian with b = 1, x0 = 0: at tol 1e-14 it is the rounding
+    floor of A x - b (5.6e-14 against a recurrence residual of 4e-15), so it
+    pins the exact bits of x and of the SpMV as well as the quotient."""
+    import re
+    import subprocess
+    path = tmp_path / "g30.mtx"
+    scipy.io.mmwrite(str(path), P.grid(30, 30).to_scipy())
+    out = subprocess.run([R.bin_path(prog), str(path), tol, maxit], capture_output=True,
+                         text=True, check=True, env=dict(os.environ, OMP_NUM_THREADS="1")).stdout
+    m = re.search(r"iters=(\d+), rel_resid=(\S+), \|\|Ax-b\|\|/\|\|b\|\| = (\S+)", out)
+    A = O.read_matrix_market(str(path))
+    kind = "ic0" if prog == "iccg_crs" else "none"
+    o = O.cg(A, np.ones(A.n), kind, ic=O.ic0_factor(A) if kind == "ic0" else None,
+             tol=float(tol), max_iter=int(maxit))
+    assert o.iters == int(m.group(1)) and f"{o.rel_resid:g}" == m.group(2)
+    assert f"{O.true_resid(A, o.x, np.ones(A.n)):g}" == m.group(3)
+
+
+# ------------------------------------------------------------------ mixed precision
+@pytest.mark.parametrize("dims,c,tol,restarts", [((24, 20, 16), 2.0, 1e-8, 0),
+                                                 ((24, 20, 16), 2.0, 1e-14, 1),
+                                                 ((12, 12, 12), 3.0, 5e-15, 2),
+                                                 ((40, 40, 1), 2.0, 1e-14, 100)])
+def test_cg_mixed_restart_invariants(dims, c, tol, restarts):
+    """orc_cg_mixed: FP64 CG with the FP32 IC(0) apply, restarted from the
+    current x while the true residual misses tol.  Without a restart it is
+    orc_cg with the FP32 apply; each restart is orc_cg started again from the
+    x the previous pass left (an independent run, segment by segment); a
+    solve under the 100-restart cap meets tol on the true residual."""
+    G = P.grid(*dims, coef=(-c, c), shift=1e-6)
+    Go = P.orc(G)
+    f = O.ic0_factor(Go)
+    b = P.I.gen_normal(G.n, 4)
+    m = O.cg_mixed(Go, b, f, tol=tol, max_iter=3000)
+    assert m.converged and m.restarts == restarts
+    assert m.true_rel_resid == O.true_resid_dd(Go, m.x, b)
+    if restarts < 100:
+        assert m.true_rel_resid < tol
+    # segment by segment
+    x, total = None, 0
+    for k in range(restarts + 1):
+        o = O.cg(Go, b, "ic0_f32", ic=f, x0=x, tol=tol, max_iter=3000 - total, dot_mode=1)
+        total += o.iters
+        x = o.x
+        tr = O.true_resid_dd(Go, x, b)
+        assert (tr >= tol) == (k < restarts) or restarts == 100
+    assert total == m.iters and np.array_equal(x, m.x) and o.rel_resid == m.rel_resid
+    # the FP32 apply with FP64 dots and recurrence differs from FP64 IC(0)-CG
+    d = O.cg(Go, b, "ic0", ic=f, tol=tol, max_iter=3000, dot_mode=1)
+    assert not np.array_equal(d.x, m.x)
