@@ -279,6 +279,7 @@ struct iccg_server_s {
         while (!ready.empty() && free_slots >= ready.front()->nrhs) {
           take.push_back(ready.front());
           ready.pop_front();
+          take.back()->state = Q_RUN;  // under mu: read by wait() and result()
           free_slots -= take.back()->nrhs;
           running_reqs++;
         }
@@ -290,7 +291,6 @@ struct iccg_server_s {
       const auto t_admit = clk::now();
       // a request the batch does not take fails alone
       for (Request *r : take) {
-        r->state = Q_RUN;
         try {
           for (int k = 0; k < r->nrhs; ++k) {
             int slot = -1;
@@ -389,8 +389,11 @@ struct iccg_server_s {
       std::string err;
       try {
         prepare(*r);
-        r->t_ready = Clock::now();
-        r->state = Q_RUN;
+        {
+          std::lock_guard<std::mutex> lk(mu);  // state is read under mu by other threads
+          r->t_ready = Clock::now();
+          r->state = Q_RUN;
+        }
         iccg_solver_t S = nullptr;
         ck(iccg_solver_create(r->A, r->M, &S));
         struct SolverGuard {
