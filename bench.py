@@ -103,15 +103,16 @@ def kernel_table(S, n, nnz, nnz_l, mixed, nc, iters):
     plan = S.M.plan()[0] if isinstance(S.M, I.IC0) else "none"
     fused = plan in ("abmc-sectioned-fused", "abmc-sectioned-fused-r")
     # with the r update and r.z in the sweeps, their bytes move from the
-    # vector work (r update 24n, r.z 12n) to the sweeps, less the sweep's own
-    # read of r that the r update shares (8n)
+    # vector work (80n: r update 24n, x update 24n, r.z 8n with z in
+    # registers, p update 24n) to the sweeps: the r update's 16n beyond the
+    # sweep's own read of r, and r.z's read of r (8n); 48n stay
     ru = plan == "abmc-sectioned-fused-r"
-    # multicolor: the r update's 16n beyond the sweep's own read of r moves
-    # into the forward sweep, r.z's read of r (8n) into the backward one
+    # multicolor: the r update's 16n into the forward sweep, r.z's read of r
+    # (8n) into the backward one
     ru_mc = plan == "mc-sliced-r"
     cls = [("spmv_dots", [0], spmv_bytes(n, nnz), "one SpMV launch (k_spmv_cg)")]
     if fused:
-        cls.append(("sweeps", [1, 2], fwd + bwd + (28 * n if ru else 0),
+        cls.append(("sweeps", [1, 2], fwd + bwd + (24 * n if ru else 0),
                     f"one forward and one backward sweep ({2 * nc - 1} color launches: "
                     "the last color forward and backward in one"
                     + ("; the sweeps also apply r -= alpha Ap and form r.z)" if ru else ")")))
@@ -122,7 +123,7 @@ def kernel_table(S, n, nnz, nnz_l, mixed, nc, iters):
         cls.append(("bwd_sweep", [2], bwd + (8 * n if ru_mc else 0),
                     f"one backward sweep ({nc} color launches"
                     + ("; forming r.z)" if ru_mc else ")")))
-    cls.append(("vector_ops", [3], 44 * n if ru or ru_mc else 80 * n,
+    cls.append(("vector_ops", [3], 48 * n if ru or ru_mc else 80 * n,
                 "one iteration's vector and scalar work (alpha finish, "
                 + ("stop-test finish" if ru or ru_mc else "r update, r.z + stop-test finish")
                 + ", x and p update)"))
@@ -140,38 +141,89 @@ def kernel_table(S, n, nnz, nnz_l, mixed, nc, iters):
     return out
 
 
-def cpu_baseline(Ap, sched, b, iters_to_tol, budget_s):
+def physical_cores():
+    """Physical cores among the CPUs this process may run on: those CPUs
+    divided by the hardware threads per core (the first CPU's sibling list),
+    capped by OMP_NUM_THREADS when it is set."""
+    cpus = sorted(os.sched_getaffinity(0))
+    tpc = 1
+    try:
+        with open(f"/sys/devices/system/cpu/cpu{cpus[0]}/topology/thread_siblings_list") as f:
+            cnt = 0
+            for part in f.read().strip().split(","):
+                a, _, b = part.partition("-")
+                cnt += int(b) - int(a) + 1 if b else 1
+            tpc = max(1, cnt)
+    except (OSError, ValueError):
+        pass
+    n = max(1, len(cpus) // tpc)
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return min(n, cap) if cap > 0 else n
+
+
+def omp_protocol_env():
+    """The reference's OpenMP placement for the CPU baseline (spread over
+    physical cores), set before libgomp is loaded."""
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
+    os.environ.setdefault("OMP_PLACES", "cores")
+
+
+def cpu_baseline(Ap, sched, b, tol, max_iter, repeats, budget_s, label):
     """The reference's own abmc_crs CG (oracle/_ref, compiled from its
-    sources with OpenMP) on the same permuted system, timed over a bounded
-    number of iterations; time to tolerance extrapolated with the GPU FP64
-    iteration count (the same method, so the same count to within one)."""
+    sources with OpenMP) on the same permuted system and right-hand side,
+    run to tolerance on the host's physical cores (OMP_PROC_BIND=spread,
+    OMP_PLACES=cores), `repeats` times or until the budget is spent; the
+    CG phase is timed as main() times it (cg_crs.cpp:507-510), the median
+    reported.  Also returns the reference's x for the parity fields."""
     from oracle import ref as R
     from oracle.oracle import Csr, Schedule
     if not R.available():
-        return None
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+        return None, None
+    threads = physical_cores()
     R.set_threads(threads)
     A = Csr(Ap.n, Ap.rowptr, Ap.colind, Ap.val)
-    s = Schedule(nc=sched.nc, mode=1, nb=sched.nb, color_ptr_blk=sched.color_ptr_blk,
-                 blocks_of_color=sched.blocks_of_color, block_ptr=sched.block_ptr,
-                 rows_of_block=sched.rows_of_block)
+    if sched.mode == 1:
+        s = Schedule(nc=sched.nc, mode=1, nb=sched.nb, color_ptr_blk=sched.color_ptr_blk,
+                     blocks_of_color=sched.blocks_of_color, block_ptr=sched.block_ptr,
+                     rows_of_block=sched.rows_of_block)
+    else:
+        s = Schedule(nc=sched.nc, color_ptr=sched.color_ptr, rows_of_color=sched.rows_of_color,
+                     mode=0)
     t0 = time.time()
     R.ic0_factor_only(A, 0.0)
-    log(f"reference IC(0) factorization (serial): {time.time() - t0:.1f}s")
-    k0 = 3
-    it0, _, _, _, t_small = R.cg_timed_ic0(A, b, s, k0, tol=0.0, build_factor=False)
-    per = max(t_small / max(it0, 1), 1e-4)
-    k1 = int(min(max(budget_s / per, k0 + 5), 2000))
-    it1, _, _, _, t_big = R.cg_timed_ic0(A, b, s, k1, tol=0.0, build_factor=False)
-    per_iter = (t_big - t_small) / max(it1 - it0, 1)
-    log(f"reference CG: {it1} iterations in {t_big:.2f}s on {threads} threads "
-        f"({per_iter * 1e3:.2f} ms/iter)")
-    return {"value": round(per_iter * iters_to_tol, 3), "unit": "s (time to 1e-8, extrapolated)",
-            "cores": threads, "kind": "reference",
-            "sample": (f"{it1} iterations of the reference's abmc_crs IC(0)-CG (oracle/_ref, "
-                       f"OpenMP {threads} threads) on the same permuted 192^3 system; "
-                       f"{per_iter * 1e3:.2f} ms/iter x {iters_to_tol} iterations"),
-            "ms_per_iter": round(per_iter * 1e3, 3)}
+    log(f"reference IC(0) factorization (serial, untimed as in main()): {time.time() - t0:.1f}s")
+    runs, x_ref = [], None
+    t0 = time.time()
+    for k in range(max(1, repeats)):
+        it, rel, conv, x, sec = R.cg_timed_ic0(A, b, s, max_iter, tol=tol, build_factor=False)
+        runs.append((sec, it, rel, conv))
+        if x_ref is None:
+            x_ref = x
+        log(f"reference CG run {k + 1}: {it} iterations, rel {rel:.6e}, converged {conv}, "
+            f"{sec:.3f}s on {threads} threads")
+        if time.time() - t0 > budget_s:
+            break
+    secs = sorted(r[0] for r in runs)
+    med = secs[len(secs) // 2] if len(secs) % 2 else 0.5 * (secs[len(secs) // 2 - 1] +
+                                                             secs[len(secs) // 2])
+    it, rel, conv = runs[0][1], runs[0][2], runs[0][3]
+    out = {"value": round(med, 4), "unit": f"s (time to {tol:g}, CG phase)",
+           "cores": threads, "kind": "reference",
+           "sample": (f"the reference's {label} IC(0)-CG (oracle/_ref, OpenMP {threads} threads "
+                      f"on {threads} physical cores, OMP_PROC_BIND="
+                      f"{os.environ.get('OMP_PROC_BIND')}, OMP_PLACES="
+                      f"{os.environ.get('OMP_PLACES')}) on the same permuted system and b, run "
+                      f"to tolerance {len(runs)} times (median)"),
+           "runs_s": [round(r[0], 4) for r in runs], "iters": it, "rel_resid": rel,
+           "converged": conv}
+    return out, x_ref
+
+
+def refuse(msg):
+    """A timed solve that misses its tolerance (or disagrees with the
+    reference) gets no bench line."""
+    print(f"bench.py: no result: {msg}", file=sys.stderr, flush=True)
+    sys.exit(3)
 
 
 def mc_system(A, seed_b):
@@ -213,7 +265,7 @@ def cpu_baseline_systems(systems, tol, budget_s, label):
     from oracle.oracle import Csr, Schedule
     if not R.available():
         return None
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    threads = physical_cores()
     R.set_threads(threads)
     done, cg_s, t0 = 0, 0.0, time.time()
     for Ap, sched, b in systems:
@@ -222,6 +274,8 @@ def cpu_baseline_systems(systems, tol, budget_s, label):
                      mode=0)
         R.ic0_factor_only(A, 0.0)
         it, rel, conv, _, sec = R.cg_timed_ic0(A, b, s, 10000, tol=tol, build_factor=False)
+        if not conv:
+            refuse(f"the reference's CG did not converge on system {done}")
         cg_s += sec
         done += 1
         if time.time() - t0 > budget_s:
@@ -232,7 +286,9 @@ def cpu_baseline_systems(systems, tol, budget_s, label):
             "sample": ((f"the first {done} systems of the {label}, solved one after another"
                         if len(systems) > 1 else f"the {label}, solved once") +
                        f" by the reference's picg_crs IC(0)-CG (oracle/_ref, OpenMP {threads} "
-                       f"threads), FP64, tol {tol:g}; CG phase timed")}
+                       f"threads on {threads} physical cores, OMP_PROC_BIND="
+                       f"{os.environ.get('OMP_PROC_BIND')}, OMP_PLACES="
+                       f"{os.environ.get('OMP_PLACES')}), FP64, tol {tol:g}; CG phase timed")}
 
 
 def batch_kernel_table(B, systems, res, count, max_iter, tol, M32, dA):
@@ -309,6 +365,10 @@ def main_batch(args):
             for s in slots:
                 B.remove(s)
     wall = time.perf_counter() - t0
+    for r in res:
+        if not (r.converged and r.true_rel_resid < tol):
+            refuse(f"a batched system missed tol: {r.iters} iterations, true "
+                   f"{r.true_rel_resid:.3e}")
     xs = [B.x(s) for s in slots]
     chunk, cw = B.tuning
     kernels = batch_kernel_table(B, systems, res, count, max_iter, tol, M32, dA)
@@ -343,6 +403,8 @@ def main_batch(args):
         log("comparison legs skipped (--no-compare)")
         return
     sps64, res64, xs64 = one_at_a_time("fp64")
+    if not all(r.converged for r in res64):
+        refuse("a one-at-a-time FP64 solve did not converge")
     tr64 = np.array([r.true_rel_resid for r in res64])
     log(f"one-at-a-time FP64: {sps64:.1f} solves/s; max true rel {tr64.max():.2e}")
     sps32_1, res32_1, _ = one_at_a_time("mixed")
@@ -438,15 +500,20 @@ def main_mc128(args):
     S.set(b)
     res, wall, prof = time_solves(S, args.steps, args.warmup, args.max_iter, args.tol, False)
     ms = wall / args.steps * 1e3
+    check_solves("fp64", res, args.tol, true_resid=True)
     r = res[-1]
+    x = S.x()
     k = kernel_table(S, n, nnz, nnz_l, False, sched.nc, prof.iters)
     gbs = iter_bytes(n, nnz, nnz_l, False) * r.iters / (ms * 1e-3) / 1e9
     log(f"fp64: {r.iters} iters, rel {r.rel_resid:.3e}, true {r.true_rel_resid:.3e}, "
         f"{ms:.1f} ms/solve, {gbs:.0f} GB/s")
-    cpu = None
+    cpu, parity = None, None
     if not args.no_cpu:
-        cpu = cpu_baseline_systems([(Ap, sched, b)], args.tol, args.cpu_budget,
-                                   "128^3 red-black system")
+        cpu, x_ref = cpu_baseline(Ap, sched, b, args.tol, args.max_iter, 10, args.cpu_budget,
+                                  "picg_crs")
+        if cpu is not None:
+            parity = reference_parity(cpu, x_ref, r, x, x)
+            parity.pop("err_mixed_vs_reference")
     dom = max(k.items(), key=lambda kv: kv[1]["avg_ms"] * kv[1]["count"])
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_mc128_{dom[0]}.json")
@@ -468,7 +535,7 @@ def main_mc128(args):
                      "frac": dom[1]["frac"], "traffic": traffic,
                      "algorithmic_bytes_per_launch": dom[1]["bytes"],
                      "avg_ms": dom[1]["avg_ms"]},
-        "cpu_baseline": cpu,
+        "cpu_baseline": cpu, "reference_parity": parity,
     }
     print(json.dumps(line), flush=True)
 
@@ -573,7 +640,9 @@ def main():
     ap.add_argument("--max-iter", type=int, default=20000)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=30.0,
+                    help="seconds of reference CG runs to tolerance (at least one runs)")
+    ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--skip-fp64", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -581,6 +650,8 @@ def main():
               "(sharding by row slabs or by whole systems) is not run here", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload in ("abmc192", "mc128", "batch"):
+        omp_protocol_env()
     import iccg_amd as I
     if I.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
@@ -605,6 +676,7 @@ def main():
     res32, wall32, prof32 = time_solves(S32, args.steps, args.warmup, args.max_iter, args.tol,
                                         True)
     ms32 = wall32 / args.steps * 1e3
+    check_solves("mixed", res32, args.tol, true_resid=True)
     r32 = res32[-1]
     x32 = S32.x()
     k32 = kernel_table(S32, n, nnz, nnz_l, True, info["nc"], prof32.iters)
@@ -622,6 +694,7 @@ def main():
         res64, wall64, prof64 = time_solves(S64, args.steps, args.warmup, args.max_iter,
                                             args.tol, False)
         ms64 = wall64 / args.steps * 1e3
+        check_solves("fp64", res64, args.tol, true_resid=True)
         r64 = res64[-1]
         x64 = S64.x()
         k64 = kernel_table(S64, n, nnz, nnz_l, False, info["nc"], prof64.iters)
@@ -641,8 +714,10 @@ def main():
 
     bytes_it = iter_bytes(n, nnz, nnz_l, True)
     gbs32 = bytes_it * r32.iters / (ms32 * 1e-3) / 1e9
-    # dominant kernel class of the headline solve
-    dom = max(k32.items(), key=lambda kv: kv[1]["avg_ms"] * kv[1]["count"])
+    # the dominant single kernel of the headline solve: the one launch with
+    # the largest share of time (the sweeps class is three launches of
+    # different kernels, each shorter than the SpMV)
+    dom = ("spmv_dots", k32["spmv_dots"])
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{dom[0]}.json")
     if os.path.exists(tpath):
@@ -665,10 +740,13 @@ def main():
         roofline["kernel_layout_bytes_per_launch"] = int(lay)
         roofline["kernel_layout_gbs"] = round(lay / (dom[1]["avg_ms"] * 1e-3) / 1e9, 1)
 
-    cpu = None
+    cpu, parity = None, None
     if not args.no_cpu:
-        iters_ref = fp64["iters"] if fp64 else r32.iters
-        cpu = cpu_baseline(Ap, sched, b, iters_ref, args.cpu_budget)
+        cpu, x_ref = cpu_baseline(Ap, sched, b, args.tol, args.max_iter, args.cpu_repeats,
+                                  args.cpu_budget, "abmc_crs")
+        if cpu is not None:
+            parity = reference_parity(cpu, x_ref, r64 if fp64 else None,
+                                      x64 if fp64 else None, x32)
 
     line = {
         "metric": METRIC, "value": round(ms32 / 1e3, 4), "unit": "s",
@@ -687,9 +765,46 @@ def main():
         "speedup_mixed_vs_fp64": round(fp64["time_s"] / (ms32 / 1e3), 3) if fp64 else None,
         "err_vs_fp64": (float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))
                         if fp64 else None),
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "reference_parity": parity,
     }
     print(json.dumps(line), flush=True)
+
+
+def check_solves(label, results, tol, true_resid):
+    """Every timed solve converged, met tol (on the FP64 true residual when
+    true_resid) and took the same number of iterations; otherwise no line."""
+    for r in results:
+        if not r.converged or r.breakdown:
+            refuse(f"{label} solve did not converge ({r.iters} iterations, rel {r.rel_resid:.3e})")
+        if not (r.rel_resid < tol) or (true_resid and not (r.true_rel_resid < tol)):
+            refuse(f"{label} solve missed tol {tol:g}: rel {r.rel_resid:.3e}, true "
+                   f"{r.true_rel_resid:.3e}")
+    if len({r.iters for r in results}) != 1:
+        refuse(f"{label} solves disagree: iterations {[r.iters for r in results]}")
+
+
+def reference_parity(cpu, x_ref, r64, x64, x32):
+    """The GPU solves against the reference's CG run to tolerance on the
+    same system: iterations (FP64 within one, or no line), final residual
+    and relative errors of x."""
+    if not cpu["converged"]:
+        refuse("the reference's CG did not converge")
+    nr = float(np.linalg.norm(x_ref))
+    out = {"reference_iters": cpu["iters"], "reference_rel_resid": cpu["rel_resid"],
+           "err_mixed_vs_reference": float(np.linalg.norm(x32 - x_ref) / nr)}
+    if r64 is not None:
+        d = r64.iters - cpu["iters"]
+        out.update({"fp64_iters": r64.iters, "fp64_rel_resid": r64.rel_resid,
+                    "fp64_iters_minus_reference": d,
+                    "fp64_rel_resid_minus_reference": r64.rel_resid - cpu["rel_resid"],
+                    "err_fp64_vs_reference": float(np.linalg.norm(x64 - x_ref) / nr)})
+        log(f"reference parity: iterations {r64.iters} vs {cpu['iters']}, rel "
+            f"{r64.rel_resid:.9e} vs {cpu['rel_resid']:.9e}, ||x64-x_ref||/||x_ref|| "
+            f"{out['err_fp64_vs_reference']:.3e}, mixed {out['err_mixed_vs_reference']:.3e}")
+        if abs(d) > 1:
+            refuse(f"FP64 iterations {r64.iters} differ from the reference's {cpu['iters']} by "
+                   "more than one")
+    return out
 
 
 if __name__ == "__main__":
