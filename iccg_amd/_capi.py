@@ -101,6 +101,7 @@ _SIGS = {
     "iccg_precond_get_factor": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "iccg_precond_get_plan": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
     "iccg_cg_solve": (_i, [_vp, _vp, _vp, _vp, _i, _d, _i, C.POINTER(CGResultC)]),
+    "iccg_cg_solve_noalloc": (_i, [_vp, _vp, _vp, _vp, _i, _d, _i, C.POINTER(CGResultC)]),
     "iccg_solver_create": (_i, [_vp, _vp, C.POINTER(_vp)]),
     "iccg_solver_destroy": (_i, [_vp]),
     "iccg_solver_set": (_i, [_vp, _vp, _vp]),

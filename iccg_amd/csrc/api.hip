@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1200,6 +1201,126 @@ int iccg_cg_solve(iccg_csr_t A, iccg_precond_t M, const double *b, double *x, in
   if (st == ICCG_OK) st = iccg_solver_get_x(S, x);
   iccg_solver_destroy(S);
   return st;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The CG solve without allocation (iccg_cg_solve_noalloc): every device
+// buffer of the solve (x, r, p, Ap, b, y, z, the partials and the scalars)
+// is carved from `scratch`, one file-scope array per device, and the
+// iterations are launched one by one (instantiating a captured graph
+// allocates).  In host code `scratch` names a host-side stand-in, so its
+// device address comes from hipGetSymbolAddress, once per device, and only
+// that address reaches kernels and copies.
+// ---------------------------------------------------------------------------
+namespace iccg {
+
+constexpr size_t kScratchBytes = size_t(64) << 20;  // 64 MiB per device
+constexpr int kMaxScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static std::mutex g_scratch_mu;
+static char *g_scratch_dev[kMaxScratchDevices];  // device address, per device
+static Scal g_scratch_scal[kMaxScratchDevices];  // host mirror of the scalars, per device
+
+// the device address of this device's scratch array (the device is current)
+static char *scratch_base(int dev) {
+  require(dev >= 0 && dev < kMaxScratchDevices, ICCG_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    require(p != nullptr, ICCG_ERR_DEVICE, "no device address for the scratch array");
+    g_scratch_dev[dev] = static_cast<char *>(p);
+  }
+  return g_scratch_dev[dev];
+}
+
+static size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// bytes of scratch a solve of A with M takes
+static size_t noalloc_bytes(const iccg_csr_s &A, const iccg_precond_s &M, int slots) {
+  const size_t n = (size_t)A.d.n;
+  const size_t vb = M.mixed ? sizeof(float) : sizeof(double);
+  size_t need = 5 * round256(sizeof(double) * n);                   // x, r, p, Ap, b
+  if (M.kind == ICCG_PRECOND_IC0) need += round256(vb * n);         // y
+  if (M.kind != ICCG_PRECOND_IDENTITY) need += round256(vb * n);    // z
+  need += round256(sizeof(double) * 2 * (size_t)slots) + round256(sizeof(Scal));
+  return need;
+}
+
+}  // namespace iccg
+
+extern "C" {
+
+int iccg_cg_solve_noalloc(iccg_csr_t A, iccg_precond_t M, const double *b, double *x,
+                          int max_iter, double tol, int flags, iccg_cg_result *res) {
+  return guard([&] {
+    require(A && M && b && x && res, ICCG_ERR_ARG, "null argument");
+    require(M->n == A->d.n, ICCG_ERR_ARG, "preconditioner size does not match the matrix");
+    require(M->device == A->device, ICCG_ERR_ARG, "matrix and preconditioner on other devices");
+    HIPCHK(hipSetDevice(A->device));
+    char *base = scratch_base(A->device);
+    const int slots = std::max(kMaxRedBlocks * 3, M->kind == ICCG_PRECOND_IC0
+                                                       ? bwd_partial_slots(M->sc, M->F)
+                                                       : 0);
+    const size_t need = noalloc_bytes(*A, *M, slots);
+    require(need <= kScratchBytes, ICCG_ERR_ARG,
+            "iccg_cg_solve_noalloc: this system needs " + std::to_string(need) +
+                " bytes of device scratch, more than the " + std::to_string(kScratchBytes) +
+                " the library holds per device (about 1.1M rows in FP64, 1.3M mixed)");
+    const int n = A->d.n;
+    const size_t vb = M->mixed ? sizeof(float) : sizeof(double);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+      char *p = base + off;
+      off += round256(bytes);
+      return p;
+    };
+    // a solver over the scratch memory, launched kernel by kernel on the
+    // matrix's stream; the guard hands every pointer back before the
+    // solver's destructor could free it
+    iccg_solver_s S;
+    struct Release {
+      iccg_solver_s &s;
+      ~Release() {
+        s.x = s.r = s.p = s.Ap = s.b = s.part = nullptr;
+        s.y = s.z = nullptr;
+        s.S = s.hS = nullptr;
+        s.st = nullptr;
+      }
+    } release{S};
+    S.device = A->device;
+    S.A = A;
+    S.M = M;
+    S.n = n;
+    S.st = A->st;
+    S.graphs = false;
+    S.x = reinterpret_cast<double *>(take(sizeof(double) * (size_t)n));
+    S.r = reinterpret_cast<double *>(take(sizeof(double) * (size_t)n));
+    S.p = reinterpret_cast<double *>(take(sizeof(double) * (size_t)n));
+    S.Ap = reinterpret_cast<double *>(take(sizeof(double) * (size_t)n));
+    S.b = reinterpret_cast<double *>(take(sizeof(double) * (size_t)n));
+    if (M->kind == ICCG_PRECOND_IC0) S.y = take(vb * (size_t)n);
+    if (M->kind != ICCG_PRECOND_IDENTITY) S.z = take(vb * (size_t)n);
+    S.part = reinterpret_cast<double *>(take(sizeof(double) * 2 * (size_t)slots));
+    S.part_slots = slots;
+    S.S = reinterpret_cast<Scal *>(take(sizeof(Scal)));
+    S.hS = &g_scratch_scal[A->device];
+    // the scalars start from zero (the last call's values stay in the
+    // array), then b and x0 go up, as iccg_cg_solve does
+    std::memset(S.hS, 0, sizeof(Scal));
+    const size_t bytes = sizeof(double) * (size_t)n;
+    HIPCHK(hipMemcpyAsync(S.S, S.hS, sizeof(Scal), hipMemcpyHostToDevice, S.st));
+    HIPCHK(hipMemcpyAsync(S.b, b, bytes, hipMemcpyHostToDevice, S.st));
+    HIPCHK(hipMemcpyAsync(S.x, x, bytes, hipMemcpyHostToDevice, S.st));
+    HIPCHK(hipStreamSynchronize(S.st));
+    std::memset(res, 0, sizeof(*res));
+    res->true_rel_resid = -1.0;
+    run_solver(S, max_iter, tol, flags, *res);
+    HIPCHK(hipMemcpyAsync(x, S.x, bytes, hipMemcpyDeviceToHost, S.st));
+    HIPCHK(hipStreamSynchronize(S.st));
+  });
 }
 
 int iccg_bench_copy(long nbytes, int reps, double *avg_ms) {

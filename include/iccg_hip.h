@@ -253,6 +253,21 @@ int iccg_precond_get_factor(iccg_precond_t M, int *Lrp, int *Lci, double *Lv,
 int iccg_cg_solve(iccg_csr_t A, iccg_precond_t M, const double *b, double *x,
                   int max_iter, double tol, int flags, iccg_cg_result *res);
 
+/* iccg_cg_solve without any allocation.  It has no counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code
+ * running while other work on the device is under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Every device buffer of
+ * the solve (x, r, p, Ap, b, y, z, partials, scalars) is carved from one
+ * 64 MiB array per device, which each device the library runs on holds
+ * until the process ends, and the iterations are launched kernel by kernel
+ * on the matrix's stream (no captured graphs).  Arguments, results and
+ * errors are those of iccg_cg_solve; a system that needs more than 64 MiB
+ * of scratch (about 56 bytes per row in FP64 with IC(0), 48 mixed, plus
+ * 100 KB: some 1.1M rows) is rejected with ICCG_ERR_ARG.  The array keeps
+ * the last call's contents; calls that overlap on one device race on it. */
+int iccg_cg_solve_noalloc(iccg_csr_t A, iccg_precond_t M, const double *b, double *x,
+                          int max_iter, double tol, int flags, iccg_cg_result *res);
+
 /* ---- device-resident solver (timing, repeated solves) --------------------- */
 int iccg_solver_create(iccg_csr_t A, iccg_precond_t M, iccg_solver_t *out);
 int iccg_solver_destroy(iccg_solver_t S);
