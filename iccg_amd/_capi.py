@@ -18,7 +18,7 @@ ICCG_ERR_DEVICE, ICCG_ERR_NOMEM, ICCG_ERR_STATE = 5, 6, 7
 PRECOND_IDENTITY, PRECOND_JACOBI, PRECOND_IC0 = 0, 1, 2
 PLAN_NAMES = {0: "none", 1: "levels", 2: "csr", 3: "mc-sliced", 4: "abmc-sliced",
               5: "abmc-sectioned", 6: "abmc-sectioned-fused", 7: "abmc-sectioned-fused-r",
-              8: "mc-sliced-r"}
+              8: "mc-sliced-r", 9: "abmc-blocks"}
 FP64, MIXED = 0, 1
 SCHED_MC, SCHED_ABMC = 0, 1
 SOLVE_TRUE_RESID = 1

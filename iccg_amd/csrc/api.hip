@@ -1007,7 +1007,9 @@ int iccg_precond_get_plan(iccg_precond_t M, int *plan, int *section_width) {
     if (M->kind != ICCG_PRECOND_IC0) return;
     const DevSched &sc = M->sc;
     if (!sliced_plan(sc, M->F))
-      *plan = sc.kind == 0 ? ICCG_PLAN_LEVELS : ICCG_PLAN_CSR;
+      *plan = sc.kind == 0            ? ICCG_PLAN_LEVELS
+              : block_plan(sc, M->F) ? ICCG_PLAN_ABMC_BLOCKS
+                                     : ICCG_PLAN_CSR;
     else if (sc.kind == 1)
       *plan = fwd_updates_r(sc, M->F) ? ICCG_PLAN_MC_SLICED_R : ICCG_PLAN_MC_SLICED;
     else if (M->F.sec > 0) {

@@ -19,6 +19,7 @@
 namespace iccg {
 
 constexpr int kWG = 256;          // threads per workgroup of streaming kernels
+constexpr int kBlockPlanMax = 2048;  // largest ABMC block of the one-workgroup-per-block sweeps
 constexpr int kMaxRedBlocks = 2048;  // workgroups of reducing kernels (8 per CU)
 // MI355X's Infinity Cache (the memory-side cache in front of HBM): 256 MiB
 constexpr double kInfinityCacheBytes = 256.0 * 1024 * 1024;
@@ -263,6 +264,7 @@ void launch_bwd(const DevSched &sc, const Factor &F, bool mixed, const void *y, 
 bool sliced_plan(const DevSched &sc, const Factor &F);
 bool bwd_fuses_rz(const DevSched &sc, const Factor &F);
 int bwd_partial_slots(const DevSched &sc, const Factor &F);
+bool block_plan(const DevSched &sc, const Factor &F);
 void launch_scatter_sell(const Factor &F, bool mixed, hipStream_t st);
 void launch_ic0_factor(const DevSched &sc, const Factor &F, double shift,
                        unsigned long long *err, hipStream_t st);

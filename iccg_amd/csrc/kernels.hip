@@ -593,6 +593,98 @@ __global__ void k_abmc(const int *__restrict__ bstart, const int *__restrict__ b
 }
 
 // ---------------------------------------------------------------------------
+// ABMC sweeps for blocks of 65 to 2048 rows (breadth-first blockings, the
+// reference's block sizes up to 2047) on the plain sliced factor: one
+// workgroup per block, thread t owning rows bs + t + q T (q < RPL).  Each
+// row's entries, diagonal and the terms that do not depend on the block's
+// own results are loaded once, before the in-block levels run: the forward
+// sweep (ic0.cpp:263-300) folds the earlier colors' terms (the columns below
+// the block, first in ascending order) into acc = r_i; the backward sweep
+// (ic0.cpp:302-339) gathers the later colors' values (the columns past the
+// block, last in ascending order).  Level l then applies, for the rows of
+// in-block level l, the in-block terms from the block's LDS window (rows of
+// lower levels, final), and in the backward sweep the later colors' terms,
+// divides by L_ii and writes the window; a barrier separates the levels.
+// ---------------------------------------------------------------------------
+template <class VT, class YT, int WM, int RPL, bool BWD, bool NT>
+__global__ __launch_bounds__(1024) void k_abmc_wg(int kb0, const int *__restrict__ bstart,
+                                                  const uint16_t *__restrict__ lev,
+                                                  const int *__restrict__ nlev, Sell L,
+                                                  const VT *__restrict__ lv,
+                                                  const VT *__restrict__ diag,
+                                                  const double *__restrict__ r,
+                                                  const YT *__restrict__ yin,
+                                                  YT *__restrict__ out, const Scal *S) {
+  extern __shared__ double wg_win[];  // the block's rows, final once written
+  YT *win = reinterpret_cast<YT *>(wg_win);
+  if (S && S->done) return;
+  const int kb = kb0 + blockIdx.x;
+  const int bs = bstart[kb], be = bstart[kb + 1];
+  const int nl = nlev[kb];
+  const int T = blockDim.x;
+  int row[RPL], rl[RPL];
+  YT acc[RPL], dd[RPL];
+  int cc[RPL][WM];
+  VT vv[RPL][WM];
+  YT xo[RPL][WM];
+#pragma unroll
+  for (int q = 0; q < RPL; ++q) {
+    const int i = bs + (int)threadIdx.x + q * T;
+    row[q] = i < be ? i : -1;
+    const int ii = i < be ? i : bs;  // lanes past the block load its first row
+    const int o = L.off[ii >> 6];
+    const int w = (L.off[(ii >> 6) + 1] - o) >> 6;
+    const int base = o + (ii & 63);
+#pragma unroll
+    for (int k = 0; k < WM; ++k) {
+      cc[q][k] = -1;
+      vv[q][k] = (VT)0;
+      if (k < w) {
+        cc[q][k] = ldm<NT>(&L.col[base + 64 * k]);
+        vv[q][k] = ldm<NT>(&lv[base + 64 * k]);
+      }
+    }
+    dd[q] = (YT)diag[ii];
+    rl[q] = (int)lev[ii];
+    acc[q] = BWD ? yin[ii] : (YT)r[ii];
+#pragma unroll
+    for (int k = 0; k < WM; ++k) {
+      const int c = cc[q][k];
+      const bool outside = BWD ? c >= be : (c >= 0 && c < bs);
+      const YT x = out[outside ? c : ii];
+      if (BWD) {
+        xo[q][k] = x;
+      } else {
+        const YT t = fma_t(-(YT)vv[q][k], x, acc[q]);
+        acc[q] = outside ? t : acc[q];
+      }
+    }
+  }
+  for (int l = 0; l < nl; ++l) {
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      if (row[q] >= 0 && rl[q] == l) {
+        YT a = acc[q];
+#pragma unroll
+        for (int k = 0; k < WM; ++k) {
+          const int c = cc[q][k];
+          if (c >= bs && c < be) a = fma_t(-(YT)vv[q][k], win[c - bs], a);
+        }
+        if (BWD) {
+#pragma unroll
+          for (int k = 0; k < WM; ++k)
+            if (cc[q][k] >= be) a = fma_t(-(YT)vv[q][k], xo[q][k], a);
+        }
+        const YT y = a / dd[q];
+        win[row[q] - bs] = y;
+        out[row[q]] = y;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Sliced-ELL multicolor sweeps (ic0.cpp:194-249): rows [c0, c1) of one
 // color, one wavefront per 64-row slice.  Every dependency of a row lies in
 // an earlier (forward) / later (backward) color, finished by a previous
@@ -1438,6 +1530,52 @@ static void bwd_sec(const DevSched &sc, const Factor &F, const VT *uv, const VT 
     else FN<VT, YT, 16, false>(__VA_ARGS__);                     \
   } while (0)
 
+// blocks of 65 to 2048 rows on the plain sliced factor (k_abmc_wg)
+bool block_plan(const DevSched &sc, const Factor &F) {
+  return F.sell && F.sec == 0 && std::max(F.ls.maxw, F.us.maxw) <= 16 && sc.kind == 2 &&
+         sc.d_brows == nullptr && sc.max_block > 64 && sc.max_block <= kBlockPlanMax;
+}
+
+// one launch per color, forward in color order, backward in reverse
+template <class VT, class YT, int WM, int RPL, bool BWD, bool NT>
+static void wg_sweep(const DevSched &sc, const Sell &L, const VT *lv, const VT *dg,
+                     const double *r, const YT *yin, YT *out, const Scal *S, hipStream_t st) {
+  const int per = (sc.max_block + RPL - 1) / RPL;
+  const int T = std::min(1024, (per + 63) / 64 * 64);
+  const size_t lds = sizeof(YT) * (size_t)sc.max_block;
+  for (int t = 0; t < sc.nc; ++t) {
+    const int c = BWD ? sc.nc - 1 - t : t;
+    const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
+    if (b1 <= b0) continue;
+    hipLaunchKernelGGL((k_abmc_wg<VT, YT, WM, RPL, BWD, NT>), dim3(b1 - b0), dim3(T), lds, st,
+                       b0, sc.d_bstart, BWD ? sc.d_lev_b : sc.d_lev_f,
+                       BWD ? sc.d_nlev_b : sc.d_nlev_f, L, lv, dg, r, yin, out, S);
+  }
+}
+
+template <class VT, class YT, bool BWD>
+static void wg_dispatch(const DevSched &sc, const Factor &F, const Sell &L, const VT *lv,
+                        const VT *dg, const double *r, const YT *yin, YT *out, const Scal *S,
+                        hipStream_t st) {
+  const int w = std::max(F.ls.maxw, F.us.maxw);
+  const bool two = sc.max_block > 1024;
+#define WG_ONE(WMV, RPLV)                                                                  \
+  do {                                                                                     \
+    if (F.nt)                                                                              \
+      wg_sweep<VT, YT, WMV, RPLV, BWD, true>(sc, L, lv, dg, r, yin, out, S, st);           \
+    else                                                                                   \
+      wg_sweep<VT, YT, WMV, RPLV, BWD, false>(sc, L, lv, dg, r, yin, out, S, st);          \
+  } while (0)
+  if (w <= 4) {
+    if (two) WG_ONE(4, 2); else WG_ONE(4, 1);
+  } else if (w <= 8) {
+    if (two) WG_ONE(8, 2); else WG_ONE(8, 1);
+  } else {
+    if (two) WG_ONE(16, 2); else WG_ONE(16, 1);
+  }
+#undef WG_ONE
+}
+
 bool sweeps_fused(const DevSched &sc, const Factor &F) {
   return sliced_plan(sc, F) && sc.kind == 2 && F.sec > 0 && F.fb && !fuse_rz_enabled();
 }
@@ -1468,6 +1606,15 @@ void launch_fwd(const DevSched &sc, const Factor &F, bool mixed, const double *r
     else
       DISPATCH_SWEEP_WM(wm, F.nt, fwd_sliced, double, double, sc, F, F.ls.v64, F.d64, r, (double *)y,
                         S, st, ap, xp, ru_count);
+    return;
+  }
+  if (block_plan(sc, F)) {
+    if (mixed)
+      wg_dispatch<float, float, false>(sc, F, F.ls, F.ls.v32, F.d32, r, nullptr, (float *)y, S,
+                                       st);
+    else
+      wg_dispatch<double, double, false>(sc, F, F.ls, F.ls.v64, F.d64, r, nullptr, (double *)y,
+                                         S, st);
     return;
   }
   if (mixed) {
@@ -1511,6 +1658,15 @@ void launch_bwd(const DevSched &sc, const Factor &F, bool mixed, const void *y, 
     else
       DISPATCH_SWEEP_WM(wm, F.nt, bwd_sliced, double, double, sc, F, F.us.v64, F.d64,
                         (const double *)y, (double *)z, r, Sw, part, rz, st, slot0);
+    return;
+  }
+  if (block_plan(sc, F)) {
+    if (mixed)
+      wg_dispatch<float, float, true>(sc, F, F.us, F.us.v32, F.d32, nullptr, (const float *)y,
+                                      (float *)z, S, st);
+    else
+      wg_dispatch<double, double, true>(sc, F, F.us, F.us.v64, F.d64, nullptr,
+                                        (const double *)y, (double *)z, S, st);
     return;
   }
   if (mixed) {

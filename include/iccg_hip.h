@@ -234,10 +234,14 @@ enum {
   ICCG_PLAN_ABMC_SECTIONED_FUSED_R = 7, /* and, in the solver's iterations,
                                            the sweeps apply the CG's r update
                                            and form r.z */
-  ICCG_PLAN_MC_SLICED_R = 8            /* multicolor, sliced ELL, and, in
+  ICCG_PLAN_MC_SLICED_R = 8,           /* multicolor, sliced ELL, and, in
                                           the solver's iterations, the
                                           forward sweep applies the r update
                                           and the backward sweep forms r.z */
+  ICCG_PLAN_ABMC_BLOCKS = 9            /* ABMC blocks of 65 to 2048 rows:
+                                          one workgroup per block, rows in
+                                          registers, a barrier per in-block
+                                          level */
 };
 int iccg_precond_get_plan(iccg_precond_t M, int *plan, int *section_width);
 /* apply(r, z, sched) (precond.hpp:12,38; ic0.hpp:34) on host buffers. */

@@ -128,17 +128,16 @@ def test_abmc_block_sizes(B):
 @pytest.mark.parametrize("dims", [(24, 22, 20), (100, 100, 1)])
 def test_abmc_large_blocks(B, dims):
     """Breadth-first blocks of more than 64 rows, the reference's protocol
-    sizes (run_abmc.sh:15: B from 32 to 2047), run one workgroup per block
-    with a level scan (the CSR plan; more rows than threads for B > 256):
-    the factor, the FP64 and FP32 applies, and FP64 and mixed CG, each
-    bitwise equal to the oracle."""
+    sizes (run_abmc.sh:15: B from 32 to 2047), one workgroup per block (two
+    rows per thread past 1024): the factor, the FP64 and FP32 applies, and
+    FP64 and mixed CG, each bitwise equal to the oracle."""
     A = P.grid(*dims, coef=(-2.0, 2.0), shift=1e-6)
     o = P.abmc_order(A, block=B)
     assert int(np.diff(o.sched.block_ptr).max()) > 64
     Ao = P.orc(o.A)
     f = O.ic0_factor(Ao, 0.0)
     M = I.IC0(o.A, 0.0, sched=o.sched)
-    assert M.plan()[0] == "csr"
+    assert M.plan()[0] == "abmc-blocks"
     _factor_equal(M, f)
     r = I.gen_normal(A.n, 8)
     assert np.array_equal(M.apply(r, o.sched), O.ic0_apply(f, r, None))
@@ -243,11 +242,13 @@ def test_schedule_switch_lays_out_factor_again(precision):
     f = O.ic0_factor(P.orc(A), 0.0)
     ref = O.ic0_apply if precision == "fp64" else O.ic0_apply_f32
     two, four = _slab_schedule(A.n, 2), _slab_schedule(A.n, 4)
+    one = _slab_schedule(A.n, 1)  # one block of 128 rows: a workgroup per block
     M = I.IC0(A, 0.0, sched=two, precision=precision)
     r = I.gen_normal(A.n, 12)
     expect = ref(f, r)
     for sched, plan in [(two, ("abmc-sectioned-fused-r", 3)), (four, ("abmc-sliced", 0)),
-                        (two, ("abmc-sectioned-fused-r", 3)), (None, ("levels", 0)),
+                        (one, ("abmc-blocks", 0)), (two, ("abmc-sectioned-fused-r", 3)),
+                        (one, ("abmc-blocks", 0)), (None, ("levels", 0)),
                         (four, ("abmc-sliced", 0))]:
         z = M.apply(r, sched)
         assert M.plan() == plan
