@@ -141,10 +141,18 @@ def kernel_table(S, n, nnz, nnz_l, mixed, nc, iters):
     return out
 
 
+_CORES = None  # physical cores, counted at start-up
+
+
 def physical_cores():
     """Physical cores among the CPUs this process may run on: those CPUs
     divided by the hardware threads per core (the first CPU's sibling list),
-    capped by OMP_NUM_THREADS when it is set."""
+    capped by OMP_NUM_THREADS when it is set.  Counted once, before OpenMP
+    starts: with OMP_PROC_BIND set, OpenMP binds the main thread to one
+    place, and the process's affinity shrinks to it."""
+    global _CORES
+    if _CORES is not None:
+        return _CORES
     cpus = sorted(os.sched_getaffinity(0))
     tpc = 1
     try:
@@ -158,7 +166,8 @@ def physical_cores():
         pass
     n = max(1, len(cpus) // tpc)
     cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
-    return min(n, cap) if cap > 0 else n
+    _CORES = min(n, cap) if cap > 0 else n
+    return _CORES
 
 
 def omp_protocol_env():
@@ -650,6 +659,7 @@ def main():
               "(sharding by row slabs or by whole systems) is not run here", file=sys.stderr)
         sys.exit(2)
 
+    physical_cores()  # before anything starts OpenMP
     if args.workload in ("abmc192", "mc128", "batch"):
         omp_protocol_env()
     import iccg_amd as I

@@ -833,12 +833,6 @@ int iccg_csr_create(int n, long nnz, const int *rowptr, const int *colind, const
         S.maxw = w;
         S.off = dupload(off.data(), off.size(), A->st);
         S.col = dupload(col.data(), col.size(), A->st);
-        S.v64 = dupload(v.data(), v.size(), A->st);
-        // the single-system SpMV reads 16-bit columns where they apply
-        // (measured 146 vs 156 us at 192^3); the triangular sweeps and the
-        // multi-system SpMV keep 32-bit columns (there the per-slice table
-        // load lengthens a latency-bound chain: measured slower)
-        upload_col16(S, off, col, A->st);
         // values that are all exact floats also get an f32 copy (half the
         // value bytes for the multi-system SpMV, same products bit for bit)
         bool exact = true;
@@ -847,6 +841,23 @@ int iccg_csr_create(int n, long nnz, const int *rowptr, const int *colind, const
           std::vector<float> v32(v.begin(), v.end());
           A->d.sell_v32 = dupload(v32.data(), v32.size(), A->st);
         }
+        // the single-system SpMV reads 16-bit columns where they apply
+        // (measured 146 vs 156 us at 192^3), with the values and columns in
+        // the paired layout (two entries of a lane per load); the triangular
+        // sweeps and the multi-system SpMV keep 32-bit columns (there the
+        // per-slice table load lengthens a latency-bound chain: measured
+        // slower), the latter reading the paired FP64 values
+        std::vector<uint16_t> c16;
+        std::vector<int> tbl;
+        if (sell_col16(off, col, c16, tbl)) {
+          pair_slices(off, c16);
+          pair_slices(off, v);
+          if (c16.empty()) c16.push_back((uint16_t)0xFFFF);
+          if (tbl.empty()) tbl.push_back(0);
+          S.col16 = dupload(c16.data(), c16.size(), A->st);
+          S.tbl = dupload(tbl.data(), tbl.size(), A->st);
+        }
+        S.v64 = dupload(v.data(), v.size(), A->st);
         HIPCHK(hipStreamSynchronize(A->st));
       }
     }

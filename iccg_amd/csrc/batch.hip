@@ -63,7 +63,7 @@ __device__ __forceinline__ GroupDesc group_of(const GroupDesc *__restrict__ grou
 // Ap.Ap).  Members are taken two at a time (the double-double partials of
 // more would cost occupancy); the later pairs find the item's entries in
 // cache.
-template <class VT, int WM, int KM>
+template <class VT, int WM, int KM, bool PV>
 __device__ __forceinline__ void ax_group(const GroupDesc &g, const SysDesc *__restrict__ desc,
                                          const SysScal *__restrict__ sc, const ChunkItem &c,
                                          const VT *__restrict__ av, double *part, int nch,
@@ -87,7 +87,10 @@ __device__ __forceinline__ void ax_group(const GroupDesc &g, const SysDesc *__re
     for (int s = c.s0 + w; s < c.s1; s += kWG >> 6) {
       int cc[WM];
       VT vv[WM];
-      sell_load<VT, WM, true>(D0.a_off, D0.a_col, av, s, lane, cc, vv);
+      if (PV)
+        sell_load_pairv<VT, WM>(D0.a_off, D0.a_col, av, s, lane, cc, vv);
+      else
+        sell_load<VT, WM, true>(D0.a_off, D0.a_col, av, s, lane, cc, vv);
       const int i = (s << 6) + lane;
 #pragma unroll
       for (int q = 0; q < KS; ++q) {
@@ -144,9 +147,11 @@ __global__ __launch_bounds__(kWG) void kb_ax(const GroupDesc *__restrict__ group
   }
   const SysDesc &D0 = desc[g.slot[0]];
   if (D0.a_v32)
-    ax_group<float, WM, KM>(g, desc, sc, c, D0.a_v32, part, nch, mode);
+    ax_group<float, WM, KM, false>(g, desc, sc, c, D0.a_v32, part, nch, mode);
+  else if (D0.a_paired)
+    ax_group<double, WM, KM, true>(g, desc, sc, c, D0.a_v64, part, nch, mode);
   else
-    ax_group<double, WM, KM>(g, desc, sc, c, D0.a_v64, part, nch, mode);
+    ax_group<double, WM, KM, false>(g, desc, sc, c, D0.a_v64, part, nch, mode);
 }
 
 // r -= alpha Ap (cg_crs.cpp:141); r.r partial -- per member.  x += alpha p
@@ -1444,6 +1449,7 @@ int iccg_batch_add(iccg_batch_t B, iccg_csr_t A, iccg_precond_t M, const double 
     D.a_off = A->d.sell.off;
     D.a_col = A->d.sell.col;
     D.a_v64 = A->d.sell.v64;
+    D.a_paired = A->d.sell.col16 != nullptr;
     D.a_v32 = A->d.sell_v32;
     D.l_off = M->F.ls.off;
     D.l_col = M->F.ls.col;

@@ -150,6 +150,7 @@ struct SysDesc {
   double tol = 0;
   const int *a_off = nullptr, *a_col = nullptr;  // A, sliced ELL
   const double *a_v64 = nullptr;
+  int a_paired = 0;                  // a_v64 in the paired layout (pair_slices)
   const float *a_v32 = nullptr;      // set when every value of A is an exact float
   const int *l_off = nullptr, *l_col = nullptr, *u_off = nullptr, *u_col = nullptr;
   const unsigned *l_inb = nullptr, *u_inb = nullptr;  // sectioned layout

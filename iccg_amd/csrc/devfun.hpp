@@ -282,19 +282,6 @@ __device__ __forceinline__ void st4_nt(double *p, int i, const double (&o)[4]) {
 // ---------------------------------------------------------------------------
 // Sliced-ELL row chains
 // ---------------------------------------------------------------------------
-// Column of a 16-bit sliced-ELL entry, given its slice's block table in LDS.
-__device__ __forceinline__ int col16_at(unsigned c, const int *tb) {
-  return c == 0xFFFFu ? -1 : (tb[(c >> 6) & 15u] << 6) | (int)(c & 63u);
-}
-
-// Copies the block table of slice s into the wavefront's LDS window tb (16
-// ints); called by every lane of the wavefront.
-__device__ __forceinline__ void load_tbl16(const int *__restrict__ tbl, int s, int *tb, int lane) {
-  wave_sync();  // earlier reads of tb are complete
-  if (lane < 16) tb[lane] = tbl[16 * s + lane];
-  wave_sync();
-}
-
 // The lane's row of slice s as registers: columns (padding -1) and values.
 // SEL: the loads of a slice with entries are issued without a branch
 // (entries k >= w load entry 0 again, in a line already fetched, and become
@@ -359,102 +346,139 @@ __device__ __forceinline__ AT sell_apply_sel(const int (&cc)[WM], const VT (&vv)
   return acc;
 }
 
-// sell_chain over the 16-bit layout (tb: the slice's table, load_tbl16)
-template <class VT, class AT, class XT, int WM, bool NEG>
-__device__ __forceinline__ AT sell_chain16(const int *__restrict__ off,
-                                           const uint16_t *__restrict__ col16, const int *tb,
-                                           const VT *__restrict__ v, const XT *__restrict__ x,
-                                           int s, int lane, AT acc) {
+// The lane's row sum 0 + v_0 x[c_0] + v_1 x[c_1] + ... (ascending stored
+// order, one fma per entry) over slice s of A's 16-bit layout, stored
+// paired (pair_slices): entries 2j and 2j+1 of a lane sit side by side, so
+// one dword load brings both 16-bit columns and one 16-byte load both
+// values; a slice of odd width keeps its last entry in place, applied after
+// the pairs (the pairs past the width are padding, so the order of the
+// terms is unchanged).  spmv16_load issues a slice's loads (for a
+// wave-uniform s: the slice header comes from scalar loads); spmv16_finish
+// puts the slice's column-block table in LDS, gathers x[c] (x[0] for
+// padding) and runs the chain; padding terms are skipped by a select, as
+// the reference's CSR loop never sees them.  A wavefront issues the next
+// slice's loads before the current slice's gathers (at 192^3: 138.3 ->
+// 129.0 us per launch; the next slice's columns alone, 130.9 us at 7 waves
+// per SIMD against 5).
+typedef double dbl2_t __attribute__((ext_vector_type(2)));
+template <int WM>
+struct Slice16 {
+  int w, te;
+  unsigned cp[WM / 2];
+  dbl2_t vp[WM / 2];
+  unsigned ct;
+  double vt;
+};
+
+template <int WM>
+__device__ __forceinline__ void spmv16_load(const int *__restrict__ off,
+                                            const uint16_t *__restrict__ col16,
+                                            const int *__restrict__ tbl,
+                                            const double *__restrict__ v, int s, int lane,
+                                            Slice16<WM> &q) {
+  constexpr int WP = WM / 2;
   const int o0 = off[s];
   const int w = (off[s + 1] - o0) >> 6;
-  const int base = o0 + lane;
-  if (WM > 0) {
-    constexpr int K = WM > 0 ? WM : 1;
-    int cc[K];
-    VT vv[K];
+  const int np = w >> 1;
+  q.w = w;
+  q.te = tbl[16 * s + (lane & 15)];
+  const unsigned *c32 = reinterpret_cast<const unsigned *>(col16 + o0);  // o0 is a multiple of 64
+  const dbl2_t *v2 = reinterpret_cast<const dbl2_t *>(v + o0);
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      cc[k] = -1;
-      vv[k] = (VT)0;
-      if (k < w) {
-        cc[k] = col16_at(col16[base + 64 * k], tb);
-        vv[k] = v[base + 64 * k];
-      }
-    }
-    AT xv[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) xv[k] = cc[k] >= 0 ? (AT)x[cc[k]] : (AT)0;
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-      if (cc[k] >= 0) acc = NEG ? fma_t(-(AT)vv[k], xv[k], acc) : fma_t((AT)vv[k], xv[k], acc);
-  } else {
-    for (int k = 0; k < w; ++k) {
-      const int c = col16_at(col16[base + 64 * k], tb);
-      if (c >= 0) {
-        const AT a = (AT)v[base + 64 * k];
-        acc = NEG ? fma_t(-a, (AT)x[c], acc) : fma_t(a, (AT)x[c], acc);
-      }
+  for (int j = 0; j < WP; ++j) {
+    q.cp[j] = 0xFFFFFFFFu;
+    q.vp[j] = dbl2_t{0.0, 0.0};
+    if (j < np) {
+      q.cp[j] = ldm<true>(&c32[64 * j + lane]);
+      q.vp[j] = ldm<true>(&v2[64 * j + lane]);
     }
   }
-  return acc;
+  q.ct = 0xFFFFu;
+  q.vt = 0.0;
+  if (w & 1) {
+    q.ct = ldm<true>(&col16[o0 + 128 * np + lane]);
+    q.vt = ldm<true>(&v[o0 + 128 * np + lane]);
+  }
 }
 
-// The lane's row sum 0 + v_0 x[c_0] + v_1 x[c_1] + ... (ascending stored
-// order, one fma per entry) over slice s of the 16-bit layout, for a
-// wave-uniform s (the caller's loop runs on readfirstlane values, so the
-// slice header comes from scalar loads).  Every load is issued without a
-// branch: the table entry, all WM columns, then all WM values (entries
-// k >= w load entry 0 again, in a line already fetched), so the one wait
-// before the table lookups covers the table and the columns only, and the
-// gathers x[c] (x[0] for padding) go out together.  Padding terms are
-// skipped by a select, as the reference's CSR loop never sees them.
 template <int WM>
-__device__ __forceinline__ double spmv_slice16(const int *__restrict__ off,
-                                               const uint16_t *__restrict__ col16,
-                                               const int *__restrict__ tbl, int *tb,
-                                               const double *__restrict__ v,
-                                               const double *__restrict__ x, int s, int lane) {
-  const int o0 = off[s];
-  const int w = (off[s + 1] - o0) >> 6;
-  if (w == 0) return 0.0;  // a slice of empty rows (uniform): entry 0 may not exist
-  const int base = o0 + lane;
-  const int te = tbl[16 * s + (lane & 15)];
-  unsigned cr[WM];
-  double vv[WM];
-#pragma unroll
-  for (int k = 0; k < WM; ++k) cr[k] = ldm<true>(&col16[k < w ? base + 64 * k : base]);
-#pragma unroll
-  for (int k = 0; k < WM; ++k) vv[k] = ldm<true>(&v[k < w ? base + 64 * k : base]);
+__device__ __forceinline__ double spmv16_finish(const Slice16<WM> &q, int *tb,
+                                                const double *__restrict__ x, int lane) {
+  constexpr int WP = WM / 2;
+  if (q.w == 0) return 0.0;  // a slice of empty rows (uniform)
   wave_sync();  // the previous slice's table reads are issued
-  tb[lane & 15] = te;
+  tb[lane & 15] = q.te;
   wave_sync();
-  int cc[WM], t[WM];
+  int cl[WP], ch[WP], t0[WP], t1[WP];
 #pragma unroll
-  for (int k = 0; k < WM; ++k) {
-    cr[k] = k < w ? cr[k] : 0xFFFFu;
-    t[k] = tb[(cr[k] >> 6) & 15u];
+  for (int j = 0; j < WP; ++j) {
+    t0[j] = tb[(q.cp[j] >> 6) & 15u];
+    t1[j] = tb[(q.cp[j] >> 22) & 15u];
   }
+  const int tt = tb[(q.ct >> 6) & 15u];
   // every lane reads its table entries, all issued before the first is used
-  // (without this the compiler puts a branch and a wait around each read)
 #pragma unroll
-  for (int k = 0; k < WM; ++k) asm volatile("" : "+v"(t[k]));
+  for (int j = 0; j < WP; ++j) asm volatile("" : "+v"(t0[j]), "+v"(t1[j]));
 #pragma unroll
-  for (int k = 0; k < WM; ++k) cc[k] = cr[k] == 0xFFFFu ? -1 : (t[k] << 6) | (int)(cr[k] & 63u);
-  double xv[WM];
+  for (int j = 0; j < WP; ++j) {
+    const unsigned lo = q.cp[j] & 0xFFFFu, hi = q.cp[j] >> 16;
+    cl[j] = lo == 0xFFFFu ? -1 : (t0[j] << 6) | (int)(lo & 63u);
+    ch[j] = hi == 0xFFFFu ? -1 : (t1[j] << 6) | (int)(hi & 63u);
+  }
+  const int ctl = q.ct == 0xFFFFu ? -1 : (tt << 6) | (int)(q.ct & 63u);
+  double xl[WP], xh[WP];
 #pragma unroll
-  for (int k = 0; k < WM; ++k) xv[k] = x[cc[k] >= 0 ? cc[k] : 0];
+  for (int j = 0; j < WP; ++j) {
+    xl[j] = x[cl[j] >= 0 ? cl[j] : 0];
+    xh[j] = x[ch[j] >= 0 ? ch[j] : 0];
+  }
+  const double xt = x[ctl >= 0 ? ctl : 0];
   double acc = 0.0;
 #pragma unroll
-  for (int k = 0; k < WM; ++k) {
-    const double t = fma(vv[k], xv[k], acc);
-    acc = cc[k] >= 0 ? t : acc;
+  for (int j = 0; j < WP; ++j) {
+    const double a = fma(q.vp[j].x, xl[j], acc);
+    acc = cl[j] >= 0 ? a : acc;
+    const double b = fma(q.vp[j].y, xh[j], acc);
+    acc = ch[j] >= 0 ? b : acc;
   }
-  return acc;
+  const double c = fma(q.vt, xt, acc);
+  return ctl >= 0 ? c : acc;
+}
+
+// sell_load of A's values in the paired layout (pair_slices): entry k < 2
+// (w / 2) of a lane at o0 + 128 (k / 2) + 2 lane + k % 2, an odd last entry
+// at o0 + 64 k + lane; columns in the plain layout.  Entries k >= w load
+// entry 0 again and get column -1 (the branch-free form of sell_load).
+template <class VT, int WM>
+__device__ __forceinline__ void sell_load_pairv(const int *__restrict__ off,
+                                                const int *__restrict__ col,
+                                                const VT *__restrict__ v, int s, int lane,
+                                                int (&cc)[WM], VT (&vv)[WM]) {
+  const int o0 = off[s];
+  const int w = (off[s + 1] - o0) >> 6;
+#pragma unroll
+  for (int k = 0; k < WM; ++k) {
+    cc[k] = -1;
+    vv[k] = (VT)0;
+  }
+  if (w == 0) return;  // a slice of rows without entries: entry 0 may not exist
+  const int np2 = (w >> 1) << 1;
+  const int base = o0 + lane;
+#pragma unroll
+  for (int k = 0; k < WM; ++k) cc[k] = ldm<true>(&col[k < w ? base + 64 * k : base]);
+#pragma unroll
+  for (int k = 0; k < WM; ++k) {
+    const int kk = k < w ? k : 0;
+    const int q = kk < np2 ? o0 + 128 * (kk >> 1) + 2 * lane + (kk & 1) : base + 64 * kk;
+    vv[k] = ldm<true>(&v[q]);
+  }
+#pragma unroll
+  for (int k = 0; k < WM; ++k) cc[k] = k < w ? cc[k] : -1;
 }
 
 // sell_chain (WM > 0) for a wave-uniform s, with every load issued without a
 // branch: entries k >= w load entry 0 again (a line already fetched) and
-// padding gathers x[0], both dropped by a select, as in spmv_slice16.  The
+// padding gathers x[0], both dropped by a select, as in spmv16_finish.  The
 // branchy form, once the kernel also carries an r.z partial, compiled to
 // copies of the whole cc/vv arrays on every path (134 VGPRs at WM = 8).
 template <class VT, class AT, class XT, int WM, bool NEG, bool NT>

@@ -42,7 +42,8 @@ __device__ __forceinline__ double row_dot(const int *__restrict__ rp, const int 
 template <int WM, class F>
 __device__ __forceinline__ void for_rows_Ax(const DevCsr &A, const double *__restrict__ x, F f) {
   if (WM > 0 && A.sell.off && A.sell.col16) {
-    constexpr int K = WM > 0 ? WM : 1;
+    // 16-bit columns: the values and columns are in the paired layout
+    constexpr int K = WM > 0 ? WM : 2;
     __shared__ int tbs[kWG >> 6][16];
     const int lane = threadIdx.x & 63;
     int *tb = tbs[threadIdx.x >> 6];
@@ -50,25 +51,21 @@ __device__ __forceinline__ void for_rows_Ax(const DevCsr &A, const double *__res
     const int first = __builtin_amdgcn_readfirstlane(wr.first);
     const int step = __builtin_amdgcn_readfirstlane(wr.step);
     const int end = __builtin_amdgcn_readfirstlane(wr.end);
-    for (int s = first; s < end; s += step) {
-      const double y =
-          spmv_slice16<K>(A.sell.off, A.sell.col16, A.sell.tbl, tb, A.sell.v64, x, s, lane);
-      const int i = (s << 6) + lane;
-      if (i < A.n) f(i, y);
+    if (first < end) {
+      // the next slice's loads go out before the current slice's gathers
+      Slice16<K> cur;
+      spmv16_load<K>(A.sell.off, A.sell.col16, A.sell.tbl, A.sell.v64, first, lane, cur);
+      for (int s = first; s < end; s += step) {
+        Slice16<K> nxt = cur;
+        if (s + step < end)
+          spmv16_load<K>(A.sell.off, A.sell.col16, A.sell.tbl, A.sell.v64, s + step, lane, nxt);
+        const double y = spmv16_finish<K>(cur, tb, x, lane);
+        const int i = (s << 6) + lane;
+        if (i < A.n) f(i, y);
+        cur = nxt;
+      }
     }
-  } else if (A.sell.off && A.sell.col16) {
-    __shared__ int tbs[kWG >> 6][16];
-    const int lane = threadIdx.x & 63;
-    int *tb = tbs[threadIdx.x >> 6];
-    const WaveRange wr = xcd_waves(0, A.sell.nslice);
-    for (int s = wr.first; s < wr.end; s += wr.step) {
-      load_tbl16(A.sell.tbl, s, tb, lane);
-      const double y = sell_chain16<double, double, double, WM, false>(
-          A.sell.off, A.sell.col16, tb, A.sell.v64, x, s, lane, 0.0);
-      const int i = (s << 6) + lane;
-      if (i < A.n) f(i, y);
-    }
-  } else if (A.sell.off) {
+  } else if (A.sell.off && !A.sell.col16) {
     const int lane = threadIdx.x & 63;
     const WaveRange wr = xcd_waves(0, A.sell.nslice);
     for (int s = wr.first; s < wr.end; s += wr.step) {

@@ -86,20 +86,30 @@ inline bool sell_col16(const std::vector<int> &off, const std::vector<int> &col,
   return true;
 }
 
-// uploads the 16-bit column layout of S (host off / col) when it applies
-inline void upload_col16(Sell &S, const std::vector<int> &off, const std::vector<int> &col,
-                         hipStream_t st) {
-  std::vector<uint16_t> c16;
-  std::vector<int> tbl;
-  if (!sell_col16(off, col, c16, tbl)) return;
-  if (c16.empty()) c16.push_back((uint16_t)0xFFFF);
-  if (tbl.empty()) tbl.push_back(0);
-  S.col16 = dupload(c16.data(), c16.size(), st);
-  S.tbl = dupload(tbl.data(), tbl.size(), st);
-  HIPCHK(hipStreamSynchronize(st));
+// The paired layout of a 64-row sliced ELL array (A's values and 16-bit
+// columns where the 16-bit layout applies): in a slice of width w at o0,
+// entries 2j and 2j+1 of lane l (j < w / 2) move to o0 + 128 j + 2 l and
+// o0 + 128 j + 2 l + 1; an odd last entry stays at o0 + 64 (w - 1) + l.
+template <class T>
+inline void pair_slices(const std::vector<int> &off, std::vector<T> &a) {
+  const int ns = (int)off.size() - 1;
+#pragma omp parallel
+  {
+    std::vector<T> tmp;
+#pragma omp for schedule(static)
+    for (int s = 0; s < ns; ++s) {
+      const int o0 = off[(size_t)s], w = (off[(size_t)s + 1] - o0) >> 6, np = w >> 1;
+      if (np == 0) continue;
+      tmp.assign(a.begin() + o0, a.begin() + o0 + 128 * np);
+      for (int j = 0; j < np; ++j)
+        for (int l = 0; l < 64; ++l)
+          for (int h = 0; h < 2; ++h)
+            a[(size_t)o0 + 128 * j + 2 * l + h] = tmp[(size_t)64 * (2 * j + h) + l];
+    }
+  }
 }
 
-// Host layout of a 64-row sliced ELL over row ranges [beg(i), end(i)) of a
+// Host layout of a 64-row sliced ELL over row ranges [beg(i), end(i)) of a// Host layout of a 64-row sliced ELL over row ranges [beg(i), end(i)) of a
 // CSR-like index array ci: off[nslice+1], col (padding -1), and for every
 // included CSR position k the ELL position (pos[k]).
 template <class Range>
