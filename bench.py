@@ -488,6 +488,8 @@ def drive(server, reqs, arrivals):
         x, res, lat, prep = server.result(rid)
         out.append((lat, prep, res, x))
         server.release(rid)
+        if len(out) % 16 == 0 or len(out) == len(ids):
+            log(f"  {len(out)}/{len(ids)} requests done ({time.perf_counter() - t0:.1f}s)")
     return out, time.perf_counter() - t0
 
 
