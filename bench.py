@@ -337,6 +337,16 @@ def batch_kernel_table(B, systems, res, count, max_iter, tol, M32, dA):
     return out
 
 
+def batch_roofline(kernels):
+    """The kernel class with the largest event time over the whole batched
+    solve: its algorithmic bytes (every system's iterations) over that time."""
+    nm, k = max(kernels.items(), key=lambda kv: kv[1]["total_ms"])
+    return {"bound": "hbm", "kernel": nm,
+            "unit_of_launch": "every launch of the class over one batched solve of all systems",
+            "achieved": k["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": k["frac"],
+            "traffic": None, "algorithmic_bytes": k["bytes"], "total_ms": k["total_ms"]}
+
+
 def main_batch(args):
     """Many-system workload: one Batch solving every system (mixed precision,
     stop on the FP64 true residual) against the plain FP64 single-system
@@ -446,6 +456,7 @@ def main_batch(args):
         "max_err_vs_fp64": err,
         "tuning": {"chunk_slices": chunk, "sweep_slices_per_wave": cw},
         "kernels": kernels,
+        "roofline": batch_roofline(kernels),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

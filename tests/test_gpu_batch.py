@@ -155,8 +155,9 @@ def test_batch_rejects_unsupported():
 def test_batch_shared_matrix_groups(kind, precision):
     """Right-hand sides of one matrix run as groups (up to 8 per group, here
     11 right-hand sides = 8 + 3) that read the matrix once; every one gives
-    its solo iterates.  A second system with 2 right-hand sides joins
-    mid-run."""
+    its solo iterates and the oracle's (FP64: the double-double CG; mixed:
+    orc_cg_mixed) bit for bit.  A second system with 2 right-hand sides
+    joins mid-run."""
     dims = (40, 36, 1) if kind == "mc" else (12, 12, 12)
     o, _ = _system(*dims, 700, kind=kind)
     M = I.IC0(o.A, 0.0, sched=o.sched, precision=precision)
@@ -173,6 +174,22 @@ def test_batch_shared_matrix_groups(kind, precision):
         pass
     _check_same(B, slots, [_alone(o, b, precision) for b in rhs])
     _check_same(B, slots2, [_alone(o2, b, precision) for b in rhs2])
+    # and against the oracle: the double-double FP64 CG, or the mixed CG
+    # with its restarts
+    for sl, (oo, bs) in ((slots, (o, rhs)), (slots2, (o2, rhs2))):
+        Ao = P.orc(oo.A)
+        f = O.ic0_factor(Ao)
+        for s, b in zip(sl, bs):
+            fin, rb = B.result(s)
+            if precision == "fp64":
+                d = O.cg(Ao, b, "ic0", ic=f, tol=1e-8, dot_mode=1)
+                assert (rb.iters, rb.rel_resid) == (d.iters, d.rel_resid)
+                assert np.array_equal(B.x(s), d.x)
+            else:
+                m = O.cg_mixed(Ao, b, f, tol=1e-8)
+                assert (rb.iters, rb.restarts, rb.rel_resid) == (m.iters, m.restarts,
+                                                                 m.rel_resid)
+                assert np.array_equal(B.x(s), m.x)
 
 
 def test_batch_timing_mode_counts_and_keeps_results():
