@@ -51,14 +51,17 @@ def iter_bytes(n, nnz, nnz_l, mixed):
     return spmv_bytes(n, nnz) + 2 * trsv_bytes(n, nnz_l, 8, 8, 8) + 80 * n
 
 
-def build_problem(N, block):
+def build_problem(N, block, dims=3):
     import iccg_amd as I
     t0 = time.time()
-    A = I.gen_grid(N, N, N, coef_exp=(-2.0, 2.0), diag_shift=1e-6, seed=SEED)
-    log(f"grid {N}^3: n={A.n} nnz={A.nnz} ({time.time() - t0:.1f}s)")
+    nz = N if dims == 3 else 1
+    A = I.gen_grid(N, N, nz, coef_exp=(-2.0, 2.0), diag_shift=1e-6, seed=SEED)
+    log(f"grid {N}^{dims}: n={A.n} nnz={A.nnz} ({time.time() - t0:.1f}s)")
     t0 = time.time()
     if block == "box":
-        bo, nb, bc, nc = I.abmc_blocking_grid(N, N, N, 4, 4, 4)
+        # 64-row boxes: 4x4x4 in 3D, 8x8 in 2D
+        bo, nb, bc, nc = (I.abmc_blocking_grid(N, N, N, 4, 4, 4) if dims == 3 else
+                          I.abmc_blocking_grid(N, N, 1, 8, 8, 1))
     else:
         bo, nb, bc, nc = I.abmc_blocking(A, int(block))
     p = I.BuildPermutationByBlockColor(bo, bc)
@@ -656,6 +659,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=192)
+    ap.add_argument("--dims", type=int, choices=[2, 3], default=3,
+                    help="3: the N^3 7-point headline; 2: an N^2 5-point grid of the same "
+                         "coefficients (8x8 boxes), as the serving stream's 2D requests")
     ap.add_argument("--block", default="box",
                     help="'box': 4x4x4 grid boxes of 64 rows; or an integer B for breadth-first "
                          "blocks of B rows (both colored first-fit)")
@@ -686,7 +692,7 @@ def main():
     if args.workload == "mc128":
         return main_mc128(args)
     N = args.size
-    Ap, sched, b, info = build_problem(N, args.block)
+    Ap, sched, b, info = build_problem(N, args.block, args.dims)
     n, nnz = Ap.n, Ap.nnz
     nnz_l = (nnz + n) // 2
 
@@ -776,7 +782,8 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms32, 3), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64 CG + f32 preconditioner", "data": "synthetic",
-        "config": {"workload": (f"abmc_crs: 3D 7-pt variable-coefficient diffusion {N}^3 "
+        "config": {"workload": (f"abmc_crs: {args.dims}D {2 * args.dims + 1}-pt "
+                                f"variable-coefficient diffusion {N}^{args.dims} "
                                 f"(n={n}, nnz={nnz}), ABMC B={args.block} greedy "
                                 f"({info['nb']} blocks, {info['nc']} colors), b~N(0,1), x0=0, "
                                 f"tol {args.tol:g} on the FP64 true residual"),
