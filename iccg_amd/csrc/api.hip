@@ -340,7 +340,8 @@ static void build_schedule(iccg_precond_s &M, const iccg_sched_s *s, const std::
 static int abmc_sections(const DevSched &sc, int n, const std::vector<int> &lrp,
                          const std::vector<int> &lci, const std::vector<int> &urp,
                          const std::vector<int> &uci, std::vector<int> &blk, std::vector<int> &lout,
-                         std::vector<int> &uout) {
+                         std::vector<int> &uout, int &so) {
+  so = 0;
   if (sc.kind != 2 || sc.d_brows != nullptr || sc.max_block > 64 || n == 0) return 0;
   blk.assign((size_t)n, 0);
   std::vector<int> blo((size_t)n), bhi((size_t)n);
@@ -354,7 +355,7 @@ static int abmc_sections(const DevSched &sc, int n, const std::vector<int> &lrp,
   std::vector<int> lin((size_t)n), uin((size_t)n);
   lout.assign((size_t)n, 0);
   uout.assign((size_t)n, 0);
-  int S = 0;
+  int S = 0, O = 0;
   for (int i = 0; i < n; ++i) {
     for (int k = lrp[i]; k < lrp[i + 1] - 1; ++k) {
       const bool in = lci[k] >= blo[(size_t)i] && lci[k] < bhi[(size_t)i];
@@ -364,18 +365,28 @@ static int abmc_sections(const DevSched &sc, int n, const std::vector<int> &lrp,
       const bool in = uci[k] >= blo[(size_t)i] && uci[k] < bhi[(size_t)i];
       (in ? uin : uout)[(size_t)i]++;
     }
-    S = std::max({S, lin[(size_t)i], lout[(size_t)i], uin[(size_t)i], uout[(size_t)i]});
+    S = std::max({S, lin[(size_t)i], uin[(size_t)i]});
+    O = std::max({O, lout[(size_t)i], uout[(size_t)i]});
   }
-  if (S > 3) return 0;
+  // kernel shapes: 2 or 3 in-block slots, as many or 6 outside ones; one
+  // wavefront per block, so blocks must fill most of theirs (40 of 64 lanes
+  // on average)
+  if (S > 3 || O > 6 || (long)n < 40L * nb) return 0;
   S = std::max(S, 2);
+  O = O <= S ? S : 6;
+  // bytes per sweep pair (8-byte values, 4-byte columns): a plain row reads
+  // its slice's widths of entries, a sectioned row the packed in-block word,
+  // S in-block values and its block's outside widths of entries; lanes past
+  // a short block's end re-read its last row, which costs no traffic
   long plain = 0, sect = 0;
   for (int s0 = 0; s0 < n; s0 += 64) {
     int wl = 0, wu = 0;
-    for (int i = s0; i < std::min(n, s0 + 64); ++i) {
+    const int rows = std::min(n, s0 + 64) - s0;
+    for (int i = s0; i < s0 + rows; ++i) {
       wl = std::max(wl, lin[(size_t)i] + lout[(size_t)i]);
       wu = std::max(wu, uin[(size_t)i] + uout[(size_t)i]);
     }
-    plain += 64L * (wl + wu);
+    plain += 12L * rows * (wl + wu);
   }
   for (int kb = 0; kb < nb; ++kb) {
     int ol = 0, ou = 0;
@@ -383,9 +394,11 @@ static int abmc_sections(const DevSched &sc, int n, const std::vector<int> &lrp,
       ol = std::max(ol, lout[(size_t)i]);
       ou = std::max(ou, uout[(size_t)i]);
     }
-    sect += 64L * (2 * S + ol + ou);
+    sect += (long)(sc.h_bstart[kb + 1] - sc.h_bstart[kb]) * (8 + 16L * S + 12L * (ol + ou));
   }
-  return 4 * sect <= 5 * plain ? S : 0;
+  if (4 * sect > 5 * plain) return 0;
+  so = O;
+  return S;
 }
 
 // Sliced-ELL copies of the strictly lower / strictly upper factor and its
@@ -410,7 +423,8 @@ static void build_sliced_factor(iccg_precond_s &M, const std::vector<int> &lrp,
     e = urp[i + 1];
   };
   std::vector<int> blk, lout, uout;
-  const int S = abmc_sections(M.sc, n, lrp, lci, urp, uci, blk, lout, uout);
+  int SO = 0;
+  const int S = abmc_sections(M.sc, n, lrp, lci, urp, uci, blk, lout, uout, SO);
   std::vector<unsigned> linb, uinb;
   if (S > 0) {
     sell_layout_sections(n, lrange, lci.data(), M.sc.h_bstart, blk.data(), lout, S, loff, lcol,
@@ -423,6 +437,7 @@ static void build_sliced_factor(iccg_precond_s &M, const std::vector<int> &lrp,
   }
   if (std::max(lw, uw) > 16) return;  // wide rows: CSR sweeps only
   F.sec = S;
+  F.seco = SO;
   F.fb = false;
   if (S > 0 && M.sc.nc > 0) {
     F.fb = true;
@@ -1002,6 +1017,7 @@ int iccg_precond_set_schedule(iccg_precond_t M, iccg_sched_t sched) {
       F.d32 = nullptr;
       F.sell = false;
       F.sec = 0;
+      F.seco = 0;
       F.fb = false;
       F.nt = false;
       build_sliced_factor(*M, lrp, lci, urp, uci, l2u);

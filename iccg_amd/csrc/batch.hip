@@ -438,7 +438,8 @@ __device__ __forceinline__ void abmc_bwd_group(int bs, int be, int nl, const Sys
 }
 
 // ABMC blocks of a factor in the sectioned layout (sell_layout_sections,
-// section width D0.sec <= SW): the single-system sweeps' mask-free level
+// D0.sec <= SW in-block slots, at most SW outside entries per row): the
+// single-system sweeps' mask-free level
 // loop (sec_levels) run for every live member on the shared entries.  my
 // holds one kSecWindow window per member (row slots and the zero slot).
 // FB (forward, blocks of the factor's last color): each member's backward
@@ -526,10 +527,10 @@ __device__ __forceinline__ void abmc_group_sec(int kb, int bs, int be, int nl, c
 #pragma unroll
   for (int j = 0; j < KM; ++j) {
     if (!live[j]) continue;
-    const FT zi = sec_levels<FT, SW, !FWD>(nl, acc[j], d, ad, va, vo, xo[j],
+    const FT zi = sec_levels<FT, SW, SW, !FWD>(nl, acc[j], d, ad, va, vo, xo[j],
                                                  my + kSecWindow * j, lane, act);
     if (FB) {
-      const FT zb = sec_levels<FT, SW, false>(D0.nlev_b[kb], zi, d, uad, uva, vo, xo[j],
+      const FT zb = sec_levels<FT, SW, SW, false>(D0.nlev_b[kb], zi, d, uad, uva, vo, xo[j],
                                               my + kSecWindow * j, lane, act);
       if (act) ((FT *)desc[g.slot[j]].z)[i] = zb;
     } else if (act) {
@@ -910,7 +911,8 @@ void build_items(iccg_batch_s &B, const std::vector<int> &ids, ItemSet &I, int c
     }
     const int sec = sc.kind == 2 ? s.M->F.sec : 0;
     const bool fb = sec > 0 && s.M->F.fb;
-    secw = std::max(secw, sec);
+    // the kernels' slot count covers the in-block and the outside entries
+    if (sec > 0) secw = std::max({secw, sec, s.M->F.seco});
     for (int c = 0; c < nc; ++c) {
       if (sc.kind == 2) {
         const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
@@ -1029,8 +1031,10 @@ void launch_resid_items(iccg_batch_s &B, const ItemSet &I, int mode) {
     const int _k = km_class(km);                                                              \
     if ((sw) <= 2 && _k == 1) hipLaunchKernelGGL((K<2, 1 EXTRA>), __VA_ARGS__);               \
     else if ((sw) <= 2) hipLaunchKernelGGL((K<2, kMaxRhs EXTRA>), __VA_ARGS__);               \
-    else if (_k == 1) hipLaunchKernelGGL((K<3, 1 EXTRA>), __VA_ARGS__);                       \
-    else hipLaunchKernelGGL((K<3, kMaxRhs EXTRA>), __VA_ARGS__);                              \
+    else if ((sw) <= 3 && _k == 1) hipLaunchKernelGGL((K<3, 1 EXTRA>), __VA_ARGS__);          \
+    else if ((sw) <= 3) hipLaunchKernelGGL((K<3, kMaxRhs EXTRA>), __VA_ARGS__);               \
+    else if (_k == 1) hipLaunchKernelGGL((K<6, 1 EXTRA>), __VA_ARGS__);                       \
+    else hipLaunchKernelGGL((K<6, kMaxRhs EXTRA>), __VA_ARGS__);                              \
   } while (0)
 
 template <bool FWD>

@@ -126,9 +126,10 @@ struct Factor {
   int *l2ls = nullptr;   // L CSR position -> ls entry, or -1 - row (diagonal)
   int *l2us = nullptr;   // L CSR position -> us entry, or -1 (diagonal)
   bool sell = false;
-  // ABMC factors: ls / us in the sectioned layout with S in-block and up to
-  // S outside entries per row (sell_layout_sections), or 0 for the plain one
-  int sec = 0;
+  // ABMC factors: ls / us in the sectioned layout with sec in-block slots
+  // and up to seco >= sec outside entries per row (sell_layout_sections), or
+  // sec = 0 for the plain one
+  int sec = 0, seco = 0;
   // sectioned, and every block of the last color has no outside entries in
   // U (its backward rows have in-block terms only)
   bool fb = false;

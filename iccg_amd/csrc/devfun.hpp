@@ -677,9 +677,9 @@ __device__ __forceinline__ YT abmc_bwd_block(int bs, int be, int nl,
 // rows (at most 64, lane = row - bstart[b]).  Entry k < S of a row is its
 // k-th in-block entry (columns in [bs, be), ascending; the kernels read their
 // rows within the block from the packed word Sell::inb, and the value, 0
-// past the last), entry S + j its j-th entry outside the block (ascending,
-// column -1 past the last): for L the earlier colors' columns (< bs), for U
-// the later colors' (>= be).  A row's chain is the sequential reference's
+// past the last), entry S + j (j < SO, SO >= S outside entries at most) its
+// j-th entry outside the block (ascending, column -1 past the last): for L
+// the earlier colors' columns (< bs), for U the later colors' (>= be).  A row's chain is the sequential reference's
 // (ic0.cpp:263-339): forward, the outside terms first, then the in-block
 // terms; backward, the in-block terms first, then the outside terms.
 //
@@ -695,10 +695,10 @@ __device__ __forceinline__ YT abmc_bwd_block(int bs, int be, int nl,
 // ---------------------------------------------------------------------------
 constexpr int kSecWindow = 65;  // LDS slots per wavefront: 64 rows + the zero slot
 
-template <class YT, int S, bool OUT>
+template <class YT, int S, int SO, bool OUT>
 __device__ __forceinline__ YT sec_levels(int nl, YT acc, YT d, const int (&ad)[S],
-                                         const YT (&va)[S], const YT (&vo)[S],
-                                         const YT (&xo)[S], YT *my, int lane, bool act) {
+                                         const YT (&va)[S], const YT (&vo)[SO],
+                                         const YT (&xo)[SO], YT *my, int lane, bool act) {
   const YT dy = rcp_refined(d);
   YT yi = (YT)0, a = acc;
   for (int l = 0; l < nl; ++l) {
@@ -710,7 +710,7 @@ __device__ __forceinline__ YT sec_levels(int nl, YT acc, YT d, const int (&ad)[S
     for (int k = 0; k < S; ++k) a = fma_t(-va[k], t[k], a);
     if (OUT) {
 #pragma unroll
-      for (int k = 0; k < S; ++k) a = fma_t(-vo[k], xo[k], a);
+      for (int k = 0; k < SO; ++k) a = fma_t(-vo[k], xo[k], a);
     }
     yi = div_steps(a, d, dy);
     my[lane] = yi;
@@ -726,7 +726,7 @@ __device__ __forceinline__ YT sec_levels(int nl, YT acc, YT d, const int (&ad)[S
       for (int k = 0; k < S; ++k) a = fma_t(-va[k], t[k], a);
       if (OUT) {
 #pragma unroll
-        for (int k = 0; k < S; ++k) a = fma_t(-vo[k], xo[k], a);
+        for (int k = 0; k < SO; ++k) a = fma_t(-vo[k], xo[k], a);
       }
       yi = a / d;
       my[lane] = yi;
@@ -740,11 +740,11 @@ __device__ __forceinline__ YT sec_levels(int nl, YT acc, YT d, const int (&ad)[S
 // stay in flight while the previous block runs: lanes past the block end and
 // absent outside entries load from in-bounds stand-in positions, and
 // sec_take masks them out.
-template <class VT, class YT, int S>
+template <class VT, class YT, int S, int SO>
 struct SecRaw {
   unsigned inb;  // packed in-block rows (Sell::inb)
-  int cout_[S];
-  VT vin[S], vout[S];
+  int cout_[SO];
+  VT vin[S], vout[SO];
   VT d;
   double a;   // r_i (forward) or y_i widened (backward)
   double ri;  // backward with r
@@ -754,13 +754,14 @@ struct SecRaw {
 // LOAD_NOW: the caller uses the loads at once, so absent outside entries
 // are skipped by (uniform) branches instead of loaded from stand-ins.
 // RU (forward): also (Ap)_i, for the r update that the sweep applies.
-template <class VT, class YT, int S, bool FWD, bool NT, bool LOAD_NOW = false, bool RU = false>
+template <class VT, class YT, int S, int SO, bool FWD, bool NT, bool LOAD_NOW = false,
+          bool RU = false>
 __device__ __forceinline__ void sec_fetch(int kb, int o0, int w, int bs, int be, int lane,
                                           const unsigned *__restrict__ inb,
                                           const int *__restrict__ col, const VT *__restrict__ v,
                                           const VT *__restrict__ diag,
                                           const double *__restrict__ r,
-                                          const YT *__restrict__ yin, SecRaw<VT, YT, S> &e,
+                                          const YT *__restrict__ yin, SecRaw<VT, YT, S, SO> &e,
                                           const double *__restrict__ Ap = nullptr) {
   const int t = min(lane, be - bs - 1);  // a row of the block
   const int base = o0 + t;
@@ -768,7 +769,7 @@ __device__ __forceinline__ void sec_fetch(int kb, int o0, int w, int bs, int be,
 #pragma unroll
   for (int k = 0; k < S; ++k) e.vin[k] = ldm<NT>(&v[base + 64 * k]);  // absent entries hold 0
 #pragma unroll
-  for (int k = 0; k < S; ++k) {
+  for (int k = 0; k < SO; ++k) {
     if (LOAD_NOW) {
       e.cout_[k] = -1;
       e.vout[k] = (VT)0;
@@ -794,14 +795,17 @@ __device__ __forceinline__ void sec_fetch(int kb, int o0, int w, int bs, int be,
 // Lanes past the block end hold a copy of the block's last row (sec_fetch
 // clamps them): they compute it again into their own window slots, which no
 // row of the block reads, and their results are neither stored nor checked.
-template <class VT, class YT, int S>
-__device__ __forceinline__ void sec_take(const SecRaw<VT, YT, S> &e, int w, int (&ad)[S],
-                                         YT (&va)[S], int (&co)[S], YT (&vo)[S], YT &d,
+template <class VT, class YT, int S, int SO>
+__device__ __forceinline__ void sec_take(const SecRaw<VT, YT, S, SO> &e, int w, int (&ad)[S],
+                                         YT (&va)[S], int (&co)[SO], YT (&vo)[SO], YT &d,
                                          YT &acc) {
 #pragma unroll
   for (int k = 0; k < S; ++k) {
     ad[k] = (int)((e.inb >> (8 * k)) & 0xFFu);
     va[k] = (YT)e.vin[k];
+  }
+#pragma unroll
+  for (int k = 0; k < SO; ++k) {
     const bool out = S + k < w && e.cout_[k] >= 0;
     co[k] = out ? e.cout_[k] : -1;
     vo[k] = out ? (YT)e.vout[k] : (YT)0;

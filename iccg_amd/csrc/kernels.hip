@@ -821,7 +821,7 @@ struct SecU {  // U's sectioned copy and the backward level counts (FB)
 // every row once).  With FB as well, the last color's z_i is final here, and
 // the r.z partials of its rows (cg_crs.cpp:152, r_i z_i as k_rz forms it)
 // go to part_rz[workgroup].
-template <class VT, class YT, int SW, bool FB, bool NT, bool RU>
+template <class VT, class YT, int SW, int SO, bool FB, bool NT, bool RU>
 __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
                                                       const int *__restrict__ bstart,
                                                       const int *__restrict__ nlev,
@@ -847,15 +847,15 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
   if (!RU && first >= end) return;  // RU: every wavefront reaches the partial
   const double alpha = RU ? S->alpha : 0.0;
   double h[1] = {0}, l[1] = {0}, hz[1] = {0}, lz[1] = {0};
-  YT none[SW];
+  YT none[SO];
 #pragma unroll
-  for (int k = 0; k < SW; ++k) none[k] = (YT)0;
+  for (int k = 0; k < SO; ++k) none[k] = (YT)0;
   const int f0 = min(first, kb1 - 1);
   int bs = bstart[f0], be = bstart[f0 + 1], nl = nlev[f0];
   int o0 = off[f0], w = (off[f0 + 1] - o0) >> 6;
-  SecRaw<VT, YT, SW> cur;
+  SecRaw<VT, YT, SW, SO> cur;
   if (PIPE)
-    sec_fetch<VT, YT, SW, true, NT, false, RU>(f0, o0, w, bs, be, lane, inb, col, lv, diag, r, y,
+    sec_fetch<VT, YT, SW, SO, true, NT, false, RU>(f0, o0, w, bs, be, lane, inb, col, lv, diag, r, y,
                                                cur, Ap);
   for (int kb = first; kb < end; kb += step) {
     if (!PIPE) {
@@ -864,13 +864,13 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
       nl = nlev[kb];
       o0 = off[kb];
       w = (off[kb + 1] - o0) >> 6;
-      sec_fetch<VT, YT, SW, true, NT, true, RU>(kb, o0, w, bs, be, lane, inb, col, lv, diag, r, y,
+      sec_fetch<VT, YT, SW, SO, true, NT, true, RU>(kb, o0, w, bs, be, lane, inb, col, lv, diag, r, y,
                                                 cur, Ap);
     }
     const bool act = bs + lane < be;
-    int ad[SW], co[SW];
-    YT va[SW], vo[SW], d, acc, xo[SW];
-    sec_take<VT, YT, SW>(cur, w, ad, va, co, vo, d, acc);
+    int ad[SW], co[SO];
+    YT va[SW], vo[SO], d, acc, xo[SO];
+    sec_take<VT, YT, SW, SO>(cur, w, ad, va, co, vo, d, acc);
     double rn = 0.0;
     if (RU) {
       rn = fma(-alpha, cur.ap, cur.a);
@@ -881,14 +881,14 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
       }
     }
 #pragma unroll
-    for (int k = 0; k < SW; ++k)
+    for (int k = 0; k < SO; ++k)
       xo[k] = PIPE ? y[co[k] >= 0 ? co[k] : 0] : co[k] >= 0 ? y[co[k]] : (YT)0;
     const int kc = min(kb + step, end - 1);
     const int nbs = PIPE ? bstart[kc] : 0, nbe = PIPE ? bstart[kc + 1] : 0;
     const int nnl = PIPE ? nlev[kc] : 0, no0 = PIPE ? off[kc] : 0;
     const int nw = PIPE ? (off[kc + 1] - no0) >> 6 : 0;
     if (PIPE)
-      sec_fetch<VT, YT, SW, true, NT, false, RU>(kc, no0, nw, nbs, nbe, lane, inb, col, lv, diag,
+      sec_fetch<VT, YT, SW, SO, true, NT, false, RU>(kc, no0, nw, nbs, nbe, lane, inb, col, lv, diag,
                                                  r, y, cur, Ap);
     int uad[SW];
     YT uva[SW];
@@ -904,12 +904,12 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
     }
     // the earlier colors' terms, ascending; absent ones are fma(-0, 0, acc)
 #pragma unroll
-    for (int k = 0; k < SW; ++k) acc = fma_t(-vo[k], co[k] >= 0 ? xo[k] : (YT)0, acc);
+    for (int k = 0; k < SO; ++k) acc = fma_t(-vo[k], co[k] >= 0 ? xo[k] : (YT)0, acc);
     if (lane == 0) my[64] = (YT)0;
     wave_sync();
-    const YT yi = sec_levels<YT, SW, false>(nl, acc, d, ad, va, none, none, my, lane, act);
+    const YT yi = sec_levels<YT, SW, SO, false>(nl, acc, d, ad, va, none, none, my, lane, act);
     if (FB) {
-      const YT zi = sec_levels<YT, SW, false>(U.nlev[kb], yi, d, uad, uva, none, none, my, lane,
+      const YT zi = sec_levels<YT, SW, SO, false>(U.nlev[kb], yi, d, uad, uva, none, none, my, lane,
                                               act);
       if (act) z[bs + lane] = zi;
       if (RU && act) dd_fma(hz[0], lz[0], rn, (double)zi);
@@ -928,7 +928,7 @@ __global__ __launch_bounds__(kWG) void k_abmc_fwd_sec(int kb0, int kb1,
   if (RU && FB) block_partial<1>(hz, lz, part_rz, blockIdx.x);
 }
 
-template <class VT, class YT, int SW, bool RZ, bool NT>
+template <class VT, class YT, int SW, int SO, bool RZ, bool NT>
 __global__ __launch_bounds__(kWG) void k_abmc_bwd_sec(
     int kb0, int kb1, const int *__restrict__ bstart, const int *__restrict__ nlev,
     const int *__restrict__ off, const unsigned *__restrict__ inb, const int *__restrict__ col,
@@ -948,8 +948,8 @@ __global__ __launch_bounds__(kWG) void k_abmc_bwd_sec(
   const int f0 = min(first, kb1 - 1);
   int bs = bstart[f0], be = bstart[f0 + 1], nl = nlev[f0];
   int o0 = off[f0], w = (off[f0 + 1] - o0) >> 6;
-  SecRaw<VT, YT, SW> cur;
-  if (PIPE) sec_fetch<VT, YT, SW, false, NT>(f0, o0, w, bs, be, lane, inb, col, uv, diag, rr, y, cur);
+  SecRaw<VT, YT, SW, SO> cur;
+  if (PIPE) sec_fetch<VT, YT, SW, SO, false, NT>(f0, o0, w, bs, be, lane, inb, col, uv, diag, rr, y, cur);
   for (int kb = first; kb < end; kb += step) {
     if (!PIPE) {
       bs = bstart[kb];
@@ -957,28 +957,28 @@ __global__ __launch_bounds__(kWG) void k_abmc_bwd_sec(
       nl = nlev[kb];
       o0 = off[kb];
       w = (off[kb + 1] - o0) >> 6;
-      sec_fetch<VT, YT, SW, false, NT, true>(kb, o0, w, bs, be, lane, inb, col, uv, diag, rr, y,
+      sec_fetch<VT, YT, SW, SO, false, NT, true>(kb, o0, w, bs, be, lane, inb, col, uv, diag, rr, y,
                                          cur);
     }
     const bool act = bs + lane < be;
-    int ad[SW], co[SW];
-    YT va[SW], vo[SW], d, acc, xo[SW];
-    sec_take<VT, YT, SW>(cur, w, ad, va, co, vo, d, acc);
+    int ad[SW], co[SO];
+    YT va[SW], vo[SO], d, acc, xo[SO];
+    sec_take<VT, YT, SW, SO>(cur, w, ad, va, co, vo, d, acc);
     const double ri = cur.ri;
 #pragma unroll
-    for (int k = 0; k < SW; ++k)
+    for (int k = 0; k < SO; ++k)
       xo[k] = PIPE ? z[co[k] >= 0 ? co[k] : 0] : co[k] >= 0 ? z[co[k]] : (YT)0;
     const int kc = min(kb + step, end - 1);
     const int nbs = PIPE ? bstart[kc] : 0, nbe = PIPE ? bstart[kc + 1] : 0;
     const int nnl = PIPE ? nlev[kc] : 0, no0 = PIPE ? off[kc] : 0;
     const int nw = PIPE ? (off[kc + 1] - no0) >> 6 : 0;
     if (PIPE)
-      sec_fetch<VT, YT, SW, false, NT>(kc, no0, nw, nbs, nbe, lane, inb, col, uv, diag, rr, y, cur);
+      sec_fetch<VT, YT, SW, SO, false, NT>(kc, no0, nw, nbs, nbe, lane, inb, col, uv, diag, rr, y, cur);
 #pragma unroll
-    for (int k = 0; k < SW; ++k) xo[k] = co[k] >= 0 ? xo[k] : (YT)0;
+    for (int k = 0; k < SO; ++k) xo[k] = co[k] >= 0 ? xo[k] : (YT)0;
     if (lane == 0) my[64] = (YT)0;
     wave_sync();
-    const YT zi = sec_levels<YT, SW, true>(nl, acc, d, ad, va, vo, xo, my, lane, act);
+    const YT zi = sec_levels<YT, SW, SO, true>(nl, acc, d, ad, va, vo, xo, my, lane, act);
     if (act) z[bs + lane] = zi;
     if (RZ && act) dd_fma(h[0], l[0], ri, (double)zi);
     if (PIPE) {
@@ -1391,7 +1391,7 @@ static void bwd_sliced(const DevSched &sc, const Factor &F, const VT *uv, const 
 // then written: the caller's r is the solver's own vector), the |r|^2
 // partials go to part[0, G) and the fused launch's r.z partials to
 // part[G, ...) (G = fwd_sec_groups, *ru_count)
-template <class VT, class YT, int SW, bool NT>
+template <class VT, class YT, int SW, int SO, bool NT>
 static void fwd_sec(const DevSched &sc, const Factor &F, const VT *lv, const VT *dg,
                     const double *r, YT *y, const Scal *S, hipStream_t st, const VT *uv,
                     YT *z, const double *Ap, double *part, int *ru_count) {
@@ -1406,19 +1406,19 @@ static void fwd_sec(const DevSched &sc, const Factor &F, const VT *lv, const VT 
     const int g = slice_grid(b1 - b0);
     const bool fb = uv && c == sc.nc - 1;
     if (Ap && fb)
-      hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, true, NT, true>), dim3(g), dim3(kWG), 0, st,
+      hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, SO, true, NT, true>), dim3(g), dim3(kWG), 0, st,
                          b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.inb, F.ls.col, lv, dg,
                          rw, y, us, z, S, Ap, part, gidx0, prz);
     else if (Ap)
-      hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, false, NT, true>), dim3(g), dim3(kWG), 0, st,
+      hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, SO, false, NT, true>), dim3(g), dim3(kWG), 0, st,
                          b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.inb, F.ls.col, lv, dg,
                          rw, y, none, nullptr, S, Ap, part, gidx0, nullptr);
     else if (fb)
-      hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, true, NT, false>), dim3(g), dim3(kWG), 0, st,
+      hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, SO, true, NT, false>), dim3(g), dim3(kWG), 0, st,
                          b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.inb, F.ls.col, lv, dg,
                          rw, y, us, z, S, nullptr, nullptr, 0, nullptr);
     else
-      hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, false, NT, false>), dim3(g), dim3(kWG), 0,
+      hipLaunchKernelGGL((k_abmc_fwd_sec<VT, YT, SW, SO, false, NT, false>), dim3(g), dim3(kWG), 0,
                          st, b0, b1, sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.inb, F.ls.col, lv,
                          dg, rw, y, none, nullptr, S, nullptr, nullptr, 0, nullptr);
     gidx0 += g;
@@ -1469,7 +1469,7 @@ void launch_fin_sweep_stop(const DevSched &sc, Scal *S, double *part, hipStream_
                      S);
 }
 
-template <class VT, class YT, int SW, bool NT>
+template <class VT, class YT, int SW, int SO, bool NT>
 static void bwd_sec(const DevSched &sc, const Factor &F, const VT *uv, const VT *dg, const YT *y,
                     YT *z, const double *r, Scal *S, double *part, int rz, hipStream_t st,
                     bool skip_last = false, int rz_slot0 = -1) {
@@ -1491,12 +1491,12 @@ static void bwd_sec(const DevSched &sc, const Factor &F, const VT *uv, const VT 
     if (g == 0) continue;
     const int b0 = sc.h_color_blk_v[c], b1 = sc.h_color_blk_v[c + 1];
     if (fuse)
-      hipLaunchKernelGGL((k_abmc_bwd_sec<VT, YT, SW, true, NT>), dim3(g), dim3(kWG), 0, st, b0, b1,
+      hipLaunchKernelGGL((k_abmc_bwd_sec<VT, YT, SW, SO, true, NT>), dim3(g), dim3(kWG), 0, st, b0, b1,
                          sc.d_bstart, sc.d_nlev_b, F.us.off, F.us.inb, F.us.col, uv, dg, y, z, r, S,
                          part,
                          gidx0);
     else
-      hipLaunchKernelGGL((k_abmc_bwd_sec<VT, YT, SW, false, NT>), dim3(g), dim3(kWG), 0, st, b0, b1,
+      hipLaunchKernelGGL((k_abmc_bwd_sec<VT, YT, SW, SO, false, NT>), dim3(g), dim3(kWG), 0, st, b0, b1,
                          sc.d_bstart, sc.d_nlev_b, F.us.off, F.us.inb, F.us.col, uv, dg, y, z, r, S,
                          part,
                          gidx0);
@@ -1508,12 +1508,17 @@ static void bwd_sec(const DevSched &sc, const Factor &F, const VT *uv, const VT 
 
 // FN<VT, YT, width, nt> over the section / slot widths and the factor's
 // load policy (Factor::nt)
-#define DISPATCH_SEC(sw, nt, FN, VT, YT, ...)                    \
-  do {                                                           \
-    if ((sw) == 2 && (nt)) FN<VT, YT, 2, true>(__VA_ARGS__);     \
-    else if ((sw) == 2) FN<VT, YT, 2, false>(__VA_ARGS__);       \
-    else if (nt) FN<VT, YT, 3, true>(__VA_ARGS__);               \
-    else FN<VT, YT, 3, false>(__VA_ARGS__);                      \
+#define DISPATCH_SEC(sw, so, nt, FN, VT, YT, ...)                       \
+  do {                                                                  \
+    const bool _w = (so) > (sw);                                        \
+    if ((sw) == 2 && _w && (nt)) FN<VT, YT, 2, 6, true>(__VA_ARGS__);   \
+    else if ((sw) == 2 && _w) FN<VT, YT, 2, 6, false>(__VA_ARGS__);     \
+    else if ((sw) == 2 && (nt)) FN<VT, YT, 2, 2, true>(__VA_ARGS__);    \
+    else if ((sw) == 2) FN<VT, YT, 2, 2, false>(__VA_ARGS__);           \
+    else if (_w && (nt)) FN<VT, YT, 3, 6, true>(__VA_ARGS__);           \
+    else if (_w) FN<VT, YT, 3, 6, false>(__VA_ARGS__);                  \
+    else if (nt) FN<VT, YT, 3, 3, true>(__VA_ARGS__);                   \
+    else FN<VT, YT, 3, 3, false>(__VA_ARGS__);                          \
   } while (0)
 
 #define DISPATCH_SWEEP_WM(wm, nt, FN, VT, YT, ...)               \
@@ -1586,10 +1591,10 @@ void launch_fwd(const DevSched &sc, const Factor &F, bool mixed, const double *r
     const double *ap = Ap && fwd_updates_r(sc, F) ? Ap : nullptr;
     double *xp = ap ? part : nullptr;
     if (mixed)
-      DISPATCH_SEC(F.sec, F.nt, fwd_sec, float, float, sc, F, F.ls.v32, F.d32, r, (float *)y, S, st,
+      DISPATCH_SEC(F.sec, F.seco, F.nt, fwd_sec, float, float, sc, F, F.ls.v32, F.d32, r, (float *)y, S, st,
                    fb ? F.us.v32 : nullptr, (float *)z_fused, ap, xp, ru_count);
     else
-      DISPATCH_SEC(F.sec, F.nt, fwd_sec, double, double, sc, F, F.ls.v64, F.d64, r, (double *)y, S,
+      DISPATCH_SEC(F.sec, F.seco, F.nt, fwd_sec, double, double, sc, F, F.ls.v64, F.d64, r, (double *)y, S,
                    st, fb ? F.us.v64 : nullptr, (double *)z_fused, ap, xp, ru_count);
     return;
   }
@@ -1638,10 +1643,10 @@ void launch_bwd(const DevSched &sc, const Factor &F, bool mixed, const void *y, 
       slot0 = skip ? slice_grid(sc.h_color_blk_v[cl + 1] - sc.h_color_blk_v[cl]) : 0;
     }
     if (mixed)
-      DISPATCH_SEC(F.sec, F.nt, bwd_sec, float, float, sc, F, F.us.v32, F.d32, (const float *)y,
+      DISPATCH_SEC(F.sec, F.seco, F.nt, bwd_sec, float, float, sc, F, F.us.v32, F.d32, (const float *)y,
                    (float *)z, r, Sw, pz, rz, st, skip, slot0);
     else
-      DISPATCH_SEC(F.sec, F.nt, bwd_sec, double, double, sc, F, F.us.v64, F.d64, (const double *)y,
+      DISPATCH_SEC(F.sec, F.seco, F.nt, bwd_sec, double, double, sc, F, F.us.v64, F.d64, (const double *)y,
                    (double *)z, r, Sw, pz, rz, st, skip, slot0);
     return;
   }

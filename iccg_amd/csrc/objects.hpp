@@ -109,7 +109,7 @@ inline void pair_slices(const std::vector<int> &off, std::vector<T> &a) {
   }
 }
 
-// Host layout of a 64-row sliced ELL over row ranges [beg(i), end(i)) of a// Host layout of a 64-row sliced ELL over row ranges [beg(i), end(i)) of a
+// Host layout of a 64-row sliced ELL over row ranges [beg(i), end(i)) of a
 // CSR-like index array ci: off[nslice+1], col (padding -1), and for every
 // included CSR position k the ELL position (pos[k]).
 template <class Range>

@@ -128,6 +128,34 @@ def test_batch_fp64_matches_oracle_and_continuous_admission():
     _check_same(B, [s], [(r1, x1)])
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_bfs_blocks_with_boxes(precision):
+    """Breadth-first ABMC blocks (up to six outside entries per row in the
+    sectioned layout) beside 4x4x4-box systems (three) in one batch: every
+    system gives its single-system iterates, and in FP64 the double-double
+    oracle's."""
+    systems = []
+    for k, dims in enumerate([(20, 18, 16), (16, 16, 16), (60, 50, 1), (24, 20, 12)]):
+        A = I.gen_grid(*dims, coef_exp=(-1.0, 1.0), seed=700 + k)
+        o = (P.abmc_order(A, block=64) if k % 2 == 0 else
+             P.abmc_order(A, grid_dims=dims, box=(4, 4, 4)))
+        systems.append((o, I.gen_normal(A.n, 710 + k)))
+    Ms = [I.IC0(o.A, 0.0, sched=o.sched, precision=precision) for o, _ in systems]
+    assert [M.plan() for M in Ms] == [("abmc-sectioned-fused-r", w) for w in (3, 3, 2, 3)]
+    alone = [_alone(o, b, precision) for o, b in systems]
+    B = I.Batch(len(systems))
+    slots = [B.add(o.A, M, b) for (o, b), M in zip(systems, Ms)]
+    assert B.run() == 0
+    _check_same(B, slots, alone)
+    if precision == "fp64":
+        for s, (o, b) in zip(slots, systems):
+            Ao = P.orc(o.A)
+            dd = O.cg(Ao, b, "ic0", ic=O.ic0_factor(Ao), sched=P.orc_sched(o.sched), tol=1e-8,
+                      dot_mode=1)
+            assert B.result(s)[1].iters == dd.iters
+            assert np.array_equal(B.x(s), dd.x)
+
+
 def test_batch_remove_while_running():
     systems = [_system(n, n, 1, 600 + n) for n in (64, 48, 56)]
     B = I.Batch(3)

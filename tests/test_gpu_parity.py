@@ -157,6 +157,43 @@ def test_abmc_large_blocks(B, dims):
     assert np.array_equal(x32, m.x)
 
 
+@pytest.mark.parametrize("dims", [(24, 22, 20), (100, 100, 1)])
+def test_abmc_bfs_blocks_sectioned(dims):
+    """Breadth-first blocks of at most 64 rows with single-row fragments
+    among them: rows of later colors have up to six (3D) or four (2D)
+    earlier-color neighbours, so the sectioned layout keeps three (3D) or
+    two (2D) in-block and six outside slots per row.  The factor, the FP64
+    and FP32 applies, and FP64 and mixed CG are each bitwise equal to the
+    oracle."""
+    A = P.grid(*dims, coef=(-2.0, 2.0), shift=1e-6)
+    o = P.abmc_order(A, block=64)
+    sizes = np.diff(o.sched.block_ptr)
+    assert sizes.max() == 64 and sizes.min() == 1
+    Ao = P.orc(o.A)
+    f = O.ic0_factor(Ao, 0.0)
+    M = I.IC0(o.A, 0.0, sched=o.sched)
+    plan = ("abmc-sectioned-fused-r", 3 if dims[2] > 1 else 2)
+    assert M.plan() == plan
+    _factor_equal(M, f)
+    r = I.gen_normal(A.n, 8)
+    assert np.array_equal(M.apply(r, o.sched), O.ic0_apply(f, r, None))
+    b = I.gen_normal(A.n, 9)
+    x = np.zeros(A.n)
+    res = I.conjugate_gradient(o.A, b, x, M, 10000, 1e-8, o.sched)
+    dd = O.cg(Ao, b, "ic0", ic=f, sched=P.orc_sched(o.sched), tol=1e-8, dot_mode=1)
+    assert res.converged and (res.iters, res.rel_resid) == (dd.iters, dd.rel_resid)
+    assert np.array_equal(x, dd.x)
+    _assert_true_resid(o.A, x, b, res)
+    M32 = I.IC0(o.A, 0.0, sched=o.sched, precision="mixed")
+    assert M32.plan() == plan
+    assert np.array_equal(M32.apply(r, o.sched), O.ic0_apply_f32(f, r))
+    x32 = np.zeros(A.n)
+    r32 = I.conjugate_gradient(o.A, b, x32, M32, 10000, 1e-8, o.sched)
+    m = O.cg_mixed(Ao, b, f, tol=1e-8)
+    assert (r32.iters, r32.restarts, r32.rel_resid) == (m.iters, m.restarts, m.rel_resid)
+    assert np.array_equal(x32, m.x)
+
+
 def test_abmc_grid_boxes():
     A = P.grid(16, 12, 8, coef=(-2.0, 2.0), shift=1e-6)
     o = P.abmc_order(A, grid_dims=(16, 12, 8), box=(4, 4, 4))
