@@ -863,14 +863,15 @@ int iccg_csr_create(int n, long nnz, const int *rowptr, const int *colind, const
         // per-slice table load lengthens a latency-bound chain: measured
         // slower), the latter reading the paired FP64 values
         std::vector<uint16_t> c16;
-        std::vector<int> tbl;
-        if (sell_col16(off, col, c16, tbl)) {
+        std::vector<int> tbl, toff;
+        if (sell_col16(off, col, c16, tbl, toff)) {
           pair_slices(off, c16);
           pair_slices(off, v);
           if (c16.empty()) c16.push_back((uint16_t)0xFFFF);
           if (tbl.empty()) tbl.push_back(0);
           S.col16 = dupload(c16.data(), c16.size(), A->st);
           S.tbl = dupload(tbl.data(), tbl.size(), A->st);
+          if (!toff.empty()) S.toff = dupload(toff.data(), toff.size(), A->st);
         }
         S.v64 = dupload(v.data(), v.size(), A->st);
         HIPCHK(hipStreamSynchronize(A->st));

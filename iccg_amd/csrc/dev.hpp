@@ -52,6 +52,8 @@ enum PrecondKind { PK_IDENT = 0, PK_JAC = 1, PK_IC = 2 };
 // one contiguous 512-byte (f64) run.  A row keeps its stored (ascending
 // column) order; padding carries column -1 and is skipped, so a row sum is
 // the same fma chain as in CSR, bit for bit.
+constexpr int kTbl16Max = 256;  // column blocks a slice may reference (16-bit layout)
+
 struct Sell {
   int n = 0;
   int nslice = 0;
@@ -60,11 +62,14 @@ struct Sell {
   int *col = nullptr;    // off[nslice], -1 = padding
   double *v64 = nullptr;
   float *v32 = nullptr;
-  // 16-bit columns, when every slice references at most 16 distinct 64-row
-  // column blocks: tbl[16 s + t] = t-th block of slice s; an entry is
-  // (t << 6 | column & 63), padding 0xFFFF.  Half the index bytes.
+  // 16-bit columns, when every slice references at most kTbl16Max distinct
+  // 64-row column blocks; an entry is (t << 6 | column & 63), padding
+  // 0xFFFF, with t the column's block in the slice's table: tbl[16 s + t]
+  // when every slice references at most 16 blocks (toff == nullptr), else
+  // tbl[toff[s] + t], t < toff[s + 1] - toff[s].  Half the index bytes.
   uint16_t *col16 = nullptr;
   int *tbl = nullptr;
+  int *toff = nullptr;
   // sectioned layout (slice = ABMC block b): inb[64 b + lane], byte k = the
   // row within the block of the lane's k-th in-block entry, 64 when absent
   unsigned *inb = nullptr;

@@ -363,17 +363,19 @@ __device__ __forceinline__ AT sell_apply_sel(const int (&cc)[WM], const VT (&vv)
 typedef double dbl2_t __attribute__((ext_vector_type(2)));
 template <int WM>
 struct Slice16 {
-  int w, te;
+  int w, te, ts, tn;  // te: table entry `lane` of the tn at tbl + ts
   unsigned cp[WM / 2];
   dbl2_t vp[WM / 2];
   unsigned ct;
   double vt;
 };
 
-template <int WM>
+// tbl / toff: Sell's tables (TV: toff is set; else 16 entries per slice)
+template <int WM, bool TV>
 __device__ __forceinline__ void spmv16_load(const int *__restrict__ off,
                                             const uint16_t *__restrict__ col16,
                                             const int *__restrict__ tbl,
+                                            const int *__restrict__ toff,
                                             const double *__restrict__ v, int s, int lane,
                                             Slice16<WM> &q) {
   constexpr int WP = WM / 2;
@@ -381,7 +383,15 @@ __device__ __forceinline__ void spmv16_load(const int *__restrict__ off,
   const int w = (off[s + 1] - o0) >> 6;
   const int np = w >> 1;
   q.w = w;
-  q.te = tbl[16 * s + (lane & 15)];
+  if (TV) {
+    q.ts = toff[s];
+    q.tn = toff[s + 1] - q.ts;
+    q.te = lane < q.tn ? tbl[q.ts + lane] : 0;
+  } else {
+    q.ts = 16 * s;
+    q.tn = 16;
+    q.te = tbl[16 * s + (lane & 15)];
+  }
   const unsigned *c32 = reinterpret_cast<const unsigned *>(col16 + o0);  // o0 is a multiple of 64
   const dbl2_t *v2 = reinterpret_cast<const dbl2_t *>(v + o0);
 #pragma unroll
@@ -401,21 +411,32 @@ __device__ __forceinline__ void spmv16_load(const int *__restrict__ off,
   }
 }
 
-template <int WM>
+// tb: the wavefront's LDS slots for the table (kTbl16Max with TV, else 16);
+// TV: Sell::toff is set (a slice's table beyond its first 64 entries is
+// read here)
+template <int WM, bool TV>
 __device__ __forceinline__ double spmv16_finish(const Slice16<WM> &q, int *tb,
+                                                const int *__restrict__ tbl,
                                                 const double *__restrict__ x, int lane) {
   constexpr int WP = WM / 2;
   if (q.w == 0) return 0.0;  // a slice of empty rows (uniform)
+  // index bits of padding (0xFFFF) select a valid slot; its value is unused
+  constexpr unsigned tm = TV ? kTbl16Max - 1u : 15u;
   wave_sync();  // the previous slice's table reads are issued
-  tb[lane & 15] = q.te;
+  if (TV) {
+    tb[lane] = q.te;
+    for (int k = 64; k < q.tn; k += 64) tb[k + lane] = k + lane < q.tn ? tbl[q.ts + k + lane] : 0;
+  } else {
+    tb[lane & 15] = q.te;
+  }
   wave_sync();
   int cl[WP], ch[WP], t0[WP], t1[WP];
 #pragma unroll
   for (int j = 0; j < WP; ++j) {
-    t0[j] = tb[(q.cp[j] >> 6) & 15u];
-    t1[j] = tb[(q.cp[j] >> 22) & 15u];
+    t0[j] = tb[(q.cp[j] >> 6) & tm];
+    t1[j] = tb[(q.cp[j] >> 22) & tm];
   }
-  const int tt = tb[(q.ct >> 6) & 15u];
+  const int tt = tb[(q.ct >> 6) & tm];
   // every lane reads its table entries, all issued before the first is used
 #pragma unroll
   for (int j = 0; j < WP; ++j) asm volatile("" : "+v"(t0[j]), "+v"(t1[j]));

@@ -39,12 +39,13 @@ __device__ __forceinline__ double row_dot(const int *__restrict__ rp, const int 
 
 // Visits the rows of A: sliced ELL (one wavefront per 64-row slice) or CSR.
 // f(i, Ax_i) for every row i < n; lanes past n get no call.
-template <int WM, class F>
+// TV: the 16-bit tables have per-slice lengths (Sell::toff set).
+template <int WM, bool TV, class F>
 __device__ __forceinline__ void for_rows_Ax(const DevCsr &A, const double *__restrict__ x, F f) {
   if (WM > 0 && A.sell.off && A.sell.col16) {
     // 16-bit columns: the values and columns are in the paired layout
     constexpr int K = WM > 0 ? WM : 2;
-    __shared__ int tbs[kWG >> 6][16];
+    __shared__ int tbs[kWG >> 6][TV ? kTbl16Max : 16];
     const int lane = threadIdx.x & 63;
     int *tb = tbs[threadIdx.x >> 6];
     const WaveRange wr = xcd_waves(0, A.sell.nslice);
@@ -54,12 +55,14 @@ __device__ __forceinline__ void for_rows_Ax(const DevCsr &A, const double *__res
     if (first < end) {
       // the next slice's loads go out before the current slice's gathers
       Slice16<K> cur;
-      spmv16_load<K>(A.sell.off, A.sell.col16, A.sell.tbl, A.sell.v64, first, lane, cur);
+      spmv16_load<K, TV>(A.sell.off, A.sell.col16, A.sell.tbl, A.sell.toff, A.sell.v64, first,
+                         lane, cur);
       for (int s = first; s < end; s += step) {
         Slice16<K> nxt = cur;
         if (s + step < end)
-          spmv16_load<K>(A.sell.off, A.sell.col16, A.sell.tbl, A.sell.v64, s + step, lane, nxt);
-        const double y = spmv16_finish<K>(cur, tb, x, lane);
+          spmv16_load<K, TV>(A.sell.off, A.sell.col16, A.sell.tbl, A.sell.toff, A.sell.v64,
+                             s + step, lane, nxt);
+        const double y = spmv16_finish<K, TV>(cur, tb, A.sell.tbl, x, lane);
         const int i = (s << 6) + lane;
         if (i < A.n) f(i, y);
         cur = nxt;
@@ -80,21 +83,21 @@ __device__ __forceinline__ void for_rows_Ax(const DevCsr &A, const double *__res
   }
 }
 
-template <int WM>
+template <int WM, bool TV>
 __global__ __launch_bounds__(kWG) void k_spmv_plain(DevCsr A, const double *__restrict__ x,
                                                     double *__restrict__ y) {
-  for_rows_Ax<WM>(A, x, [&](int i, double v) { y[i] = v; });
+  for_rows_Ax<WM, TV>(A, x, [&](int i, double v) { y[i] = v; });
 }
 
 // Ap = A p with p.Ap, ||p||^2, ||Ap||^2; the last workgroup applies the
 // breakdown test (cg_crs.cpp:127-133) and computes alpha (135).
-template <int WM>
+template <int WM, bool TV>
 __global__ __launch_bounds__(kWG) void k_spmv_cg(DevCsr A, const double *__restrict__ p,
                                                  double *__restrict__ Ap, Scal *S,
                                                  double *part) {
   if (S->done) return;
   double h[3] = {0, 0, 0}, l[3] = {0, 0, 0};
-  for_rows_Ax<WM>(A, p, [&](int i, double y) {
+  for_rows_Ax<WM, TV>(A, p, [&](int i, double y) {
     __builtin_nontemporal_store(y, &Ap[i]);
     const double pi = p[i];
     dd_fma(h[0], l[0], pi, y);
@@ -129,13 +132,13 @@ __global__ __launch_bounds__(kFinWG) void k_fin_alpha(const double *__restrict__
 }
 
 // r = b - A x (cg_crs.cpp:102-110, 118-123; true residual 517-528)
-template <int WM>
+template <int WM, bool TV>
 __global__ __launch_bounds__(kWG) void k_resid(DevCsr A, const double *__restrict__ x,
                                                const double *__restrict__ b,
                                                double *__restrict__ r, Scal *S, double *part,
                                                int mode, double tol, int max_iter) {
   double h[2] = {0, 0}, l[2] = {0, 0};
-  for_rows_Ax<WM>(A, x, [&](int i, double ax) {
+  for_rows_Ax<WM, TV>(A, x, [&](int i, double ax) {
     const double bi = b[i];
     const double ri = bi - ax;
     if (mode != 0) r[i] = ri;
@@ -1061,25 +1064,36 @@ static int ax_grid(const DevCsr &A) {
   return A.sell.off ? slice_grid(A.sell.nslice) : red_grid((A.n + kWG - 1) / kWG);
 }
 
-#define DISPATCH_WM(wm, KERNEL, ...)                                  \
-  do {                                                                \
-    switch (wm) {                                                     \
-      case 4: hipLaunchKernelGGL(KERNEL<4>, __VA_ARGS__); break;      \
-      case 8: hipLaunchKernelGGL(KERNEL<8>, __VA_ARGS__); break;      \
-      case 16: hipLaunchKernelGGL(KERNEL<16>, __VA_ARGS__); break;    \
-      default: hipLaunchKernelGGL(KERNEL<0>, __VA_ARGS__); break;     \
-    }                                                                 \
+// KERNEL<row width class, 16-bit tables of per-slice lengths> for A
+#define DISPATCH_WM(A, KERNEL, ...)                                              \
+  do {                                                                           \
+    const bool _tv = (A).sell.col16 && (A).sell.toff;                            \
+    switch (wm_of((A).sell.maxw)) {                                              \
+      case 4:                                                                    \
+        if (_tv) hipLaunchKernelGGL((KERNEL<4, true>), __VA_ARGS__);             \
+        else hipLaunchKernelGGL((KERNEL<4, false>), __VA_ARGS__);                \
+        break;                                                                   \
+      case 8:                                                                    \
+        if (_tv) hipLaunchKernelGGL((KERNEL<8, true>), __VA_ARGS__);             \
+        else hipLaunchKernelGGL((KERNEL<8, false>), __VA_ARGS__);                \
+        break;                                                                   \
+      case 16:                                                                   \
+        if (_tv) hipLaunchKernelGGL((KERNEL<16, true>), __VA_ARGS__);            \
+        else hipLaunchKernelGGL((KERNEL<16, false>), __VA_ARGS__);               \
+        break;                                                                   \
+      default: hipLaunchKernelGGL((KERNEL<0, false>), __VA_ARGS__); break;       \
+    }                                                                            \
   } while (0)
 
 void launch_spmv_plain(const DevCsr &A, const double *x, double *y, hipStream_t st) {
   if (A.n == 0) return;
-  DISPATCH_WM(wm_of(A.sell.maxw), k_spmv_plain, dim3(ax_grid(A)), dim3(kWG), 0, st, A, x, y);
+  DISPATCH_WM(A, k_spmv_plain, dim3(ax_grid(A)), dim3(kWG), 0, st, A, x, y);
 }
 
 void launch_spmv_cg(const DevCsr &A, const double *p, double *Ap, Scal *S, double *part,
                     hipStream_t st, bool finish) {
   const int g = ax_grid(A);
-  DISPATCH_WM(wm_of(A.sell.maxw), k_spmv_cg, dim3(g), dim3(kWG), 0, st, A, p, Ap, S, part);
+  DISPATCH_WM(A, k_spmv_cg, dim3(g), dim3(kWG), 0, st, A, p, Ap, S, part);
   if (finish) launch_fin_alpha(A, S, part, st);
 }
 
@@ -1090,7 +1104,7 @@ void launch_fin_alpha(const DevCsr &A, Scal *S, double *part, hipStream_t st) {
 void launch_resid(const DevCsr &A, const double *x, const double *b, double *r, Scal *S,
                   double *part, int mode, double tol, int max_iter, hipStream_t st) {
   const int g = ax_grid(A);
-  DISPATCH_WM(wm_of(A.sell.maxw), k_resid, dim3(g), dim3(kWG), 0, st, A, x, b, r, S, part, mode,
+  DISPATCH_WM(A, k_resid, dim3(g), dim3(kWG), 0, st, A, x, b, r, S, part, mode,
               tol, max_iter);
   hipLaunchKernelGGL(k_fin_resid, dim3(1), dim3(kFinWG), 0, st, part, g, S, mode, tol, max_iter);
 }

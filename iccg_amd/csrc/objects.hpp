@@ -52,37 +52,58 @@ inline void free_sell(Sell &S) {
   dfree(S.v32);
   dfree(S.col16);
   dfree(S.tbl);
+  dfree(S.toff);
   dfree(S.inb);
   S = Sell();
 }
 
-// The 16-bit column layout of a sliced ELL (see Sell): false, and nothing
-// built, when some slice references more than 16 distinct 64-row blocks.
+// The 16-bit column layout of a sliced ELL (see Sell): tables of 16 entries
+// per slice, or when some slice references more than 16 distinct 64-row
+// blocks, of each slice's own length with offsets toff; false, and nothing
+// built, when some slice references more than kTbl16Max.
 inline bool sell_col16(const std::vector<int> &off, const std::vector<int> &col,
-                       std::vector<uint16_t> &c16, std::vector<int> &tbl) {
+                       std::vector<uint16_t> &c16, std::vector<int> &tbl,
+                       std::vector<int> &toff) {
   const int ns = (int)off.size() - 1;
+  int maxc = -1;
+  for (int c : col) maxc = std::max(maxc, c);
+  // a slice's blocks in order of first reference: ids[first[s] ...]; a
+  // block's index in the current slice's table: at[b] when seen[b] == s
+  std::vector<int> ids, first((size_t)std::max(ns, 0) + 1, 0);
+  std::vector<int> seen((size_t)(maxc >> 6) + 1, -1), at((size_t)(maxc >> 6) + 1, 0);
   c16.assign(col.size(), (uint16_t)0xFFFF);
-  tbl.assign((size_t)16 * std::max(ns, 0), 0);
+  int most = 0;
   for (int s = 0; s < ns; ++s) {
-    int t[16], nt = 0;
+    first[(size_t)s] = (int)ids.size();
+    int nt = 0;
     for (int q = off[s]; q < off[s + 1]; ++q) {
       const int c = col[(size_t)q];
       if (c < 0) continue;
       const int b = c >> 6;
-      int x = 0;
-      while (x < nt && t[x] != b) ++x;
-      if (x == nt) {
-        if (nt == 16) {
+      if (seen[(size_t)b] != s) {
+        if (nt == kTbl16Max) {
           c16.clear();
-          tbl.clear();
           return false;
         }
-        t[nt++] = b;
+        seen[(size_t)b] = s;
+        at[(size_t)b] = nt++;
+        ids.push_back(b);
       }
-      c16[(size_t)q] = (uint16_t)((x << 6) | (c & 63));
+      c16[(size_t)q] = (uint16_t)((at[(size_t)b] << 6) | (c & 63));
     }
-    for (int x = 0; x < 16; ++x) tbl[(size_t)16 * s + x] = x < nt ? t[x] : 0;
+    most = std::max(most, nt);
   }
+  first[(size_t)ns] = (int)ids.size();
+  toff.clear();
+  if (most > 16) {
+    tbl = ids;
+    toff = first;
+    return true;
+  }
+  tbl.assign((size_t)16 * std::max(ns, 0), 0);
+  for (int s = 0; s < ns; ++s)
+    for (int x = first[(size_t)s]; x < first[(size_t)s + 1]; ++x)
+      tbl[(size_t)16 * s + (x - first[(size_t)s])] = ids[(size_t)x];
   return true;
 }
 

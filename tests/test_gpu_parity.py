@@ -35,21 +35,27 @@ def test_spmv_bitwise(name):
     assert np.array_equal(y, O.spmv(P.orc(A), x))
 
 
-@pytest.mark.parametrize("spread", [3, 40])
-def test_spmv_16bit_columns_and_fallback(spread):
-    """The SpMV reads 16-bit columns (per-slice tables of at most 16 target
-    64-row blocks) when every slice fits; rows whose entries reach 40
-    different blocks per slice keep 32-bit columns.  Both bitwise."""
-    rng = np.random.default_rng(spread)
-    n = 64 * 48
+@pytest.mark.parametrize("nblk,per_row,most", [(12, 3, 12), (48, 8, 48), (100, 8, 100)])
+def test_spmv_16bit_columns_and_fallback(nblk, per_row, most):
+    """The SpMV reads 16-bit columns with a per-slice table of the 64-row
+    column blocks its entries reach: 16 entries when every slice reaches at
+    most 16 blocks, 64 when every slice reaches at most 64; beyond that the
+    columns stay 32-bit.  Rows here reach per_row random blocks of the first
+    nblk, so every slice reaches at most (and here exactly) `most` of them.
+    Every case bitwise."""
+    rng = np.random.default_rng(nblk)
+    n = 64 * max(nblk, 48)
     rows, cols = [], []
     for i in range(n):
-        blocks = rng.choice(48, size=min(spread, 8), replace=False)
+        blocks = rng.choice(nblk, size=per_row, replace=False)
         js = np.unique(blocks * 64 + rng.integers(0, 64, size=len(blocks)))
         rows += [i] * len(js)
         cols += list(js)
+    rows, cols = np.array(rows), np.array(cols)
+    reach = [len(np.unique(cols[rows // 64 == s] // 64)) for s in range(n // 64)]
+    assert max(reach) == most
     v = rng.normal(size=len(rows))
-    A = I.coo_to_crs(n, np.array(rows), np.array(cols), v)
+    A = I.coo_to_crs(n, rows, cols, v)
     x = I.gen_normal(A.n, 12)
     assert np.array_equal(I.DeviceCSR(A).spmv(x), O.spmv(P.orc(A), x))
 
