@@ -641,12 +641,82 @@ def main_serve(args):
     print(json.dumps(line), flush=True)
 
 
+def main_multi(args):
+    """Several right-hand sides of one system: the multi-right-hand-side
+    solver (each right-hand side its own CG recurrence, the operator read
+    once per set of up to four) against the same right-hand sides solved one
+    after another by the single-system solver, both mixed precision with the
+    FP64 true-residual stop, on the headline system (or --dims 2)."""
+    import iccg_amd as I
+    N, K = args.size, args.nrhs
+    Ap, sched, b, info = build_problem(N, args.block, args.dims)
+    n, nnz = Ap.n, Ap.nnz
+    B = np.stack([b] + [I.gen_normal(n, SEED + 100 + k) for k in range(1, K)])
+    dA = I.DeviceCSR(Ap)
+    M = I.IC0(dA, 0.0, sched=sched, precision=args.precision)
+    true_res = args.precision == "mixed"
+    MS = I.MultiSolver(dA, M, K)
+    MS.set(B)
+    for _ in range(args.warmup):
+        MS.run(args.max_iter, args.tol, true_residual=true_res, zero_x0=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = MS.run(args.max_iter, args.tol, true_residual=true_res, zero_x0=True)
+    wall_m = (time.perf_counter() - t0) / args.steps
+    X = MS.x()
+    for r in res:
+        check_solves("multi", [r], args.tol, true_resid=True)
+    log(f"multi ({K} right-hand sides, layout {MS.layout()}): {wall_m * 1e3:.1f} ms, "
+        f"iters {[r.iters for r in res]}")
+    del MS
+    S = I.Solver(dA, M)
+    single, xs, wall_s = [], [], 0.0
+    for k in range(K):
+        S.set(B[k])
+        for _ in range(args.warmup):
+            S.run(args.max_iter, args.tol, true_residual=true_res, zero_x0=True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            r = S.run(args.max_iter, args.tol, true_residual=true_res, zero_x0=True)
+        wall_s += (time.perf_counter() - t0) / args.steps
+        single.append(r)
+        xs.append(S.x())
+    same = all((a.iters, a.rel_resid, a.restarts) == (c.iters, c.rel_resid, c.restarts)
+               for a, c in zip(res, single)) and np.array_equal(X, np.stack(xs))
+    if not same:
+        refuse("the multi-right-hand-side solves differ from the single-system ones")
+    log(f"one at a time: {wall_s * 1e3:.1f} ms for {K} right-hand sides "
+        f"({wall_m and wall_s / wall_m:.2f}x); results bitwise equal")
+    line = {
+        "metric": "right-hand sides of one system solved to tol per second",
+        "value": round(K / wall_m, 3), "unit": "right-hand sides/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall_m * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": ("f64 CG + f32 preconditioner" if true_res else "f64"),
+        "data": "synthetic",
+        "config": {"workload": (f"{K} right-hand sides of {args.dims}D "
+                                f"{2 * args.dims + 1}-pt variable-coefficient diffusion "
+                                f"{N}^{args.dims} (n={n}, nnz={nnz}), ABMC B={args.block} "
+                                f"({info['nb']} blocks, {info['nc']} colors), b~N(0,1), "
+                                f"tol {args.tol:g}"), "n": n, "nnz": nnz, "nrhs": K},
+        "iters": [r.iters for r in res],
+        "one_at_a_time": {"value": round(K / wall_s, 3), "ms": round(wall_s * 1e3, 3)},
+        "speedup_vs_one_at_a_time": round(wall_s / wall_m, 3),
+        "bitwise_equal_to_one_at_a_time": bool(same),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["abmc192", "mc128", "batch", "serve"],
+    ap.add_argument("--workload", choices=["abmc192", "mc128", "batch", "serve", "multi"],
                     default="abmc192",
                     help="abmc192: the single-system headline; mc128: red-black FP64 at 128^3; "
-                         "batch: many systems at once; serve: the request loop")
+                         "batch: many systems at once; serve: the request loop; multi: several "
+                         "right-hand sides of the abmc192 system (--nrhs)")
+    ap.add_argument("--nrhs", type=int, default=4)
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
+                    help="multi workload: the preconditioner's precision")
     ap.add_argument("--systems", type=int, default=512)
     ap.add_argument("--requests", type=int, default=64)
     ap.add_argument("--no-compare", action="store_true",
@@ -691,6 +761,8 @@ def main():
         return main_serve(args)
     if args.workload == "mc128":
         return main_mc128(args)
+    if args.workload == "multi":
+        return main_multi(args)
     N = args.size
     Ap, sched, b, info = build_problem(N, args.block, args.dims)
     n, nnz = Ap.n, Ap.nnz

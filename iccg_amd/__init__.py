@@ -4,7 +4,8 @@ The compute lives in libiccg_hip.so (hand-written HIP kernels behind a C
 ABI, include/iccg_hip.h); this package mirrors the stomo0511/ICCG interface
 on top of it.
 """
-from .api import (CRS, CGResult, Batch, ColorSchedule, DeviceCSR, IC0, Identity, Jacobi, Server, Solver,
+from .api import (CRS, CGResult, Batch, ColorSchedule, DeviceCSR, IC0, Identity, Jacobi,
+                  MultiSolver, Server, Solver, conjugate_gradient_multi,
                   BuildABMCSchedule, BuildColorPtrRows_FromPermutationAndColor,
                   BuildPermutationByBlockColor, BuildPermutationByColor,
                   Permute_PtAP_LowerCRS_to_LowerCRS, ReadBlockColorFile_1Based,

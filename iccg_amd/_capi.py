@@ -111,6 +111,13 @@ _SIGS = {
     "iccg_solver_get_timing": (_i, [_vp, _i, _lp, _dp]),
     "iccg_solver_set_graphs": (_i, [_vp, _i]),
     "iccg_solver_reset_timing": (_i, [_vp]),
+    "iccg_msolver_create": (_i, [_vp, _vp, _i, C.POINTER(_vp)]),
+    "iccg_msolver_destroy": (_i, [_vp]),
+    "iccg_msolver_native": (_i, [_vp, _ip, _ip]),
+    "iccg_msolver_set": (_i, [_vp, _vp, _vp]),
+    "iccg_msolver_run": (_i, [_vp, _i, _d, _i, _vp]),
+    "iccg_msolver_get_x": (_i, [_vp, _vp]),
+    "iccg_cg_solve_multi": (_i, [_vp, _vp, _i, _vp, _vp, _i, _d, _i, _vp]),
     "iccg_bench_copy": (_i, [_l, _i, _dp]),
     "iccg_batch_create": (_i, [_i, C.POINTER(_vp)]),
     "iccg_batch_destroy": (_i, [_vp]),

@@ -538,6 +538,72 @@ class Solver:
         return n.value, ms.value
 
 
+class MultiSolver:
+    """Several right-hand sides of one system solved together
+    (iccg_msolver_*): each keeps its own CG recurrence and gets the iterates
+    Solver gives it alone, bit for bit, while the kernels read each matrix
+    and factor entry once for up to four of them.  b, x0 and x() are
+    (nrhs, n) arrays."""
+
+    def __init__(self, A, M, nrhs):
+        self.A = _device_csr(A)
+        self.M = M
+        self.nrhs = int(nrhs)
+        h = _vp()
+        check(lib().iccg_msolver_create(self.A.handle, M.handle, self.nrhs, C.byref(h)))
+        self._h = _Handle(h, lib().iccg_msolver_destroy)
+        self.n = self.A.n
+
+    def layout(self):
+        """(shared-kernel sets, right-hand sides on the single-system solver)"""
+        s, l = C.c_int(0), C.c_int(0)
+        check(lib().iccg_msolver_native(self._h.h, C.byref(s), C.byref(l)))
+        return s.value, l.value
+
+    def _rhs(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != (self.nrhs, self.n):
+            raise ValueError(f"expected shape {(self.nrhs, self.n)}, got {a.shape}")
+        return a
+
+    def set(self, b, x0=None):
+        b = self._rhs(b)
+        x0 = None if x0 is None else self._rhs(x0)
+        check(lib().iccg_msolver_set(self._h.h, _p(b), _p(x0)))
+
+    def run(self, max_iter, tol, true_residual=None, zero_x0=False):
+        """Returns one CGResult per right-hand side."""
+        if true_residual is None:
+            true_residual = self.M.precision == _capi.MIXED
+        flags = (_capi.SOLVE_TRUE_RESID if true_residual else 0) | \
+            (_capi.SOLVE_ZERO_X0 if zero_x0 else 0)
+        res = (_capi.CGResultC * self.nrhs)()
+        check(lib().iccg_msolver_run(self._h.h, max_iter, tol, flags, res))
+        return [_result(r) for r in res]
+
+    def x(self):
+        out = np.zeros((self.nrhs, self.n))
+        check(lib().iccg_msolver_get_x(self._h.h, _p(out)))
+        return out
+
+
+def conjugate_gradient_multi(A, B, X, M, max_iter, tol, *, true_residual=None):
+    """conjugate_gradient for the rows of B (nrhs, n) at once
+    (iccg_cg_solve_multi); X (nrhs, n) float64 is updated in place."""
+    dA = _device_csr(A)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    if (not isinstance(X, np.ndarray) or X.dtype != np.float64 or not X.flags.c_contiguous
+            or X.shape != B.shape or B.ndim != 2):
+        raise TypeError("X must be a contiguous float64 (nrhs, n) array shaped like B")
+    if true_residual is None:
+        true_residual = M.precision == _capi.MIXED
+    flags = _capi.SOLVE_TRUE_RESID if true_residual else 0
+    res = (_capi.CGResultC * B.shape[0])()
+    check(lib().iccg_cg_solve_multi(dA.handle, M.handle, B.shape[0], _p(B), _p(X), max_iter,
+                                    tol, flags, res))
+    return [_result(r) for r in res]
+
+
 def device_count():
     c = C.c_int(0)
     check(lib().iccg_device_count(C.byref(c)))

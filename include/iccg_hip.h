@@ -56,6 +56,7 @@ typedef struct iccg_csr_s *iccg_csr_t;
 typedef struct iccg_sched_s *iccg_sched_t;
 typedef struct iccg_precond_s *iccg_precond_t;
 typedef struct iccg_solver_s *iccg_solver_t;
+typedef struct iccg_msolver_s *iccg_msolver_t;
 
 /* CGResult (cg_crs.cpp:82-86) plus what the GPU path reports beside it */
 typedef struct {
@@ -292,6 +293,33 @@ int iccg_solver_set_graphs(iccg_solver_t S, int on);
 int iccg_solver_get_timing(iccg_solver_t S, int cls, long *launches,
                            double *total_ms);
 int iccg_solver_reset_timing(iccg_solver_t S);
+
+/* ---- several right-hand sides of one system (no reference counterpart) -----
+ * The north star's multi-RHS pattern: nrhs right-hand sides of one matrix and
+ * preconditioner solved together.  Each right-hand side keeps its own CG
+ * recurrence (cg_crs.cpp:89-162: its own alpha, beta, stop test, iteration
+ * count and true-residual restarts) and its iterates are those of
+ * iccg_solver_run on it alone, bit for bit; the kernels read each matrix
+ * and factor entry once for up to four of them (vectors interleaved in
+ * HBM).  The shared kernels run on the sectioned ABMC plan (box and
+ * breadth-first blockings of at most 64 rows); on other plans, and for a
+ * right-hand side left over from the sets of four (or two), the
+ * single-system solver runs them one after another.  b, x0 and x hold nrhs
+ * vectors of n entries (b[k*n + i]); res receives nrhs results. */
+int iccg_msolver_create(iccg_csr_t A, iccg_precond_t M, int nrhs, iccg_msolver_t *out);
+int iccg_msolver_destroy(iccg_msolver_t S);
+/* sets: shared-kernel sets; lone: right-hand sides on the single-system solver */
+int iccg_msolver_native(iccg_msolver_t S, int *sets, int *lone);
+/* Uploads b and x0 (NULL x0 -> zeros). */
+int iccg_msolver_set(iccg_msolver_t S, const double *b, const double *x0);
+/* The CG phase of every right-hand side (flags as iccg_solver_run). */
+int iccg_msolver_run(iccg_msolver_t S, int max_iter, double tol, int flags,
+                     iccg_cg_result *res);
+int iccg_msolver_get_x(iccg_msolver_t S, double *x);
+/* create + set + run + get_x + destroy on host buffers (x: x0 in, solution out) */
+int iccg_cg_solve_multi(iccg_csr_t A, iccg_precond_t M, int nrhs, const double *b,
+                        double *x, int max_iter, double tol, int flags,
+                        iccg_cg_result *res);
 
 /* ---- many systems at once (no reference counterpart) -----------------------
  * Independent systems, each with its own matrix, IC0 preconditioner and
