@@ -601,7 +601,8 @@ def main_serve(args):
         return d
 
     served_mode = I.Server("batched", precision="mixed", max_slots=args.slots,
-                           workers=args.workers, hbm_budget=int(args.hbm_budget * (1 << 30)))
+                           workers=args.workers, hbm_budget=int(args.hbm_budget * (1 << 30)),
+                           multi_rows=args.multi_rows, multi_runners=args.multi_runners)
     out_b, span_b = drive(served_mode, reqs, arrivals)
     st = served_mode.stats()
     engine = {k: round(st[k], 1) for k in ("engine_run_ms", "engine_admit_ms",
@@ -631,6 +632,7 @@ def main_serve(args):
                                 f"serves the whole sample, latency = submit to result"),
                    "requests": count, "right_hand_sides": n_rhs, "rate_per_s": args.rate,
                    "max_slots": args.slots, "workers": args.workers,
+                   "multi_rows": args.multi_rows, "multi_runners": args.multi_runners,
                    "hbm_budget_gib": args.hbm_budget},
         "p50_latency_ms": sb["p50_latency_ms"], "p99_latency_ms": sb["p99_latency_ms"],
         "batched_mixed": sb, "one_at_a_time_fp64": ss,
@@ -724,6 +726,10 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0, help="request arrivals per second")
     ap.add_argument("--slots", type=int, default=256)
     ap.add_argument("--workers", type=int, default=12)
+    ap.add_argument("--multi-rows", type=int, default=50000,
+                    help="serve: requests of at least this many rows with 2+ right-hand sides "
+                         "run on the multi-right-hand-side solver beside the batch (0: none)")
+    ap.add_argument("--multi-runners", type=int, default=3)
     ap.add_argument("--hbm-budget", type=float, default=96.0, help="GiB")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)

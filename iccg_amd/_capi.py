@@ -45,7 +45,8 @@ SERVE_BATCHED, SERVE_SEQUENTIAL = 0, 1
 class ServerConfigC(C.Structure):
     _fields_ = [("mode", C.c_int), ("precision", C.c_int), ("max_slots", C.c_int),
                 ("workers", C.c_int), ("block_size", C.c_int), ("flags", C.c_int),
-                ("hbm_budget", C.c_longlong)]
+                ("hbm_budget", C.c_longlong), ("multi_rows", C.c_int),
+                ("multi_runners", C.c_int)]
 
 
 class ServerStatsC(C.Structure):
@@ -117,6 +118,9 @@ _SIGS = {
     "iccg_msolver_set": (_i, [_vp, _vp, _vp]),
     "iccg_msolver_run": (_i, [_vp, _i, _d, _i, _vp]),
     "iccg_msolver_get_x": (_i, [_vp, _vp]),
+    "iccg_msolver_set_timing": (_i, [_vp, _i]),
+    "iccg_msolver_get_timing": (_i, [_vp, _i, _lp, _dp]),
+    "iccg_msolver_reset_timing": (_i, [_vp]),
     "iccg_cg_solve_multi": (_i, [_vp, _vp, _i, _vp, _vp, _i, _d, _i, _vp]),
     "iccg_bench_copy": (_i, [_l, _i, _dp]),
     "iccg_batch_create": (_i, [_i, C.POINTER(_vp)]),

@@ -586,6 +586,18 @@ class MultiSolver:
         check(lib().iccg_msolver_get_x(self._h.h, _p(out)))
         return out
 
+    def set_timing(self, on=True):
+        check(lib().iccg_msolver_set_timing(self._h.h, int(on)))
+
+    def reset_timing(self):
+        check(lib().iccg_msolver_reset_timing(self._h.h))
+
+    def timing(self, cls):
+        n = C.c_long(0)
+        ms = C.c_double(0)
+        check(lib().iccg_msolver_get_timing(self._h.h, cls, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
 
 def conjugate_gradient_multi(A, B, X, M, max_iter, tol, *, true_residual=None):
     """conjugate_gradient for the rows of B (nrhs, n) at once
@@ -704,19 +716,24 @@ class Server:
     mode "batched": requests are admitted by HBM budget, prepared (ABMC
     ordering, P^T A P, IC(0)) on `workers` host threads and solved with
     continuous batching, every right-hand side a slot of one Batch.
+    Requests of at least `multi_rows` rows with two or more right-hand sides
+    (multi_rows=0: none) are solved instead by `multi_runners` threads with
+    the multi-right-hand-side solver, beside the batch.
     mode "sequential": the plain one-at-a-time path (prepare, then solve each
     right-hand side alone) for comparison.  submit() returns at once; wait()
     and result() collect finished requests."""
 
     def __init__(self, mode="batched", precision="mixed", max_slots=256, workers=8,
-                 block_size=64, true_residual=None, hbm_budget=64 << 30):
+                 block_size=64, true_residual=None, hbm_budget=64 << 30,
+                 multi_rows=50000, multi_runners=3):
         prec = {"fp64": _capi.FP64, "mixed": _capi.MIXED}[precision]
         if true_residual is None:
             true_residual = prec == _capi.MIXED
         cfg = _capi.ServerConfigC(
             {"batched": _capi.SERVE_BATCHED, "sequential": _capi.SERVE_SEQUENTIAL}[mode], prec,
             int(max_slots), int(workers), int(block_size),
-            _capi.SOLVE_TRUE_RESID if true_residual else 0, int(hbm_budget))
+            _capi.SOLVE_TRUE_RESID if true_residual else 0, int(hbm_budget), int(multi_rows),
+            int(multi_runners))
         h = _vp()
         check(lib().iccg_server_create(C.byref(cfg), C.byref(h)))
         self._h = _Handle(h, lib().iccg_server_destroy)

@@ -970,12 +970,17 @@ struct MOps {
   // one CG iteration of every member (the single-system sectioned
   // iteration: SpMV + dots, alpha, forward sweep with the r update, backward
   // sweep with r.z, stop test and beta, x and p update)
+  // mark(c) before each launch group of kernel class c (0 SpMV, 1 forward
+  // sweep, 2 backward sweep, 3 finish and vector work), mark(-1) at the end
+  template <class Mark>
   static void iteration(const iccg_csr_s &A, const iccg_precond_s &M, MSet &s, long nk,
-                        hipStream_t st) {
+                        hipStream_t st, Mark mark) {
     const DevCsr &D = A.d;
     const Factor &F = M.F;
     const int ga = slice_grid(D.sell.nslice);
+    mark(0);
     MDISPATCH_A(D, km_spmv_cg, K, dim3(ga), dim3(kWG), 0, st, D, s.P, s.AP, s.S, s.part);
+    mark(3);
     hipLaunchKernelGGL(km_fin_alpha<K>, dim3(1), dim3(kFinWG), 0, st, s.part, ga, s.S);
     int G = 0;
     const std::vector<int> g = color_groups(M.sc, G);
@@ -986,12 +991,16 @@ struct MOps {
       const VT *lv = M.mixed ? (const VT *)F.ls.v32 : (const VT *)F.ls.v64;
       const VT *uv = M.mixed ? (const VT *)F.us.v32 : (const VT *)F.us.v64;
       const VT *dg = M.mixed ? (const VT *)F.d32 : (const VT *)F.d64;
+      mark(1);
       SWP::fwd(M.sc, F, fb, lv, uv, dg, s.R, (YT *)s.Y, (YT *)s.Z, s.S, s.AP, s.part, st);
+      mark(2);
       SWP::bwd(M.sc, F, fb, uv, dg, (const YT *)s.Y, (YT *)s.Z, s.R, s.S, prz, slot0, st);
+      mark(3);
       hipLaunchKernelGGL(km_fin_stop<K>, dim3(1), dim3(kFinWG), 0, st, s.part, G, prz, G, s.S);
       hipLaunchKernelGGL((km_update_xp<K, YT>), dim3(quad_grid(nk)), dim3(kWG), 0, st, nk, s.X,
                          (const YT *)s.Z, s.P, s.S);
     });
+    mark(-1);
   }
 };
 
@@ -1012,9 +1021,21 @@ struct iccg_msolver_s {
   iccg_solver_t single = nullptr;
   std::vector<double> hb, hx;  // nrhs * n: b and x0 / x of the lone members
   double *stage = nullptr;     // pinned staging, 4 n doubles
+  // per-class event timing (kernel-by-kernel launches while on)
+  bool timing = false;
+  long t_launch[4] = {0, 0, 0, 0};
+  double t_ms[4] = {0, 0, 0, 0};
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;  // events of pool recorded since the last harvest
+  struct Mark {
+    int cls, iter;  // class (-1: end of an iteration), iteration in the chunk
+    hipEvent_t ev;
+  };
+  std::vector<Mark> marks;
   ~iccg_msolver_s() {
     (void)hipSetDevice(device);
     if (st) (void)hipStreamSynchronize(st);
+    for (auto e : pool) (void)hipEventDestroy(e);
     for (auto &s : sets) s.release();
     if (single) iccg_solver_destroy(single);
     if (stage) (void)hipHostFree(stage);
@@ -1029,14 +1050,36 @@ static void mset_sync(iccg_msolver_s &S, MSet &s) {
   HIPCHK(hipStreamSynchronize(S.st));
 }
 
+// k iterations: a replayed hipGraph, or (timing) launched one by one with an
+// event at every class boundary (mset_harvest adds the classes' times up)
 static void mset_iterations(iccg_msolver_s &S, MSet &s, int k) {
   const long nk = (long)S.n * s.K;
+  if (S.timing) {
+    for (int j = 0; j < k; ++j) {
+      auto mark = [&](int c) {
+        if (S.used == S.pool.size()) {
+          hipEvent_t e;
+          HIPCHK(hipEventCreate(&e));
+          S.pool.push_back(e);
+        }
+        hipEvent_t e = S.pool[S.used++];
+        HIPCHK(hipEventRecord(e, S.st));
+        S.marks.push_back({c, j, e});
+      };
+      if (s.K == 4)
+        MOps<4>::iteration(*S.A, *S.M, s, nk, S.st, mark);
+      else
+        MOps<2>::iteration(*S.A, *S.M, s, nk, S.st, mark);
+    }
+    return;
+  }
   auto enqueue = [&] {
+    auto none = [](int) {};
     for (int j = 0; j < k; ++j) {
       if (s.K == 4)
-        MOps<4>::iteration(*S.A, *S.M, s, nk, S.st);
+        MOps<4>::iteration(*S.A, *S.M, s, nk, S.st, none);
       else
-        MOps<2>::iteration(*S.A, *S.M, s, nk, S.st);
+        MOps<2>::iteration(*S.A, *S.M, s, nk, S.st, none);
     }
   };
   if (s.graph_version != S.M->plan_version) {
@@ -1057,6 +1100,22 @@ static void mset_iterations(iccg_msolver_s &S, MSet &s, int k) {
     s.execs.push_back({k, exec});
   }
   HIPCHK(hipGraphLaunch(exec, S.st));
+}
+
+// the timed chunk's class times, for its first `useful` iterations (those in
+// which a member still ran); the stream has been synchronized
+static void mset_harvest(iccg_msolver_s &S, int useful) {
+  for (size_t i = 0; i + 1 < S.marks.size(); ++i) {
+    const auto &m = S.marks[i];
+    if (m.cls < 0 || m.iter >= useful) continue;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, m.ev, S.marks[i + 1].ev) == hipSuccess) {
+      S.t_ms[m.cls] += ms;
+      S.t_launch[m.cls]++;
+    }
+  }
+  S.marks.clear();
+  S.used = 0;
 }
 
 static void mset_resid(iccg_msolver_s &S, MSet &s, int mode, double tol, int max_iter, int mask) {
@@ -1090,15 +1149,23 @@ static void mset_run(iccg_msolver_s &S, MSet &s, int max_iter, double tol, int f
   mset_resid(S, s, 1, tol, max_iter, all);
   mset_precond_init(S, s);
   HIPCHK(hipGetLastError());
+  if (S.timing) mset_sync(S, s);  // iteration counts before the first chunk
   int restarts[4] = {0, 0, 0, 0};
   int cap = S.n < 100000 ? 8 : 16, chunk = cap, issued = 0;
   double rel_prev = -1.0;
   int it_prev = 0;
   for (;;) {
+    int before[4] = {0, 0, 0, 0};
+    for (int m = 0; m < K; ++m) before[m] = s.hS[m].iters;
     mset_iterations(S, s, chunk);
     issued += chunk;
     HIPCHK(hipGetLastError());
     mset_sync(S, s);
+    if (S.timing) {
+      int ran = 0;  // iterations of the chunk in which some member ran
+      for (int m = 0; m < K; ++m) ran = std::max(ran, s.hS[m].iters - before[m]);
+      mset_harvest(S, ran);
+    }
     bool done = true;
     double rel = 0.0;
     int it = 0;
@@ -1288,6 +1355,31 @@ int iccg_msolver_run(iccg_msolver_t S, int max_iter, double tol, int flags,
       ck(iccg_solver_set(S->single, S->hb.data() + k * n, S->hx.data() + k * n));
       ck(iccg_solver_run(S->single, max_iter, tol, flags, &res[k]));
       ck(iccg_solver_get_x(S->single, S->hx.data() + k * n));
+    }
+  });
+}
+
+int iccg_msolver_set_timing(iccg_msolver_t S, int on) {
+  return guard([&] {
+    require(S, ICCG_ERR_ARG, "null argument");
+    S->timing = on != 0;
+  });
+}
+
+int iccg_msolver_get_timing(iccg_msolver_t S, int cls, long *launches, double *total_ms) {
+  return guard([&] {
+    require(S && cls >= 0 && cls < 4 && launches && total_ms, ICCG_ERR_ARG, "bad argument");
+    *launches = S->t_launch[cls];
+    *total_ms = S->t_ms[cls];
+  });
+}
+
+int iccg_msolver_reset_timing(iccg_msolver_t S) {
+  return guard([&] {
+    require(S, ICCG_ERR_ARG, "null argument");
+    for (int c = 0; c < 4; ++c) {
+      S->t_launch[c] = 0;
+      S->t_ms[c] = 0.0;
     }
   });
 }

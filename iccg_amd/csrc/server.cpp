@@ -11,7 +11,11 @@
 //      preconditioner's own stream, overlapping running solves);
 //   3. runs every right-hand side as one slot of a Batch (iccg_batch_*),
 //      admitting new ones whenever slots free up, so solves that arrive while
-//      others run share every kernel launch with them;
+//      others run share every kernel launch with them -- except requests of
+//      at least cfg.multi_rows rows and two or more right-hand sides, which
+//      runner threads solve with the multi-right-hand-side solver
+//      (iccg_msolver_*: the operator read once per set of up to four
+//      right-hand sides) on streams of their own, beside the batch;
 //   4. returns x in the caller's numbering and frees the request's device
 //      memory as soon as its last right-hand side finishes.
 // The sequential mode is the plain one-at-a-time path for comparison: the
@@ -104,17 +108,27 @@ struct iccg_server_s {
   iccg_server_config cfg{};
   int device = 0;
   std::mutex mu;
-  std::condition_variable cv_work, cv_engine, cv_done;
-  std::deque<Request *> waiting, prep, ready;
+  std::condition_variable cv_work, cv_engine, cv_done, cv_multi;
+  std::deque<Request *> waiting, prep, ready, mready;
   std::map<long long, std::unique_ptr<Request>> all;
   long long next_id = 1;
   long long hbm_used = 0;
   long long submitted = 0, finished = 0, failed = 0;
-  int in_prep = 0, running_reqs = 0;
+  int in_prep = 0, running_reqs = 0, running_multi = 0;
   double run_ms = 0, admit_ms = 0, harvest_ms = 0;  // engine thread (under mu)
-  bool stop = false;
-  std::vector<std::thread> workers;
-  std::thread engine;
+  bool stop = false, stop_reaper = false;
+  std::vector<std::thread> workers, runners;
+  std::thread engine, reaper;
+  // device objects of finished requests, freed by the reaper thread; their
+  // HBM budget returns once they are freed
+  struct Reap {
+    iccg_csr_t A;
+    iccg_precond_t M;
+    long long hbm;
+  };
+  std::deque<Reap> reaps;
+  std::condition_variable cv_reap;
+  int untrimmed = 0;  // engine thread: requests finished since the last trim
   // engine-thread state
   iccg_batch_t B = nullptr;
   std::vector<std::pair<Request *, int>> owner;  // slot -> (request, rhs)
@@ -127,9 +141,18 @@ struct iccg_server_s {
     }
     cv_work.notify_all();
     cv_engine.notify_all();
+    cv_multi.notify_all();
     for (auto &t : workers)
       if (t.joinable()) t.join();
+    for (auto &t : runners)
+      if (t.joinable()) t.join();
     if (engine.joinable()) engine.join();
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop_reaper = true;  // after every thread that hands it work
+    }
+    cv_reap.notify_all();
+    if (reaper.joinable()) reaper.join();
     all.clear();
     if (B) iccg_batch_destroy(B);
   }
@@ -148,8 +171,39 @@ struct iccg_server_s {
     cv_work.notify_all();
   }
 
-  // under mu
-  void complete_locked(Request *r, int status, const std::string &err) {
+  // under mu: the request's device objects go to the reaper thread.  Freeing
+  // device memory (hipFree) waits for all work queued on the device, the
+  // runners' long captured chunks included: on the reaper that wait delays
+  // no engine, runner or worker thread.
+  void reap_locked(Request *r) {
+    reaps.push_back({r->A, r->M, r->hbm});
+    r->A = nullptr;
+    r->M = nullptr;
+    cv_reap.notify_one();
+  }
+
+  void reaper_loop() {
+    iccg_device_set(device);
+    for (;;) {
+      Reap t{};
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_reap.wait(lk, [&] { return stop_reaper || !reaps.empty(); });
+        if (reaps.empty()) return;
+        t = reaps.front();
+        reaps.pop_front();
+      }
+      if (t.M) iccg_precond_destroy(t.M);
+      if (t.A) iccg_csr_destroy(t.A);
+      std::lock_guard<std::mutex> lk(mu);
+      hbm_used -= t.hbm;
+      admit_locked();
+    }
+  }
+
+  // under mu; reaped: the request's device objects went to the reaper, which
+  // returns its HBM budget
+  void complete_locked(Request *r, int status, const std::string &err, bool reaped = false) {
     r->t_done = Clock::now();
     r->status = status;
     r->err = err;
@@ -158,8 +212,10 @@ struct iccg_server_s {
       finished++;
     else
       failed++;
-    hbm_used -= r->hbm;
-    admit_locked();
+    if (!reaped) {
+      hbm_used -= r->hbm;
+      admit_locked();
+    }
     cv_done.notify_all();
   }
 
@@ -223,9 +279,14 @@ struct iccg_server_s {
     r.remaining = r.nrhs;
   }
 
-  void unpermute(Request &r, int k, const std::vector<double> &xp) {
+  void unpermute(Request &r, int k, const double *xp) {
     double *x = r.x.data() + (size_t)k * r.n;
     for (int i = 0; i < r.n; ++i) x[i] = xp[(size_t)r.new_of_old[i]];
+  }
+
+  // requests the runner threads take (batched mode)
+  bool multi_request(const Request &r) const {
+    return cfg.multi_rows > 0 && r.nrhs >= 2 && r.n >= cfg.multi_rows;
   }
 
   void worker_loop(int threads) {
@@ -260,9 +321,58 @@ struct iccg_server_s {
       } else {
         r->t_ready = Clock::now();
         r->state = Q_READY;
-        ready.push_back(r);
-        cv_engine.notify_all();
+        if (multi_request(*r)) {
+          mready.push_back(r);
+          cv_multi.notify_one();
+        } else {
+          ready.push_back(r);
+          cv_engine.notify_all();
+        }
       }
+    }
+  }
+
+  // runner thread: solves a prepared request's right-hand sides together
+  // with the multi-right-hand-side solver on a stream of its own
+  void multi_loop() {
+    iccg_device_set(device);
+    std::vector<double> xs;
+    for (;;) {
+      Request *r = nullptr;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_multi.wait(lk, [&] { return stop || !mready.empty(); });
+        if (stop) return;
+        r = mready.front();
+        mready.pop_front();
+        r->state = Q_RUN;
+        running_multi++;
+      }
+      int st = ICCG_OK;
+      std::string err;
+      try {
+        iccg_msolver_t M = nullptr;
+        ck(iccg_msolver_create(r->A, r->M, r->nrhs, &M));
+        struct Guard {
+          iccg_msolver_t m;
+          ~Guard() { iccg_msolver_destroy(m); }
+        } g{M};
+        ck(iccg_msolver_set(M, r->bp.data(), nullptr));
+        ck(iccg_msolver_run(M, r->max_iter, r->tol, cfg.flags, r->res.data()));
+        xs.resize((size_t)r->nrhs * r->n);
+        ck(iccg_msolver_get_x(M, xs.data()));
+        for (int k = 0; k < r->nrhs; ++k) unpermute(*r, k, xs.data() + (size_t)k * r->n);
+      } catch (const Error &e) {
+        st = e.status;
+        err = e.what();
+      } catch (const std::exception &e) {
+        st = ICCG_ERR_NOMEM;
+        err = e.what();
+      }
+      std::lock_guard<std::mutex> lk(mu);
+      running_multi--;
+      reap_locked(r);
+      complete_locked(r, st, err, true);
     }
   }
 
@@ -329,17 +439,27 @@ struct iccg_server_s {
           const int k = owner[(size_t)s].second;
           xp.resize((size_t)r->n);
           ck(iccg_batch_get_x(B, s, xp.data()));
-          unpermute(*r, k, xp);
+          unpermute(*r, k, xp.data());
           r->res[(size_t)k] = res;
           ck(iccg_batch_remove(B, s));
           owner[(size_t)s] = {nullptr, 0};
           if (--r->remaining == 0) {
-            r->free_device();
-            ck(iccg_batch_trim(B));  // the request's vectors leave HBM with it
+            // the request's vectors leave HBM with it; while runners have
+            // work queued (which a free waits for) every 8th request
+            int multi = 0;
+            {
+              std::lock_guard<std::mutex> lk(mu);
+              multi = running_multi;
+            }
+            if (multi == 0 || ++untrimmed >= 8) {
+              ck(iccg_batch_trim(B));
+              untrimmed = 0;
+            }
             std::lock_guard<std::mutex> lk(mu);
             free_slots += r->nrhs;
             running_reqs--;
-            complete_locked(r, ICCG_OK, "");
+            reap_locked(r);
+            complete_locked(r, ICCG_OK, "", true);
           }
         }
         const double h_ms = ms_since(t_harvest);
@@ -405,7 +525,7 @@ struct iccg_server_s {
           ck(iccg_solver_set(S, r->bp.data() + (size_t)k * r->n, nullptr));
           ck(iccg_solver_run(S, r->max_iter, r->tol, cfg.flags, &r->res[(size_t)k]));
           ck(iccg_solver_get_x(S, xp.data()));
-          unpermute(*r, k, xp);
+          unpermute(*r, k, xp.data());
         }
       } catch (const Error &e) {
         st = e.status;
@@ -431,7 +551,8 @@ int iccg_server_create(const iccg_server_config *cfg, iccg_server_t *out) {
     require(cfg->precision == ICCG_FP64 || cfg->precision == ICCG_MIXED, ICCG_ERR_ARG,
             "unknown precision");
     require(cfg->hbm_budget > 0 && cfg->max_slots > 0 && cfg->block_size >= 1 &&
-                cfg->block_size <= 64 && cfg->workers >= 0,
+                cfg->block_size <= 64 && cfg->workers >= 0 && cfg->multi_rows >= 0 &&
+                cfg->multi_runners >= 0,
             ICCG_ERR_ARG, "bad server configuration");
     std::unique_ptr<iccg_server_s> S(new iccg_server_s);
     S->cfg = *cfg;
@@ -448,6 +569,11 @@ int iccg_server_create(const iccg_server_config *cfg, iccg_server_t *out) {
       for (int k = 0; k < nw; ++k)
         S->workers.emplace_back([p = S.get(), per] { p->worker_loop(per); });
       S->engine = std::thread([p = S.get()] { p->engine_loop(); });
+      S->reaper = std::thread([p = S.get()] { p->reaper_loop(); });
+      if (cfg->multi_rows > 0) {
+        const int nr = cfg->multi_runners > 0 ? cfg->multi_runners : 2;
+        for (int k = 0; k < nr; ++k) S->runners.emplace_back([p = S.get()] { p->multi_loop(); });
+      }
     } else {
       S->engine = std::thread([p = S.get()] { p->sequential_loop(); });
     }
@@ -570,7 +696,7 @@ int iccg_server_get_stats(iccg_server_t S, iccg_server_stats *st) {
     st->waiting = (int)S->waiting.size();
     st->preparing = (int)S->prep.size() + S->in_prep;
     st->ready = (int)S->ready.size();
-    st->running = S->running_reqs;
+    st->running = S->running_reqs + S->running_multi;
     st->hbm_in_use = S->hbm_used;
     st->engine_run_ms = S->run_ms;
     st->engine_admit_ms = S->admit_ms;

@@ -316,6 +316,13 @@ int iccg_msolver_set(iccg_msolver_t S, const double *b, const double *x0);
 int iccg_msolver_run(iccg_msolver_t S, int max_iter, double tol, int flags,
                      iccg_cg_result *res);
 int iccg_msolver_get_x(iccg_msolver_t S, double *x);
+/* Per-kernel-class event timing of the shared-kernel sets (kernel-by-kernel
+ * launches while on; classes as iccg_solver_get_timing, a launch counted
+ * once per set iteration in which some member ran). */
+int iccg_msolver_set_timing(iccg_msolver_t S, int on);
+int iccg_msolver_get_timing(iccg_msolver_t S, int cls, long *launches,
+                            double *total_ms);
+int iccg_msolver_reset_timing(iccg_msolver_t S);
 /* create + set + run + get_x + destroy on host buffers (x: x0 in, solution out) */
 int iccg_cg_solve_multi(iccg_csr_t A, iccg_precond_t M, int nrhs, const double *b,
                         double *x, int max_iter, double tol, int flags,
@@ -395,7 +402,13 @@ int iccg_batch_trim(iccg_batch_t B);
  *                         runs on the GPU beside the running solves), and
  *                         every right-hand side runs as one slot of a batch
  *                         of max_slots (continuous batching: slots freed by
- *                         finished solves take the next ready request).
+ *                         finished solves take the next ready request);
+ *                         requests of at least multi_rows rows (0: none)
+ *                         with two or more right-hand sides go instead to
+ *                         multi_runners threads (0 = 2), each solving one
+ *                         request at a time with the multi-right-hand-side
+ *                         solver (iccg_msolver_*) on a stream of its own,
+ *                         their kernels running beside the batch's.
  *   ICCG_SERVE_SEQUENTIAL the plain one-at-a-time path: one request is
  *                         prepared and its right-hand sides solved one after
  *                         another with the single-system solver before the
@@ -411,6 +424,8 @@ typedef struct {
   int block_size;         /* rows per ABMC block of a general matrix, <= 64 */
   int flags;              /* ICCG_SOLVE_TRUE_RESID */
   long long hbm_budget;   /* bytes of HBM for admitted requests */
+  int multi_rows;         /* batched mode: see above (0 = every request batched) */
+  int multi_runners;      /* threads running those requests (0 = 2) */
 } iccg_server_config;
 typedef struct {
   long long submitted, finished, failed, hbm_in_use;

@@ -49,10 +49,15 @@ def _collect(S, ids):
     return out
 
 
+@pytest.mark.parametrize("multi_rows", [0, 500])
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
-def test_server_batched_equals_direct(precision):
+def test_server_batched_equals_direct(precision, multi_rows):
+    """multi_rows=500: the requests of 1,600 to 4,096 rows with two or three
+    right-hand sides run on the multi-right-hand-side solver beside the
+    batch, the others in it; every result as the single-system solver's."""
     reqs = [_request(d, s, k) for d, s, k in REQS]
-    S = I.Server("batched", precision=precision, max_slots=8, workers=4)
+    S = I.Server("batched", precision=precision, max_slots=8, workers=4,
+                 multi_rows=multi_rows)
     ids = [S.submit(A, b, grid=d) for (A, b), (d, _, _) in zip(reqs, REQS)]
     got = _collect(S, ids)
     for rid, (A, b), (d, _, _) in zip(ids, reqs, REQS):
@@ -67,16 +72,18 @@ def test_server_batched_equals_direct(precision):
     assert st["finished"] == len(REQS) and st["failed"] == 0 and st["hbm_in_use"] == 0
 
 
-def test_server_alone_equals_in_the_mix():
+@pytest.mark.parametrize("multi_rows", [0, 1000])
+def test_server_alone_equals_in_the_mix(multi_rows):
     """One request served alone, then again while others arrive before,
-    during and after it: identical results."""
+    during and after it: identical results (multi_rows=1000: it and the
+    larger multi-right-hand-side requests on the runner threads)."""
     A0, b0 = _request((48, 40, 1), 11, 3)
-    S1 = I.Server("batched", max_slots=8, workers=2)
+    S1 = I.Server("batched", max_slots=8, workers=2, multi_rows=multi_rows)
     r0 = S1.submit(A0, b0, grid=(48, 40, 1))
     x_alone, res_alone, _, _ = _collect(S1, [r0])[r0]
     del S1
     others = [_request(d, 20 + s, k) for d, s, k in REQS]
-    S = I.Server("batched", max_slots=6, workers=3)
+    S = I.Server("batched", max_slots=6, workers=3, multi_rows=multi_rows)
     ids = [S.submit(A, b, grid=d) for (A, b), (d, _, _) in zip(others[:3], REQS[:3])]
     time.sleep(0.05)
     rid = S.submit(A0, b0, grid=(48, 40, 1))
