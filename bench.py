@@ -670,6 +670,16 @@ def main_multi(args):
         check_solves("multi", [r], args.tol, true_resid=True)
     log(f"multi ({K} right-hand sides, layout {MS.layout()}): {wall_m * 1e3:.1f} ms, "
         f"iters {[r.iters for r in res]}")
+    # one more solve launched kernel by kernel, HIP events around each class
+    MS.reset_timing()
+    MS.set_timing(True)
+    MS.run(args.max_iter, args.tol, true_residual=true_res, zero_x0=True)
+    MS.set_timing(False)
+    classes = {}
+    for c, name in enumerate(["spmv", "fwd_sweep", "bwd_sweep", "vector_finish"]):
+        cnt, ms = MS.timing(c)
+        classes[name] = {"launch_groups": cnt, "avg_us": round(ms / max(cnt, 1) * 1e3, 2)}
+    log(f"multi per set iteration (events): {classes}")
     del MS
     S = I.Solver(dA, M)
     single, xs, wall_s = [], [], 0.0
@@ -701,7 +711,7 @@ def main_multi(args):
                                 f"{N}^{args.dims} (n={n}, nnz={nnz}), ABMC B={args.block} "
                                 f"({info['nb']} blocks, {info['nc']} colors), b~N(0,1), "
                                 f"tol {args.tol:g}"), "n": n, "nnz": nnz, "nrhs": K},
-        "iters": [r.iters for r in res],
+        "iters": [r.iters for r in res], "classes_per_set_iteration": classes,
         "one_at_a_time": {"value": round(K / wall_s, 3), "ms": round(wall_s * 1e3, 3)},
         "speedup_vs_one_at_a_time": round(wall_s / wall_m, 3),
         "bitwise_equal_to_one_at_a_time": bool(same),

@@ -128,8 +128,9 @@ def test_multi_mixed_restarts_per_member():
 
 def test_multi_x0_max_iter_and_repeat():
     """Per-member starting vectors, members stopped by max_iter before they
-    converge, and a second run on the same solver (captured graphs reused)
-    with new right-hand sides: each equals the single-system solver."""
+    converge, a second run on the same solver (captured graphs reused) with
+    new right-hand sides, and a third launched kernel by kernel with event
+    timing: each equals the single-system solver."""
     o = _system("box2d")
     M = I.IC0(o.A, 0.0, sched=o.sched, precision="mixed")
     B = _rhs(o.A.n, 3, 61)
@@ -148,6 +149,18 @@ def test_multi_x0_max_iter_and_repeat():
     for k in range(3):
         assert _same(res2[k], ref2[k])
     assert np.array_equal(MS.x(), Xr2)
+    # kernel-by-kernel launches with event timing: the same results, and
+    # one launch group per class and set iteration in which a member ran
+    MS.set(B2)
+    MS.set_timing(True)
+    res3 = MS.run(10000, 1e-8)
+    MS.set_timing(False)
+    for k in range(3):
+        assert _same(res3[k], ref2[k])
+    assert np.array_equal(MS.x(), Xr2)
+    most = max(r.iters for r in res3)
+    assert MS.timing(0)[0] == most and MS.timing(1)[0] == most
+    assert all(MS.timing(c)[1] > 0.0 for c in range(4))
 
 
 def test_multi_other_plans_run_alone():
