@@ -23,7 +23,10 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <exception>
+#include <functional>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "dev.hpp"
@@ -821,6 +824,7 @@ static int mwm(const DevCsr &A) {
 // one set of K members: its interleaved vectors, partials and scalars
 struct MSet {
   int K = 0, m = 0, first = 0;  // members used of K, first right-hand side
+  hipStream_t st = nullptr;     // the set's own stream: sets of one solver run concurrently
   double *X = nullptr, *R = nullptr, *P = nullptr, *AP = nullptr, *B = nullptr;
   void *Y = nullptr, *Z = nullptr;
   double *part = nullptr;
@@ -843,6 +847,7 @@ struct MSet {
     dfree(part);
     dfree(S);
     if (hS) (void)hipHostFree(hS);
+    if (st) (void)hipStreamDestroy(st);
   }
 };
 
@@ -1068,8 +1073,8 @@ struct iccg_msolver_s {
 namespace iccg {
 
 static void mset_sync(iccg_msolver_s &S, MSet &s) {
-  HIPCHK(hipMemcpyAsync(s.hS, s.S, sizeof(Scal) * s.K, hipMemcpyDeviceToHost, S.st));
-  HIPCHK(hipStreamSynchronize(S.st));
+  HIPCHK(hipMemcpyAsync(s.hS, s.S, sizeof(Scal) * s.K, hipMemcpyDeviceToHost, s.st));
+  HIPCHK(hipStreamSynchronize(s.st));
 }
 
 // k iterations: a replayed hipGraph, or (timing) launched one by one with an
@@ -1085,13 +1090,13 @@ static void mset_iterations(iccg_msolver_s &S, MSet &s, int k) {
           S.pool.push_back(e);
         }
         hipEvent_t e = S.pool[S.used++];
-        HIPCHK(hipEventRecord(e, S.st));
+        HIPCHK(hipEventRecord(e, s.st));
         S.marks.push_back({c, j, e});
       };
       if (s.K == 4)
-        MOps<4>::iteration(*S.A, *S.M, s, nk, S.st, mark);
+        MOps<4>::iteration(*S.A, *S.M, s, nk, s.st, mark);
       else
-        MOps<2>::iteration(*S.A, *S.M, s, nk, S.st, mark);
+        MOps<2>::iteration(*S.A, *S.M, s, nk, s.st, mark);
     }
     return;
   }
@@ -1099,9 +1104,9 @@ static void mset_iterations(iccg_msolver_s &S, MSet &s, int k) {
     auto none = [](int) {};
     for (int j = 0; j < k; ++j) {
       if (s.K == 4)
-        MOps<4>::iteration(*S.A, *S.M, s, nk, S.st, none);
+        MOps<4>::iteration(*S.A, *S.M, s, nk, s.st, none);
       else
-        MOps<2>::iteration(*S.A, *S.M, s, nk, S.st, none);
+        MOps<2>::iteration(*S.A, *S.M, s, nk, s.st, none);
     }
   };
   if (s.graph_version != S.M->plan_version) {
@@ -1113,15 +1118,15 @@ static void mset_iterations(iccg_msolver_s &S, MSet &s, int k) {
     if (e.first == k) exec = e.second;
   if (!exec) {
     hipGraph_t g = nullptr;
-    HIPCHK(hipStreamBeginCapture(S.st, hipStreamCaptureModeThreadLocal));
+    HIPCHK(hipStreamBeginCapture(s.st, hipStreamCaptureModeThreadLocal));
     enqueue();
-    HIPCHK(hipStreamEndCapture(S.st, &g));
+    HIPCHK(hipStreamEndCapture(s.st, &g));
     hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     HIPCHK(e);
     s.execs.push_back({k, exec});
   }
-  HIPCHK(hipGraphLaunch(exec, S.st));
+  HIPCHK(hipGraphLaunch(exec, s.st));
 }
 
 // the timed chunk's class times, for its first `useful` iterations (those in
@@ -1142,17 +1147,17 @@ static void mset_harvest(iccg_msolver_s &S, int useful) {
 
 static void mset_resid(iccg_msolver_s &S, MSet &s, int mode, double tol, int max_iter, int mask) {
   if (s.K == 4)
-    MOps<4>::resid(*S.A, s, mode, tol, max_iter, mask, S.st);
+    MOps<4>::resid(*S.A, s, mode, tol, max_iter, mask, s.st);
   else
-    MOps<2>::resid(*S.A, s, mode, tol, max_iter, mask, S.st);
+    MOps<2>::resid(*S.A, s, mode, tol, max_iter, mask, s.st);
 }
 
 static void mset_precond_init(iccg_msolver_s &S, MSet &s) {
   const long nk = (long)S.n * s.K;
   if (s.K == 4)
-    MOps<4>::precond_init(*S.M, s, nk, S.st);
+    MOps<4>::precond_init(*S.M, s, nk, s.st);
   else
-    MOps<2>::precond_init(*S.M, s, nk, S.st);
+    MOps<2>::precond_init(*S.M, s, nk, s.st);
 }
 
 // The CG phase of one set: run_solver's loop for K members at once.  Chunks
@@ -1167,7 +1172,7 @@ static void mset_run(iccg_msolver_s &S, MSet &s, int max_iter, double tol, int f
   const bool true_resid = (flags & ICCG_SOLVE_TRUE_RESID) != 0;
   auto t0 = std::chrono::steady_clock::now();
   if (flags & ICCG_SOLVE_ZERO_X0)
-    HIPCHK(hipMemsetAsync(s.X, 0, sizeof(double) * (size_t)S.n * K, S.st));
+    HIPCHK(hipMemsetAsync(s.X, 0, sizeof(double) * (size_t)S.n * K, s.st));
   mset_resid(S, s, 1, tol, max_iter, all);
   mset_precond_init(S, s);
   HIPCHK(hipGetLastError());
@@ -1301,6 +1306,7 @@ int iccg_msolver_create(iccg_csr_t A, iccg_precond_t M, int nrhs, iccg_msolver_t
     const size_t vb = M->mixed ? sizeof(float) : sizeof(double);
     for (MSet &s : S->sets) {
       const size_t nk = (size_t)n * s.K;
+      HIPCHK(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
       s.X = dalloc<double>(nk);
       s.R = dalloc<double>(nk);
       s.P = dalloc<double>(nk);
@@ -1368,15 +1374,48 @@ int iccg_msolver_run(iccg_msolver_t S, int max_iter, double tol, int flags,
     HIPCHK(hipSetDevice(S->device));
     require(S->sets.empty() || multi_plan(*S->A, *S->M), ICCG_ERR_STATE,
             "the preconditioner's plan changed since the solver was made: make it again");
-    for (MSet &s : S->sets) mset_run(*S, s, max_iter, tol, flags, res);
-    const size_t n = (size_t)S->n;
-    for (int k : S->lone) {
-      auto ck = [](int st) {
-        if (st != ICCG_OK) throw Error(st, iccg_last_error());
-      };
-      ck(iccg_solver_set(S->single, S->hb.data() + k * n, S->hx.data() + k * n));
-      ck(iccg_solver_run(S->single, max_iter, tol, flags, &res[k]));
-      ck(iccg_solver_get_x(S->single, S->hx.data() + k * n));
+    // the sets, and the right-hand sides run alone, are independent: each
+    // runs on a host thread of its own (its own stream), so one request's
+    // sets share the device instead of queueing behind each other (event
+    // timing runs them one after another)
+    std::vector<std::function<void()>> units;
+    for (MSet &s : S->sets)
+      units.push_back([&, ps = &s] { mset_run(*S, *ps, max_iter, tol, flags, res); });
+    if (!S->lone.empty())
+      units.push_back([&] {
+        const size_t n = (size_t)S->n;
+        auto ck = [](int st) {
+          if (st != ICCG_OK) throw Error(st, iccg_last_error());
+        };
+        for (int k : S->lone) {
+          ck(iccg_solver_set(S->single, S->hb.data() + k * n, S->hx.data() + k * n));
+          ck(iccg_solver_run(S->single, max_iter, tol, flags, &res[k]));
+          ck(iccg_solver_get_x(S->single, S->hx.data() + k * n));
+        }
+      });
+    if (S->timing || units.size() == 1) {
+      for (auto &u : units) u();
+    } else {
+      std::vector<std::exception_ptr> errs(units.size());
+      std::vector<std::thread> th;
+      const int dev = S->device;
+      for (size_t u = 1; u < units.size(); ++u)
+        th.emplace_back([&, u] {
+          try {
+            HIPCHK(hipSetDevice(dev));
+            units[u]();
+          } catch (...) {
+            errs[u] = std::current_exception();
+          }
+        });
+      try {
+        units[0]();
+      } catch (...) {
+        errs[0] = std::current_exception();
+      }
+      for (auto &t : th) t.join();
+      for (auto &e : errs)
+        if (e) std::rethrow_exception(e);
     }
   });
 }
