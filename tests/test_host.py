@@ -324,3 +324,29 @@ def test_c_abi_header_is_plain_c_and_links(tmp_path):
                         "-liccg_hip", "-Wl,-rpath," + os.path.join(root, "iccg_amd", "lib")],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_cpp_mirror_headers_compile_with_multi_rhs(tmp_path):
+    """include/iccg/*.hpp compile as C++17 with conjugate_gradient and
+    conjugate_gradient_multi instantiated for IC0 and linked against the
+    library (no GPU call is made: the program only takes their addresses)."""
+    import shutil
+    import subprocess
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("no C++ compiler")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "use.cpp"
+    src.write_text(
+        '#include "iccg/cg.hpp"\n'
+        "int main() {\n"
+        "  auto f = &conjugate_gradient<IC0>;\n"
+        "  auto g = &conjugate_gradient_multi<IC0>;\n"
+        "  return (f && g) ? 0 : 1;\n"
+        "}\n")
+    exe = tmp_path / "use"
+    r = subprocess.run([cxx, "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(root, "include"),
+                        str(src), "-o", str(exe), "-L", os.path.join(root, "iccg_amd", "lib"),
+                        "-liccg_hip", "-Wl,-rpath," + os.path.join(root, "iccg_amd", "lib")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
