@@ -427,6 +427,8 @@ typedef struct {
   int multi_rows;         /* batched mode: see above (0 = every request batched) */
   int multi_runners;      /* threads running those requests (0 = 2) */
 } iccg_server_config;
+/* hbm_in_use: the admitted requests' footprint estimates, a finished
+ * request's until the reaper thread has freed its device memory. */
 typedef struct {
   long long submitted, finished, failed, hbm_in_use;
   int waiting, preparing, ready, running;

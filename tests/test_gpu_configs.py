@@ -22,6 +22,7 @@ import bench
 import iccg_amd as I
 from oracle import oracle as O
 from tests import problems as P
+from tests.test_gpu_server import _settled
 
 pytestmark = pytest.mark.gpu
 
@@ -103,7 +104,7 @@ def test_config5_serving_stream_requests():
         ids.append(S.submit(A, b, grid=dims))
         time.sleep(0.02)
     mix = _collect(S, ids)
-    st = S.stats()
+    st = _settled(S)
     assert st["failed"] == 0 and st["hbm_in_use"] == 0
     S.close()
     for (A, b, dims), (xa, ra, _, _), rid in zip(reqs, alone, ids):

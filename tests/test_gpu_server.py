@@ -41,6 +41,17 @@ REQS = [((24, 20, 1), 1, 2), ((8, 8, 8), 2, 1), ((40, 40, 1), 3, 3), ((12, 10, 1
         ((64, 16, 1), 5, 1), ((16, 16, 16), 6, 2)]
 
 
+def _settled(S, timeout_s=60.0):
+    """Stats once the reaper thread has freed every finished request's
+    device memory and returned its HBM budget (it runs after completion)."""
+    t0 = time.time()
+    while True:
+        st = S.stats()
+        if st["hbm_in_use"] == 0 or time.time() - t0 > timeout_s:
+            return st
+        time.sleep(0.01)
+
+
 def _collect(S, ids):
     out = {}
     for rid in ids:
@@ -68,7 +79,7 @@ def test_server_batched_equals_direct(precision, multi_rows):
             assert r.converged and (r.iters, r.rel_resid, r.restarts) == (q.iters, q.rel_resid,
                                                                          q.restarts)
         assert lat >= prep >= 0.0
-    st = S.stats()
+    st = _settled(S)
     assert st["finished"] == len(REQS) and st["failed"] == 0 and st["hbm_in_use"] == 0
 
 
