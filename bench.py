@@ -681,6 +681,13 @@ def main_multi(args):
         classes[name] = {"launch_groups": cnt, "avg_us": round(ms / max(cnt, 1) * 1e3, 2)}
     log(f"multi per set iteration (events): {classes}")
     del MS
+    if args.no_compare:
+        print(json.dumps({"metric": "right-hand sides of one system solved to tol per second",
+                          "value": round(K / wall_m, 3), "unit": "right-hand sides/s",
+                          "ms_per_step": round(wall_m * 1e3, 3), "nrhs": K,
+                          "iters": [r.iters for r in res],
+                          "classes_per_set_iteration": classes}), flush=True)
+        return
     S = I.Solver(dA, M)
     single, xs, wall_s = [], [], 0.0
     for k in range(K):
@@ -732,7 +739,8 @@ def main():
     ap.add_argument("--systems", type=int, default=512)
     ap.add_argument("--requests", type=int, default=64)
     ap.add_argument("--no-compare", action="store_true",
-                    help="batch workload: time the batched solve only (for profiling)")
+                    help="batch, serve and multi workloads: time the batched / served / "
+                         "multi-right-hand-side solves only (for profiling)")
     ap.add_argument("--rate", type=float, default=20.0, help="request arrivals per second")
     ap.add_argument("--slots", type=int, default=256)
     ap.add_argument("--workers", type=int, default=12)

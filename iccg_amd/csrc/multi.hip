@@ -64,6 +64,90 @@ __device__ __forceinline__ void stK(T *__restrict__ p, const T (&o)[K]) {
     *reinterpret_cast<vk_t *>(p) = u;
 }
 
+// Workgroup partials in component-major order: component c of slot g at
+// part[2 (c stride + g)] (hi, lo), so that the one-workgroup fold reads each
+// component's slots contiguously (in block_partial's slot-major order a fold
+// over the 3K components of K members strides its loads by 6K doubles per
+// slot).  The lanes' and the wavefronts' trees are block_partial's, the
+// fold's finish_reduce2's.
+template <int NC>
+__device__ void block_partial_cm(double (&h)[NC], double (&l)[NC], double *part, int gidx,
+                                 int stride) {
+  __shared__ double sh[NC][2][16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  wave_dd_reduce<NC>(h, l);
+  __syncthreads();  // a previous call's reads of sh are complete
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      sh[c][0][w] = h[c];
+      sh[c][1][w] = l[c];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < NC) {
+    const int c = threadIdx.x;
+    double ah = sh[c][0][0], al = sh[c][1][0];
+    for (int k = 1; k < nw; ++k) dd_add(ah, al, sh[c][0][k], sh[c][1][k]);
+    part[((size_t)c * stride + gidx) * 2 + 0] = ah;
+    part[((size_t)c * stride + gidx) * 2 + 1] = al;
+  }
+}
+
+// finish_reduce2 over component-major partials (ca slots of stride sa at
+// pa, cb of stride sb at pb): the same fold, in the same order
+template <int NA, int NB>
+__device__ bool finish_reduce2_cm(const double *__restrict__ pa, int ca, int sa,
+                                  const double *__restrict__ pb, int cb, int sb, double *oa,
+                                  double *ob) {
+  constexpr int N = NA + NB;
+  __shared__ double sh[N][2][16];
+  __shared__ double res[N];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  double h[N], l[N];
+#pragma unroll
+  for (int c = 0; c < N; ++c) h[c] = l[c] = 0.0;
+  for (int b = threadIdx.x; b < ca; b += blockDim.x) {
+#pragma unroll
+    for (int c = 0; c < NA; ++c)
+      dd_add(h[c], l[c], pa[((size_t)c * sa + b) * 2], pa[((size_t)c * sa + b) * 2 + 1]);
+  }
+  for (int b = threadIdx.x; b < cb; b += blockDim.x) {
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+      dd_add(h[NA + c], l[NA + c], pb[((size_t)c * sb + b) * 2], pb[((size_t)c * sb + b) * 2 + 1]);
+  }
+  wave_dd_reduce<N>(h, l);
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+      sh[c][0][w] = h[c];
+      sh[c][1][w] = l[c];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < N) {
+    const int c = threadIdx.x;
+    double ah = sh[c][0][0], al = sh[c][1][0];
+    for (int k = 1; k < nw; ++k) dd_add(ah, al, sh[c][0][k], sh[c][1][k]);
+    res[c] = ah + al;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return false;
+#pragma unroll
+  for (int c = 0; c < NA; ++c) oa[c] = res[c];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) ob[c] = res[NA + c];
+  return true;
+}
+
+template <int NC>
+__device__ bool finish_reduce_cm(const double *__restrict__ part, int count, double (&out)[NC]) {
+  return finish_reduce2_cm<NC, 0>(part, count, count, nullptr, 0, 0, out, nullptr);
+}
+
 template <int K>
 __device__ __forceinline__ bool all_done(const Scal *S) {
   bool d = true;
@@ -220,7 +304,7 @@ __global__ __launch_bounds__(kWG) void km_spmv_cg(DevCsr A, const double *__rest
       dd_fma(h[3 * m + 2], l[3 * m + 2], y[m], y[m]);
     }
   });
-  block_partial<3 * K>(h, l, part, blockIdx.x);
+  block_partial_cm<3 * K>(h, l, part, blockIdx.x, gridDim.x);
 }
 
 // R = B - A X with ||b||^2, ||r||^2 per member (components 2m + c), as
@@ -253,19 +337,24 @@ __global__ __launch_bounds__(kWG) void km_resid(DevCsr A, const double *__restri
       dd_fma(h[2 * m + 1], l[2 * m + 1], rv[m], rv[m]);
     }
   });
-  block_partial<2 * K>(h, l, part, blockIdx.x);
+  block_partial_cm<2 * K>(h, l, part, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
-// finish kernels: the single-system scalar logic, member by member
+// finish kernels: the single-system scalar logic, member by member.  One
+// workgroup of four wavefronts: with the 2K-3K components of K members the
+// shuffle trees of sixteen wavefronts cost more than they save (192^3, four
+// members: the alpha finish 21.9 -> 14.5 us; 1,024 threads with eight slots
+// per thread loaded ahead spill, 48 us)
 // ---------------------------------------------------------------------------
+constexpr int kMFinWG = 256;
 // k_fin_resid for the members of mask (mode 0: every member's rr_true)
 template <int K>
-__global__ __launch_bounds__(kFinWG) void km_fin_resid(const double *__restrict__ part,
+__global__ __launch_bounds__(kMFinWG) void km_fin_resid(const double *__restrict__ part,
                                                        int count, Scal *S, int mode, double tol,
                                                        int max_iter, int mask) {
   double out[2 * K];
-  if (!finish_reduce<2 * K>(part, count, out)) return;
+  if (!finish_reduce_cm<2 * K>(part, count, out)) return;
   for (int m = 0; m < K; ++m) {
     if (!((mask >> m) & 1)) continue;
     Scal &s = S[m];
@@ -298,7 +387,7 @@ __global__ __launch_bounds__(kFinWG) void km_fin_resid(const double *__restrict_
 
 // k_fin_alpha per member
 template <int K>
-__global__ __launch_bounds__(kFinWG) void km_fin_alpha(const double *__restrict__ part, int count,
+__global__ __launch_bounds__(kMFinWG) void km_fin_alpha(const double *__restrict__ part, int count,
                                                        Scal *S) {
   if (all_done<K>(S)) {
     __syncthreads();  // every thread has read the flags
@@ -307,7 +396,7 @@ __global__ __launch_bounds__(kFinWG) void km_fin_alpha(const double *__restrict_
     return;
   }
   double out[3 * K];
-  if (!finish_reduce<3 * K>(part, count, out)) return;
+  if (!finish_reduce_cm<3 * K>(part, count, out)) return;
   for (int m = 0; m < K; ++m) {
     Scal &s = S[m];
     if (s.done) {
@@ -329,12 +418,12 @@ __global__ __launch_bounds__(kFinWG) void km_fin_alpha(const double *__restrict_
 // k_fin_xr_rz per member: the stop test and beta from the sweeps' |r|^2 and
 // r.z partials
 template <int K>
-__global__ __launch_bounds__(kFinWG) void km_fin_stop(const double *__restrict__ pxr, int cxr,
+__global__ __launch_bounds__(kMFinWG) void km_fin_stop(const double *__restrict__ pxr, int cxr,
                                                       const double *__restrict__ prz, int crz,
                                                       Scal *S) {
   if (all_done<K>(S)) return;
   double rr[K], rz[K];
-  if (!finish_reduce2<K, K>(pxr, cxr, prz, crz, rr, rz)) return;
+  if (!finish_reduce2_cm<K, K>(pxr, cxr, cxr, prz, crz, crz, rr, rz)) return;
   for (int m = 0; m < K; ++m) {
     Scal &s = S[m];
     if (s.done) continue;
@@ -356,11 +445,11 @@ __global__ __launch_bounds__(kFinWG) void km_fin_stop(const double *__restrict__
 
 // k_fin_rz per member (init: rz_old at the start of a solve)
 template <int K>
-__global__ __launch_bounds__(kFinWG) void km_fin_rz(const double *__restrict__ part, int count,
+__global__ __launch_bounds__(kMFinWG) void km_fin_rz(const double *__restrict__ part, int count,
                                                     Scal *S, int init) {
   if (all_done<K>(S)) return;
   double out[K];
-  if (!finish_reduce<K>(part, count, out)) return;
+  if (!finish_reduce_cm<K>(part, count, out)) return;
   for (int m = 0; m < K; ++m) {
     Scal &s = S[m];
     if (s.done) continue;
@@ -411,7 +500,7 @@ __global__ __launch_bounds__(kWG) void km_rz(long nk, const double *__restrict__
     for (int m = 0; m < K; ++m)
       if (i % K == m) dd_fma(h[m], l[m], R[i], (double)Z[i]);
   }
-  block_partial<K>(h, l, part, blockIdx.x);
+  block_partial_cm<K>(h, l, part, blockIdx.x, gridDim.x);
 }
 
 // p = z + beta p, or p = z at the start (k_update_p), members not done
@@ -618,7 +707,8 @@ __global__ __launch_bounds__(kWG) void km_fwd_sec(int kb0, int kb1, const int *_
                                                   const unsigned *__restrict__ uinb,
                                                   const VT *__restrict__ uv, YT *__restrict__ Z,
                                                   const Scal *S, const double *__restrict__ AP,
-                                                  double *part, int gidx0, double *part_rz) {
+                                                  double *part, int gidx0, double *part_rz,
+                                                  int nslot) {
   __shared__ YT win[(kWG / 64) * K * kSecWindow];
   if (all_done<K>(S)) return;
   const int lane = threadIdx.x & 63;
@@ -713,8 +803,8 @@ __global__ __launch_bounds__(kWG) void km_fwd_sec(int kb0, int kb1, const int *_
       stK<K>(Y + (size_t)K * (bs + lane), yi);
     }
   }
-  if (RU) block_partial<K>(h, l, part, gidx0 + blockIdx.x);
-  if (RU && FB) block_partial<K>(hz, lz, part_rz, blockIdx.x);
+  if (RU) block_partial_cm<K>(h, l, part, gidx0 + blockIdx.x, nslot);
+  if (RU && FB) block_partial_cm<K>(hz, lz, part_rz, blockIdx.x, nslot);
 }
 
 // L^T z = y for K members over the blocks [kb0, kb1) of one color; RZ: the
@@ -729,7 +819,7 @@ __global__ __launch_bounds__(kWG) void km_bwd_sec(int kb0, int kb1, const int *_
                                                   const VT *__restrict__ diag,
                                                   const YT *__restrict__ Y, YT *__restrict__ Z,
                                                   const double *__restrict__ R, const Scal *S,
-                                                  double *part, int gidx0) {
+                                                  double *part, int gidx0, int nslot) {
   __shared__ YT win[(kWG / 64) * K * kSecWindow];
   if (all_done<K>(S)) return;
   const int lane = threadIdx.x & 63;
@@ -777,7 +867,7 @@ __global__ __launch_bounds__(kWG) void km_bwd_sec(int kb0, int kb1, const int *_
       }
     }
   }
-  if (RZ) block_partial<K>(h, l, part, gidx0 + blockIdx.x);
+  if (RZ) block_partial_cm<K>(h, l, part, gidx0 + blockIdx.x, nslot);
 }
 
 }  // namespace iccg
@@ -893,7 +983,7 @@ struct MSweeps {
 #define KM_FWD(FB_, RU_)                                                                      \
   hipLaunchKernelGGL((km_fwd_sec<K, VT, YT, SW, SO, FB_, NT, RU_>), grid, blk, 0, st, b0, b1, \
                      sc.d_bstart, sc.d_nlev_f, F.ls.off, F.ls.inb, F.ls.col, lv, dg, R, Y,     \
-                     sc.d_nlev_b, F.us.off, F.us.inb, uv, Z, S, AP, part, gidx0, prz)
+                     sc.d_nlev_b, F.us.off, F.us.inb, uv, Z, S, AP, part, gidx0, prz, G)
       if (last && AP)
         KM_FWD(true, true);
       else if (AP)
@@ -922,11 +1012,11 @@ struct MSweeps {
       if (prz)
         hipLaunchKernelGGL((km_bwd_sec<K, VT, YT, SW, SO, true, NT>), grid, blk, 0, st, b0, b1,
                            sc.d_bstart, sc.d_nlev_b, F.us.off, F.us.inb, F.us.col, uv, dg, Y, Z,
-                           R, S, prz, gidx0);
+                           R, S, prz, gidx0, G);
       else
         hipLaunchKernelGGL((km_bwd_sec<K, VT, YT, SW, SO, false, NT>), grid, blk, 0, st, b0, b1,
                            sc.d_bstart, sc.d_nlev_b, F.us.off, F.us.inb, F.us.col, uv, dg, Y, Z,
-                           R, S, nullptr, 0);
+                           R, S, nullptr, 0, G);
       gidx0 += g[(size_t)c];
     }
   }
@@ -970,7 +1060,7 @@ struct MOps {
     const DevCsr &D = A.d;
     const int g = slice_grid(D.sell.nslice);
     MDISPATCH_A(D, km_resid, K, dim3(g), dim3(kWG), 0, st, D, s.X, s.B, s.R, s.part, mode, mask);
-    hipLaunchKernelGGL(km_fin_resid<K>, dim3(1), dim3(kFinWG), 0, st, s.part, g, s.S, mode, tol,
+    hipLaunchKernelGGL(km_fin_resid<K>, dim3(1), dim3(kMFinWG), 0, st, s.part, g, s.S, mode, tol,
                        max_iter, mask);
   }
 
@@ -988,7 +1078,7 @@ struct MOps {
       const int gq = quad_grid(nk);
       hipLaunchKernelGGL((km_rz<K, YT>), dim3(gq), dim3(kWG), 0, st, nk, s.R, (const YT *)s.Z,
                          s.S, s.part);
-      hipLaunchKernelGGL(km_fin_rz<K>, dim3(1), dim3(kFinWG), 0, st, s.part, gq, s.S, 1);
+      hipLaunchKernelGGL(km_fin_rz<K>, dim3(1), dim3(kMFinWG), 0, st, s.part, gq, s.S, 1);
       hipLaunchKernelGGL((km_update_p<K, YT>), dim3(gq), dim3(kWG), 0, st, nk, (const YT *)s.Z,
                          s.P, s.S, 1);
     });
@@ -1008,7 +1098,7 @@ struct MOps {
     mark(0);
     MDISPATCH_A(D, km_spmv_cg, K, dim3(ga), dim3(kWG), 0, st, D, s.P, s.AP, s.S, s.part);
     mark(3);
-    hipLaunchKernelGGL(km_fin_alpha<K>, dim3(1), dim3(kFinWG), 0, st, s.part, ga, s.S);
+    hipLaunchKernelGGL(km_fin_alpha<K>, dim3(1), dim3(kMFinWG), 0, st, s.part, ga, s.S);
     int G = 0;
     const std::vector<int> g = color_groups(M.sc, G);
     const bool fb = F.fb;
@@ -1023,7 +1113,7 @@ struct MOps {
       mark(2);
       SWP::bwd(M.sc, F, fb, uv, dg, (const YT *)s.Y, (YT *)s.Z, s.R, s.S, prz, slot0, st);
       mark(3);
-      hipLaunchKernelGGL(km_fin_stop<K>, dim3(1), dim3(kFinWG), 0, st, s.part, G, prz, G, s.S);
+      hipLaunchKernelGGL(km_fin_stop<K>, dim3(1), dim3(kMFinWG), 0, st, s.part, G, prz, G, s.S);
       hipLaunchKernelGGL((km_update_xp<K, YT>), dim3(quad_grid(nk)), dim3(kWG), 0, st, nk, s.X,
                          (const YT *)s.Z, s.P, s.S);
     });
