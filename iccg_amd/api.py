@@ -716,9 +716,9 @@ class Server:
     mode "batched": requests are admitted by HBM budget, prepared (ABMC
     ordering, P^T A P, IC(0)) on `workers` host threads and solved with
     continuous batching, every right-hand side a slot of one Batch.
-    Requests of at least `multi_rows` rows with two or more right-hand sides
-    (multi_rows=0: none) are solved instead by `multi_runners` threads with
-    the multi-right-hand-side solver, beside the batch.
+    Requests of at least `multi_rows` rows (multi_rows=0: none) are solved
+    instead by `multi_runners` threads with the multi-right-hand-side solver
+    (one right-hand side: the single-system solver), beside the batch.
     mode "sequential": the plain one-at-a-time path (prepare, then solve each
     right-hand side alone) for comparison.  submit() returns at once; wait()
     and result() collect finished requests."""

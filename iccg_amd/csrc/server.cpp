@@ -12,10 +12,10 @@
 //   3. runs every right-hand side as one slot of a Batch (iccg_batch_*),
 //      admitting new ones whenever slots free up, so solves that arrive while
 //      others run share every kernel launch with them -- except requests of
-//      at least cfg.multi_rows rows and two or more right-hand sides, which
-//      runner threads solve with the multi-right-hand-side solver
-//      (iccg_msolver_*: the operator read once per set of up to four
-//      right-hand sides) on streams of their own, beside the batch;
+//      at least cfg.multi_rows rows, which runner threads solve with the
+//      multi-right-hand-side solver (iccg_msolver_*: the operator read once
+//      per set of up to four right-hand sides; a single right-hand side on
+//      the single-system solver) on streams of their own, beside the batch;
 //   4. returns x in the caller's numbering and frees the request's device
 //      memory as soon as its last right-hand side finishes.
 // The sequential mode is the plain one-at-a-time path for comparison: the
@@ -286,7 +286,7 @@ struct iccg_server_s {
 
   // requests the runner threads take (batched mode)
   bool multi_request(const Request &r) const {
-    return cfg.multi_rows > 0 && r.nrhs >= 2 && r.n >= cfg.multi_rows;
+    return cfg.multi_rows > 0 && r.n >= cfg.multi_rows;
   }
 
   void worker_loop(int threads) {

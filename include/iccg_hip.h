@@ -403,12 +403,12 @@ int iccg_batch_trim(iccg_batch_t B);
  *                         every right-hand side runs as one slot of a batch
  *                         of max_slots (continuous batching: slots freed by
  *                         finished solves take the next ready request);
- *                         requests of at least multi_rows rows (0: none)
- *                         with two or more right-hand sides go instead to
- *                         multi_runners threads (0 = 2), each solving one
- *                         request at a time with the multi-right-hand-side
- *                         solver (iccg_msolver_*) on a stream of its own,
- *                         their kernels running beside the batch's.
+ *                         requests of at least multi_rows rows (0: none) go
+ *                         instead to multi_runners threads (0 = 2), each
+ *                         solving one request at a time with the
+ *                         multi-right-hand-side solver (iccg_msolver_*; one
+ *                         right-hand side runs on the single-system solver)
+ *                         on a stream of its own, beside the batch.
  *   ICCG_SERVE_SEQUENTIAL the plain one-at-a-time path: one request is
  *                         prepared and its right-hand sides solved one after
  *                         another with the single-system solver before the

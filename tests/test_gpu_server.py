@@ -63,9 +63,11 @@ def _collect(S, ids):
 @pytest.mark.parametrize("multi_rows", [0, 500])
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_server_batched_equals_direct(precision, multi_rows):
-    """multi_rows=500: the requests of 1,600 to 4,096 rows with two or three
-    right-hand sides run on the multi-right-hand-side solver beside the
-    batch, the others in it; every result as the single-system solver's."""
+    """multi_rows=500: the requests of 512 to 4,096 rows run on the runner
+    threads beside the batch (those of two or three right-hand sides on the
+    multi-right-hand-side solver, the single ones on the single-system
+    solver), the 480-row one in the batch; every result as the
+    single-system solver's."""
     reqs = [_request(d, s, k) for d, s, k in REQS]
     S = I.Server("batched", precision=precision, max_slots=8, workers=4,
                  multi_rows=multi_rows)
@@ -87,7 +89,7 @@ def test_server_batched_equals_direct(precision, multi_rows):
 def test_server_alone_equals_in_the_mix(multi_rows):
     """One request served alone, then again while others arrive before,
     during and after it: identical results (multi_rows=1000: it and the
-    larger multi-right-hand-side requests on the runner threads)."""
+    other requests of 1,000 rows or more on the runner threads)."""
     A0, b0 = _request((48, 40, 1), 11, 3)
     S1 = I.Server("batched", max_slots=8, workers=2, multi_rows=multi_rows)
     r0 = S1.submit(A0, b0, grid=(48, 40, 1))
