@@ -747,7 +747,7 @@ def main():
     ap.add_argument("--multi-rows", type=int, default=50000,
                     help="serve: requests of at least this many rows run on the "
                          "multi-right-hand-side solver beside the batch (0: none)")
-    ap.add_argument("--multi-runners", type=int, default=3)
+    ap.add_argument("--multi-runners", type=int, default=4)
     ap.add_argument("--hbm-budget", type=float, default=96.0, help="GiB")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)

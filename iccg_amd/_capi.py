@@ -111,6 +111,8 @@ _SIGS = {
     "iccg_solver_set_timing": (_i, [_vp, _i]),
     "iccg_solver_get_timing": (_i, [_vp, _i, _lp, _dp]),
     "iccg_solver_set_graphs": (_i, [_vp, _i]),
+    "iccg_solver_set_blocking": (_i, [_vp, _i]),
+    "iccg_batch_set_blocking": (_i, [_vp, _i]),
     "iccg_solver_reset_timing": (_i, [_vp]),
     "iccg_msolver_create": (_i, [_vp, _vp, _i, C.POINTER(_vp)]),
     "iccg_msolver_destroy": (_i, [_vp]),

@@ -45,6 +45,7 @@ struct iccg_solver_s {
   int part_slots = 0;  // (hi, lo) pairs the partial buffer holds
   Scal *S = nullptr;
   Scal *hS = nullptr;  // pinned mirror
+  hipEvent_t wait_ev = nullptr;  // blocking waits (iccg_solver_set_blocking)
   bool timing = false;
   // hipGraphs of k consecutive iterations, replayed while the plan is the
   // one they were captured from
@@ -72,6 +73,7 @@ struct iccg_solver_s {
     dfree(part);
     dfree(S);
     if (hS) (void)hipHostFree(hS);
+    if (wait_ev) (void)hipEventDestroy(wait_ev);
     for (auto &e : pool) (void)hipEventDestroy(e);
     for (auto &t : pending) {
       (void)hipEventDestroy(t.a);
@@ -518,7 +520,13 @@ static void build_ic0(iccg_precond_s &M, const iccg_sched_s *s) {
   launch_ic0_factor(M.sc, F, M.shift, derr, st);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(&herr, derr, sizeof(herr), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  // the factorization's waits sleep: preconditioners are often set up on host
+  // threads beside running solves (the serving loop's workers)
+  struct Ev {
+    hipEvent_t e = blocking_event();
+    ~Ev() { (void)hipEventDestroy(e); }
+  } ev;
+  stream_wait(st, ev.e);
   dfree(derr);
   if (herr != ~0ULL) {
     unsigned row = (unsigned)(herr >> 32), pr = (unsigned)(herr & 0xffffffffULL);
@@ -529,7 +537,7 @@ static void build_ic0(iccg_precond_s &M, const iccg_sched_s *s) {
   launch_scatter_u(F, M.mixed, st);
   HIPCHK(hipGetLastError());
   build_sliced_factor(M, lrp, lci, urp, uci, l2u);
-  HIPCHK(hipStreamSynchronize(st));
+  stream_wait(st, ev.e);
   dfree(F.l2u);
   F.l2u = nullptr;
 }
@@ -658,7 +666,7 @@ static void enqueue_iteration(iccg_solver_s &S, int it) {
 
 static void sync_scal(iccg_solver_s &S) {
   HIPCHK(hipMemcpyAsync(S.hS, S.S, sizeof(Scal), hipMemcpyDeviceToHost, S.st));
-  HIPCHK(hipStreamSynchronize(S.st));
+  stream_wait(S.st, S.wait_ev);
 }
 
 // Enqueues k iterations: replays a captured hipGraph of k iterations, or
@@ -1193,6 +1201,18 @@ int iccg_solver_set_timing(iccg_solver_t S, int on) {
   return guard([&] {
     require(S, ICCG_ERR_ARG, "null argument");
     S->timing = on != 0;
+  });
+}
+
+int iccg_solver_set_blocking(iccg_solver_t S, int on) {
+  return guard([&] {
+    require(S, ICCG_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(S->device));
+    if (on && !S->wait_ev) S->wait_ev = blocking_event();
+    if (!on && S->wait_ev) {
+      (void)hipEventDestroy(S->wait_ev);
+      S->wait_ev = nullptr;
+    }
   });
 }
 

@@ -824,6 +824,7 @@ struct iccg_batch_s {
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   hipEvent_t open_ev = nullptr;
+  hipEvent_t wait_ev = nullptr;  // blocking waits (iccg_batch_set_blocking)
   std::vector<Mark> marks;
   double t_ms[4] = {0, 0, 0, 0};
   long t_cnt[4] = {0, 0, 0, 0};
@@ -844,6 +845,7 @@ struct iccg_batch_s {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+    if (wait_ev) (void)hipEventDestroy(wait_ev);
     if (st) (void)hipStreamDestroy(st);
   }
   void drop_graphs() {
@@ -1164,7 +1166,7 @@ void enqueue_iterations(iccg_batch_s &B, int k) {
 void read_scalars(iccg_batch_s &B) {
   HIPCHK(hipMemcpyAsync(B.h_sc, B.d_sc, sizeof(SysScal) * (size_t)B.cap, hipMemcpyDeviceToHost,
                         B.st));
-  HIPCHK(hipStreamSynchronize(B.st));
+  stream_wait(B.st, B.wait_ev);
   if (B.timing) harvest(B);
 }
 
@@ -1591,6 +1593,18 @@ int iccg_batch_trim(iccg_batch_t B) {
         s.hstage = nullptr;
         s.hcap = 0;
       }
+  });
+}
+
+int iccg_batch_set_blocking(iccg_batch_t B, int on) {
+  return guard([&] {
+    require(B, ICCG_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(B->device));
+    if (on && !B->wait_ev) B->wait_ev = blocking_event();
+    if (!on && B->wait_ev) {
+      (void)hipEventDestroy(B->wait_ev);
+      B->wait_ev = nullptr;
+    }
   });
 }
 

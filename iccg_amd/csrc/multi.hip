@@ -915,6 +915,7 @@ static int mwm(const DevCsr &A) {
 struct MSet {
   int K = 0, m = 0, first = 0;  // members used of K, first right-hand side
   hipStream_t st = nullptr;     // the set's own stream: sets of one solver run concurrently
+  hipEvent_t wait_ev = nullptr;  // blocking waits on st
   double *X = nullptr, *R = nullptr, *P = nullptr, *AP = nullptr, *B = nullptr;
   void *Y = nullptr, *Z = nullptr;
   double *part = nullptr;
@@ -937,6 +938,7 @@ struct MSet {
     dfree(part);
     dfree(S);
     if (hS) (void)hipHostFree(hS);
+    if (wait_ev) (void)hipEventDestroy(wait_ev);
     if (st) (void)hipStreamDestroy(st);
   }
 };
@@ -1162,9 +1164,11 @@ struct iccg_msolver_s {
 
 namespace iccg {
 
+// the set's scalars to the host; the wait sleeps (the sets of one solver,
+// and the serving loop's runners, wait on host threads of their own)
 static void mset_sync(iccg_msolver_s &S, MSet &s) {
   HIPCHK(hipMemcpyAsync(s.hS, s.S, sizeof(Scal) * s.K, hipMemcpyDeviceToHost, s.st));
-  HIPCHK(hipStreamSynchronize(s.st));
+  stream_wait(s.st, s.wait_ev);
 }
 
 // k iterations: a replayed hipGraph, or (timing) launched one by one with an
@@ -1397,6 +1401,7 @@ int iccg_msolver_create(iccg_csr_t A, iccg_precond_t M, int nrhs, iccg_msolver_t
     for (MSet &s : S->sets) {
       const size_t nk = (size_t)n * s.K;
       HIPCHK(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
+      s.wait_ev = blocking_event();
       s.X = dalloc<double>(nk);
       s.R = dalloc<double>(nk);
       s.P = dalloc<double>(nk);
@@ -1416,6 +1421,7 @@ int iccg_msolver_create(iccg_csr_t A, iccg_precond_t M, int nrhs, iccg_msolver_t
                            hipHostMallocDefault));
     if (!S->lone.empty()) {
       int st = iccg_solver_create(A, M, &S->single);
+      if (st == ICCG_OK) st = iccg_solver_set_blocking(S->single, 1);
       if (st != ICCG_OK) throw Error(st, iccg_last_error());
       S->hb.assign((size_t)nrhs * n, 0.0);
       S->hx.assign((size_t)nrhs * n, 0.0);

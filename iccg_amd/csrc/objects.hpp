@@ -45,6 +45,27 @@ inline void dfree(void *p) {
   if (p) (void)hipFree(p);
 }
 
+// Waits for the work queued on st: through a blocking-sync event when ev is
+// set (the host thread sleeps until the device signals it), else with
+// hipStreamSynchronize, which spins while the device works.  Threads that
+// wait beside others on a shared host (the serving loop's engine, runners
+// and their sets) use the event, so that their waits leave the cores to the
+// threads preparing requests.
+inline void stream_wait(hipStream_t st, hipEvent_t ev) {
+  if (ev) {
+    HIPCHK(hipEventRecord(ev, st));
+    HIPCHK(hipEventSynchronize(ev));
+  } else {
+    HIPCHK(hipStreamSynchronize(st));
+  }
+}
+
+inline hipEvent_t blocking_event() {
+  hipEvent_t e = nullptr;
+  HIPCHK(hipEventCreateWithFlags(&e, hipEventBlockingSync | hipEventDisableTiming));
+  return e;
+}
+
 inline void free_sell(Sell &S) {
   dfree(S.off);
   dfree(S.col);

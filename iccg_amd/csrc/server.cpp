@@ -562,6 +562,7 @@ int iccg_server_create(const iccg_server_config *cfg, iccg_server_t *out) {
     ck(iccg_device_get(&S->device));
     if (cfg->mode == ICCG_SERVE_BATCHED) {
       ck(iccg_batch_create(cfg->max_slots, &S->B));
+      ck(iccg_batch_set_blocking(S->B, 1));  // the engine's waits sleep
       S->owner.assign((size_t)cfg->max_slots, {nullptr, 0});
       S->free_slots = cfg->max_slots;
       const int nw = cfg->workers > 0 ? cfg->workers : 8;

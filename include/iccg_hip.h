@@ -287,6 +287,10 @@ int iccg_solver_get_x(iccg_solver_t S, double *x);
 int iccg_solver_set_timing(iccg_solver_t S, int on);
 /* Replay captured hipGraphs of whole iterations (default on). */
 int iccg_solver_set_graphs(iccg_solver_t S, int on);
+/* Waits for the device inside iccg_solver_run sleep on a blocking-sync
+ * event instead of spinning (default off), for callers that run solves on
+ * several host threads beside other host work. */
+int iccg_solver_set_blocking(iccg_solver_t S, int on);
 /* Kernel classes: 0 SpMV(+dots), 1 forward sweep, 2 backward sweep,
  * 3 vector updates.  Returns launches and summed milliseconds since the last
  * reset. */
@@ -356,6 +360,8 @@ int iccg_batch_get_x(iccg_batch_t B, int slot, double *x);
 /* Frees a slot (finished or not). */
 int iccg_batch_remove(iccg_batch_t B, int slot);
 int iccg_batch_set_graphs(iccg_batch_t B, int on);
+/* iccg_solver_set_blocking for the batch's waits in iccg_batch_run. */
+int iccg_batch_set_blocking(iccg_batch_t B, int on);
 /* Work decomposition of the batched kernels: 64-row slices per workgroup of
  * the SpMV / vector kernels (chunk_slices) and 64-row slices (multicolor)
  * or blocks (ABMC) per wavefront of the triangular sweeps (sweep_units).
