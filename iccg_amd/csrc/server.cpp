@@ -74,6 +74,7 @@ struct Request {
   std::vector<double> x;
   std::vector<iccg_cg_result> res;
   int remaining = 0;
+  int readers = 0;  // result() calls copying x outside the lock
   int state = Q_WAIT;
   int status = ICCG_OK;
   std::string err;
@@ -124,11 +125,11 @@ struct iccg_server_s {
   struct Reap {
     iccg_csr_t A;
     iccg_precond_t M;
+    iccg_msolver_t MS;
     long long hbm;
   };
   std::deque<Reap> reaps;
   std::condition_variable cv_reap;
-  int untrimmed = 0;  // engine thread: requests finished since the last trim
   // engine-thread state
   iccg_batch_t B = nullptr;
   std::vector<std::pair<Request *, int>> owner;  // slot -> (request, rhs)
@@ -171,12 +172,13 @@ struct iccg_server_s {
     cv_work.notify_all();
   }
 
-  // under mu: the request's device objects go to the reaper thread.  Freeing
+  // under mu: the request's device objects (and the runner's solver of its
+  // right-hand sides) go to the reaper thread.  Freeing
   // device memory (hipFree) waits for all work queued on the device, the
   // runners' long captured chunks included: on the reaper that wait delays
   // no engine, runner or worker thread.
-  void reap_locked(Request *r) {
-    reaps.push_back({r->A, r->M, r->hbm});
+  void reap_locked(Request *r, iccg_msolver_t ms = nullptr) {
+    reaps.push_back({r->A, r->M, ms, r->hbm});
     r->A = nullptr;
     r->M = nullptr;
     cv_reap.notify_one();
@@ -193,6 +195,7 @@ struct iccg_server_s {
         t = reaps.front();
         reaps.pop_front();
       }
+      if (t.MS) iccg_msolver_destroy(t.MS);
       if (t.M) iccg_precond_destroy(t.M);
       if (t.A) iccg_csr_destroy(t.A);
       std::lock_guard<std::mutex> lk(mu);
@@ -350,13 +353,9 @@ struct iccg_server_s {
       }
       int st = ICCG_OK;
       std::string err;
+      iccg_msolver_t M = nullptr;  // freed by the reaper, with the request
       try {
-        iccg_msolver_t M = nullptr;
         ck(iccg_msolver_create(r->A, r->M, r->nrhs, &M));
-        struct Guard {
-          iccg_msolver_t m;
-          ~Guard() { iccg_msolver_destroy(m); }
-        } g{M};
         ck(iccg_msolver_set(M, r->bp.data(), nullptr));
         ck(iccg_msolver_run(M, r->max_iter, r->tol, cfg.flags, r->res.data()));
         xs.resize((size_t)r->nrhs * r->n);
@@ -371,7 +370,7 @@ struct iccg_server_s {
       }
       std::lock_guard<std::mutex> lk(mu);
       running_multi--;
-      reap_locked(r);
+      reap_locked(r, M);
       complete_locked(r, st, err, true);
     }
   }
@@ -444,17 +443,16 @@ struct iccg_server_s {
           ck(iccg_batch_remove(B, s));
           owner[(size_t)s] = {nullptr, 0};
           if (--r->remaining == 0) {
-            // the request's vectors leave HBM with it; while runners have
-            // work queued (which a free waits for) every 8th request
+            // the request's vectors leave HBM with it, once no runner has
+            // work queued: a free waits for all the device's queued work,
+            // and the batch holds the small requests, whose idle slot
+            // vectors are small
             int multi = 0;
             {
               std::lock_guard<std::mutex> lk(mu);
               multi = running_multi;
             }
-            if (multi == 0 || ++untrimmed >= 8) {
-              ck(iccg_batch_trim(B));
-              untrimmed = 0;
-            }
+            if (multi == 0) ck(iccg_batch_trim(B));
             std::lock_guard<std::mutex> lk(mu);
             free_slots += r->nrhs;
             running_reqs--;
@@ -662,27 +660,43 @@ int iccg_server_result(iccg_server_t S, long long id, double *x, iccg_cg_result 
                        double *latency_ms, double *prepare_ms) {
   return guard([&] {
     require(S, ICCG_ERR_ARG, "null argument");
+    Request *r = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      auto it = S->all.find(id);
+      require(it != S->all.end(), ICCG_ERR_ARG, "unknown request id");
+      r = it->second.get();
+      require(r->state == Q_DONE || r->state == Q_FAILED, ICCG_ERR_STATE,
+              "request not finished");
+      if (r->state == Q_FAILED) throw Error(r->status, r->err);
+      if (res) std::memcpy(res, r->res.data(), sizeof(iccg_cg_result) * r->res.size());
+      if (latency_ms) *latency_ms = ms_between(r->t_submit, r->t_done);
+      if (prepare_ms) *prepare_ms = ms_between(r->t_submit, r->t_ready);
+      if (!x) return;
+      r->readers++;  // x is final; release() waits for the copy
+    }
+    // x (up to nrhs n doubles) is copied without the lock, which the
+    // engine, runner and worker threads take for every request they move
+    std::memcpy(x, r->x.data(), sizeof(double) * r->x.size());
     std::lock_guard<std::mutex> lk(S->mu);
-    auto it = S->all.find(id);
-    require(it != S->all.end(), ICCG_ERR_ARG, "unknown request id");
-    Request *r = it->second.get();
-    require(r->state == Q_DONE || r->state == Q_FAILED, ICCG_ERR_STATE, "request not finished");
-    if (r->state == Q_FAILED) throw Error(r->status, r->err);
-    if (x) std::memcpy(x, r->x.data(), sizeof(double) * r->x.size());
-    if (res) std::memcpy(res, r->res.data(), sizeof(iccg_cg_result) * r->res.size());
-    if (latency_ms) *latency_ms = ms_between(r->t_submit, r->t_done);
-    if (prepare_ms) *prepare_ms = ms_between(r->t_submit, r->t_ready);
+    if (--r->readers == 0) S->cv_done.notify_all();
   });
 }
 
 int iccg_server_release(iccg_server_t S, long long id) {
   return guard([&] {
     require(S, ICCG_ERR_ARG, "null argument");
-    std::lock_guard<std::mutex> lk(S->mu);
+    std::unique_lock<std::mutex> lk(S->mu);
     auto it = S->all.find(id);
     require(it != S->all.end(), ICCG_ERR_ARG, "unknown request id");
     const int st = it->second->state;
     require(st == Q_DONE || st == Q_FAILED, ICCG_ERR_STATE, "request not finished");
+    S->cv_done.wait(lk, [&] {  // a result() copy of x in progress finishes first
+      auto j = S->all.find(id);
+      return j == S->all.end() || j->second->readers == 0;
+    });
+    it = S->all.find(id);
+    require(it != S->all.end(), ICCG_ERR_ARG, "unknown request id");
     S->all.erase(it);
   });
 }
