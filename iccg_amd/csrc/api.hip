@@ -62,6 +62,7 @@ struct iccg_solver_s {
   }
   ~iccg_solver_s() {
     (void)hipSetDevice(device);
+    if (st) (void)hipStreamSynchronize(st);  // nothing queued still uses its vectors
     drop_graphs();
     dfree(x);
     dfree(r);
@@ -72,7 +73,7 @@ struct iccg_solver_s {
     dfree(z);
     dfree(part);
     dfree(S);
-    if (hS) (void)hipHostFree(hS);
+    hfree(hS);
     if (wait_ev) (void)hipEventDestroy(wait_ev);
     for (auto &e : pool) (void)hipEventDestroy(e);
     for (auto &t : pending) {
@@ -1013,6 +1014,7 @@ int iccg_precond_set_schedule(iccg_precond_t M, iccg_sched_t sched) {
     tmp.n = M->n;
     tmp.st = M->st;
     build_schedule(tmp, sched, lrp, lci, urp, uci);
+    HIPCHK(hipStreamSynchronize(M->st));  // before the old plan's memory is freed
     std::swap(M->sc, tmp.sc);
     // an ABMC factor's sliced copies may be laid out by the old schedule's
     // blocks (sectioned layout): lay them out again for the new one
@@ -1154,7 +1156,7 @@ int iccg_solver_create(iccg_csr_t A, iccg_precond_t M, iccg_solver_t *out) {
     HIPCHK(hipMemsetAsync(S->S, 0, sizeof(Scal), S->st));
     HIPCHK(hipMemsetAsync(S->x, 0, sizeof(double) * (size_t)n, S->st));
     HIPCHK(hipMemsetAsync(S->b, 0, sizeof(double) * (size_t)n, S->st));
-    HIPCHK(hipHostMalloc((void **)&S->hS, sizeof(Scal), hipHostMallocDefault));
+    S->hS = halloc<Scal>(1);
     HIPCHK(hipStreamSynchronize(S->st));
     *out = S.release();
   });

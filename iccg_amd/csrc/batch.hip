@@ -835,13 +835,13 @@ struct iccg_batch_s {
     running.clear();
     for (auto &s : slots) {
       dfree(s.dbuf);
-      if (s.hstage) (void)hipHostFree(s.hstage);
+      hfree(s.hstage);
     }
     dfree(d_desc);
     dfree(d_sc);
     dfree(part);
-    if (h_sc) (void)hipHostFree(h_sc);
-    if (h_desc) (void)hipHostFree(h_desc);
+    hfree(h_sc);
+    hfree(h_desc);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
@@ -871,6 +871,7 @@ void ensure_part(iccg_batch_s &B, size_t slots) {
 // Slots of one matrix and preconditioner are grouped, kMaxRhs at most, in
 // slot order; fin lists the slots in the order of ids.
 void build_items(iccg_batch_s &B, const std::vector<int> &ids, ItemSet &I, int ch, int cw) {
+  if (I.buf) HIPCHK(hipStreamSynchronize(B.st));  // launches reading the old lists have run
   I.clear();
   std::vector<GroupDesc> groups;
   std::map<std::pair<const void *, const void *>, int> open;  // (A, M) -> group with room
@@ -1365,10 +1366,8 @@ int iccg_batch_create(int capacity, iccg_batch_t *out) {
     HIPCHK(hipStreamCreateWithFlags(&B->st, hipStreamNonBlocking));
     B->d_desc = dalloc<SysDesc>((size_t)capacity);
     B->d_sc = dalloc<SysScal>((size_t)capacity);
-    HIPCHK(hipHostMalloc((void **)&B->h_sc, sizeof(SysScal) * (size_t)capacity,
-                         hipHostMallocDefault));
-    HIPCHK(hipHostMalloc((void **)&B->h_desc, sizeof(SysDesc) * (size_t)capacity,
-                         hipHostMallocDefault));
+    B->h_sc = halloc<SysScal>((size_t)capacity);
+    B->h_desc = halloc<SysDesc>((size_t)capacity);
     std::memset((void *)B->h_sc, 0, sizeof(SysScal) * (size_t)capacity);
     for (int k = 0; k < capacity; ++k) B->h_desc[k] = SysDesc();
     HIPCHK(hipMemcpyAsync(B->d_sc, B->h_sc, sizeof(SysScal) * (size_t)capacity,
@@ -1418,10 +1417,10 @@ int iccg_batch_add(iccg_batch_t B, iccg_csr_t A, iccg_precond_t M, const double 
     }
     const size_t hneed = sizeof(double) * 2 * (size_t)n;
     if (s.hcap < hneed) {
-      if (s.hstage) (void)hipHostFree(s.hstage);
+      hfree(s.hstage);
       s.hstage = nullptr;
       s.hcap = 0;
-      HIPCHK(hipHostMalloc((void **)&s.hstage, hneed, hipHostMallocDefault));
+      s.hstage = halloc<double>(2 * (size_t)n);
       s.hcap = hneed;
     }
     std::memcpy(s.hstage, b, sizeof(double) * (size_t)n);
@@ -1589,7 +1588,7 @@ int iccg_batch_trim(iccg_batch_t B) {
         dfree(s.dbuf);
         s.dbuf = nullptr;
         s.dcap = 0;
-        if (s.hstage) (void)hipHostFree(s.hstage);
+        hfree(s.hstage);
         s.hstage = nullptr;
         s.hcap = 0;
       }

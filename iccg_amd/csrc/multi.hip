@@ -927,6 +927,7 @@ struct MSet {
     execs.clear();
   }
   void release() {
+    if (st) (void)hipStreamSynchronize(st);  // nothing queued still uses its vectors
     drop_graphs();
     dfree(X);
     dfree(R);
@@ -937,7 +938,7 @@ struct MSet {
     dfree(Z);
     dfree(part);
     dfree(S);
-    if (hS) (void)hipHostFree(hS);
+    hfree(hS);
     if (wait_ev) (void)hipEventDestroy(wait_ev);
     if (st) (void)hipStreamDestroy(st);
   }
@@ -1157,7 +1158,7 @@ struct iccg_msolver_s {
     for (auto e : pool) (void)hipEventDestroy(e);
     for (auto &s : sets) s.release();
     if (single) iccg_solver_destroy(single);
-    if (stage) (void)hipHostFree(stage);
+    hfree(stage);
     if (st) (void)hipStreamDestroy(st);
   }
 };
@@ -1414,11 +1415,10 @@ int iccg_msolver_create(iccg_csr_t A, iccg_precond_t M, int nrhs, iccg_msolver_t
       HIPCHK(hipMemsetAsync(s.S, 0, sizeof(Scal) * 4, S->st));
       HIPCHK(hipMemsetAsync(s.X, 0, sizeof(double) * nk, S->st));
       HIPCHK(hipMemsetAsync(s.B, 0, sizeof(double) * nk, S->st));
-      HIPCHK(hipHostMalloc((void **)&s.hS, sizeof(Scal) * 4, hipHostMallocDefault));
+      s.hS = halloc<Scal>(4);
     }
     if (!S->sets.empty())
-      HIPCHK(hipHostMalloc((void **)&S->stage, sizeof(double) * 4 * (size_t)std::max(n, 1),
-                           hipHostMallocDefault));
+      S->stage = halloc<double>(4 * (size_t)std::max(n, 1));
     if (!S->lone.empty()) {
       int st = iccg_solver_create(A, M, &S->single);
       if (st == ICCG_OK) st = iccg_solver_set_blocking(S->single, 1);

@@ -5,7 +5,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
+#include <iterator>
+#include <map>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "dev.hpp"
@@ -23,14 +28,60 @@ int current_device();
       throw Error(ICCG_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
   } while (0)
 
+// Device memory comes from a stream-ordered pool of the device's own,
+// allocated and freed on a stream that carries nothing else.  hipFree waits
+// until every stream of the device is idle -- the serving loop keeps chunks
+// of up to 64 iterations of multi-million-row systems queued on its runner
+// streams -- and holds other threads' allocations while it waits; a pool
+// free waits for nothing.  The pool keeps the memory freed into it (release
+// threshold at its maximum), so the next allocation reuses it rather than
+// asking the driver.  A free is therefore no longer a device-wide wait:
+// every object waits for its own stream before it frees, and memory is
+// freed only once the work that uses it has finished.
+struct DevicePool {
+  hipMemPool_t pool = nullptr;
+  hipStream_t st = nullptr;
+};
+
+inline DevicePool device_pool() {
+  static std::mutex mu;
+  static std::vector<DevicePool> pools;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)pools.size() <= dev) pools.resize((size_t)dev + 1);
+  DevicePool &P = pools[(size_t)dev];
+  if (!P.pool) {
+    hipMemPoolProps props{};
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = dev;
+    hipMemPool_t pool = nullptr;
+    HIPCHK(hipMemPoolCreate(&pool, &props));
+    uint64_t keep = ~0ULL;
+    HIPCHK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+    HIPCHK(hipStreamCreateWithFlags(&P.st, hipStreamNonBlocking));
+    P.pool = pool;
+  }
+  return P;
+}
+
 template <class T>
 inline T *dalloc(size_t count) {
   if (count == 0) count = 1;
+  const size_t bytes = count * sizeof(T);
+  const DevicePool P = device_pool();
   void *p = nullptr;
-  hipError_t e = hipMalloc(&p, count * sizeof(T));
+  hipError_t e = hipMallocFromPoolAsync(&p, bytes, P.pool, P.st);
+  if (e == hipSuccess) e = hipStreamSynchronize(P.st);  // usable on every stream
   if (e != hipSuccess)
-    throw Error(ICCG_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+    throw Error(ICCG_ERR_NOMEM, "device allocation of " + std::to_string(bytes) +
                                     " bytes failed: " + hipGetErrorString(e));
+  if (((uintptr_t)p & 255u) != 0) {
+    (void)hipFreeAsync(p, P.st);
+    throw Error(ICCG_ERR_DEVICE, "device allocation not 256-byte aligned");
+  }
   return (T *)p;
 }
 
@@ -41,8 +92,79 @@ inline T *dupload(const T *h, size_t count, hipStream_t st) {
   return d;
 }
 
+// returns p to its pool at once (the caller has waited for the work using it)
 inline void dfree(void *p) {
-  if (p) (void)hipFree(p);
+  if (!p) return;
+  try {
+    (void)hipFreeAsync(p, device_pool().st);
+  } catch (...) {  // destructors free: no exception leaves here
+  }
+}
+
+// Pinned host staging (hipHostMalloc) from a process-wide cache: a block
+// freed here waits, idle, for the next request of at most twice its size,
+// and at most kPinnedKeep bytes wait so.  hipHostFree, like hipFree, waits
+// for the whole device; pinning fresh pages costs milliseconds per megabyte.
+constexpr size_t kPinnedKeep = size_t(4) << 30;
+
+struct PinnedCache {
+  std::mutex mu;
+  std::multimap<size_t, void *> idle;         // bytes -> block
+  std::unordered_map<void *, size_t> blocks;  // every block of the cache, idle or in use
+  size_t idle_bytes = 0;
+};
+
+inline PinnedCache &pinned_cache() {
+  static PinnedCache c;
+  return c;
+}
+
+template <class T>
+inline T *halloc(size_t count) {
+  const size_t bytes = std::max<size_t>(count * sizeof(T), 1);
+  PinnedCache &c = pinned_cache();
+  {
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto it = c.idle.lower_bound(bytes);
+    if (it != c.idle.end() && it->first <= 2 * bytes) {
+      void *p = it->second;
+      c.idle_bytes -= it->first;
+      c.idle.erase(it);
+      return (T *)p;
+    }
+  }
+  void *p = nullptr;
+  hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+  if (e != hipSuccess)
+    throw Error(ICCG_ERR_NOMEM, "pinned host allocation of " + std::to_string(bytes) +
+                                    " bytes failed: " + hipGetErrorString(e));
+  std::lock_guard<std::mutex> lk(c.mu);
+  c.blocks[p] = bytes;
+  return (T *)p;
+}
+
+inline void hfree(void *p) {
+  if (!p) return;
+  PinnedCache &c = pinned_cache();
+  std::vector<void *> drop;
+  {
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto it = c.blocks.find(p);
+    if (it == c.blocks.end()) {
+      drop.push_back(p);
+    } else {
+      c.idle.emplace(it->second, p);
+      c.idle_bytes += it->second;
+      while (c.idle_bytes > kPinnedKeep) {  // the largest idle blocks go first
+        auto last = std::prev(c.idle.end());
+        c.idle_bytes -= last->first;
+        c.blocks.erase(last->second);
+        drop.push_back(last->second);
+        c.idle.erase(last);
+      }
+    }
+  }
+  for (void *q : drop) (void)hipHostFree(q);
 }
 
 // Waits for the work queued on st: through a blocking-sync event when ev is
@@ -251,6 +373,7 @@ struct iccg_csr_s {
   hipStream_t st = nullptr;
   ~iccg_csr_s() {
     (void)hipSetDevice(device);
+    if (st) (void)hipStreamSynchronize(st);  // its uploads have landed
     dfree(d.rp);
     dfree(d.ci);
     dfree(d.v);
@@ -280,6 +403,7 @@ struct iccg_precond_s {
   int plan_version = 0;  // bumped whenever the device plan (sc) is replaced
   ~iccg_precond_s() {
     (void)hipSetDevice(device);
+    if (st) (void)hipStreamSynchronize(st);  // no work of its own is left in flight
     dfree(inv_diag);
     dfree(F.lrp);
     dfree(F.lci);
