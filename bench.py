@@ -496,14 +496,17 @@ def drive(server, reqs, arrivals):
         if dt > 0:
             time.sleep(dt)
         ids.append(server.submit(A, b, grid=dims))
-    out = []
-    for rid in ids:
-        server.wait(rid)
+    # collected as they finish (the progress lines are completion times),
+    # returned in submission order
+    where = {rid: k for k, rid in enumerate(ids)}
+    out = [None] * len(ids)
+    for done in range(1, len(ids) + 1):
+        rid = server.wait()
         x, res, lat, prep = server.result(rid)
-        out.append((lat, prep, res, x))
+        out[where[rid]] = (lat, prep, res, x)
         server.release(rid)
-        if len(out) % 16 == 0 or len(out) == len(ids):
-            log(f"  {len(out)}/{len(ids)} requests done ({time.perf_counter() - t0:.1f}s)")
+        if done % 16 == 0 or done == len(ids):
+            log(f"  {done}/{len(ids)} requests done ({time.perf_counter() - t0:.1f}s)")
     return out, time.perf_counter() - t0
 
 
@@ -747,7 +750,7 @@ def main():
     ap.add_argument("--multi-rows", type=int, default=50000,
                     help="serve: requests of at least this many rows run on the "
                          "multi-right-hand-side solver beside the batch (0: none)")
-    ap.add_argument("--multi-runners", type=int, default=4)
+    ap.add_argument("--multi-runners", type=int, default=6)
     ap.add_argument("--hbm-budget", type=float, default=96.0, help="GiB")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)

@@ -725,7 +725,7 @@ class Server:
 
     def __init__(self, mode="batched", precision="mixed", max_slots=256, workers=8,
                  block_size=64, true_residual=None, hbm_budget=64 << 30,
-                 multi_rows=50000, multi_runners=4):
+                 multi_rows=50000, multi_runners=6):
         prec = {"fp64": _capi.FP64, "mixed": _capi.MIXED}[precision]
         if true_residual is None:
             true_residual = prec == _capi.MIXED

@@ -570,7 +570,7 @@ int iccg_server_create(const iccg_server_config *cfg, iccg_server_t *out) {
       S->engine = std::thread([p = S.get()] { p->engine_loop(); });
       S->reaper = std::thread([p = S.get()] { p->reaper_loop(); });
       if (cfg->multi_rows > 0) {
-        const int nr = cfg->multi_runners > 0 ? cfg->multi_runners : 2;
+        const int nr = cfg->multi_runners > 0 ? cfg->multi_runners : 6;
         for (int k = 0; k < nr; ++k) S->runners.emplace_back([p = S.get()] { p->multi_loop(); });
       }
     } else {

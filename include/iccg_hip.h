@@ -410,7 +410,7 @@ int iccg_batch_trim(iccg_batch_t B);
  *                         of max_slots (continuous batching: slots freed by
  *                         finished solves take the next ready request);
  *                         requests of at least multi_rows rows (0: none) go
- *                         instead to multi_runners threads (0 = 2), each
+ *                         instead to multi_runners threads (0 = 6), each
  *                         solving one request at a time with the
  *                         multi-right-hand-side solver (iccg_msolver_*; one
  *                         right-hand side runs on the single-system solver)
@@ -431,7 +431,7 @@ typedef struct {
   int flags;              /* ICCG_SOLVE_TRUE_RESID */
   long long hbm_budget;   /* bytes of HBM for admitted requests */
   int multi_rows;         /* batched mode: see above (0 = every request batched) */
-  int multi_runners;      /* threads running those requests (0 = 2) */
+  int multi_runners;      /* threads running those requests (0 = 6) */
 } iccg_server_config;
 /* hbm_in_use: the admitted requests' footprint estimates, a finished
  * request's until the reaper thread has freed its device memory. */
