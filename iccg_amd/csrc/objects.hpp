@@ -43,12 +43,18 @@ struct DevicePool {
   hipStream_t st = nullptr;
 };
 
+struct DevicePools {
+  std::mutex mu;
+  std::vector<DevicePool> pools;
+};
+
 inline DevicePool device_pool() {
-  static std::mutex mu;
-  static std::vector<DevicePool> pools;
+  // never destroyed: objects may be freed during static destruction at exit
+  static DevicePools &all = *new DevicePools;
+  std::vector<DevicePool> &pools = all.pools;
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lk(mu);
+  std::lock_guard<std::mutex> lk(all.mu);
   if ((int)pools.size() <= dev) pools.resize((size_t)dev + 1);
   DevicePool &P = pools[(size_t)dev];
   if (!P.pool) {
@@ -115,7 +121,7 @@ struct PinnedCache {
 };
 
 inline PinnedCache &pinned_cache() {
-  static PinnedCache c;
+  static PinnedCache &c = *new PinnedCache;  // never destroyed, as the device pools
   return c;
 }
 
