@@ -79,6 +79,36 @@ __device__ void block_partial(double (&h)[NC], double (&l)[NC], double *part, in
   }
 }
 
+// Adds this thread's slots b = tid, tid + T, tid + 2T, ... (T threads) of
+// NC slot-major partials, in ascending order, into (h, l); the loads of
+// kFoldAhead slots go out before their adds, instead of one slot's loads
+// per memory round trip.
+constexpr int kFoldAhead = 4;
+template <int NC>
+__device__ __forceinline__ void fold_slots(const double *__restrict__ p, int count, double *h,
+                                           double *l) {
+  if (NC == 0) return;
+  const int T = blockDim.x;
+  for (int b0 = threadIdx.x; b0 < count; b0 += kFoldAhead * T) {
+    double vh[kFoldAhead][NC > 0 ? NC : 1], vl[kFoldAhead][NC > 0 ? NC : 1];
+#pragma unroll
+    for (int u = 0; u < kFoldAhead; ++u) {
+      const int b = b0 + u * T;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        vh[u][c] = b < count ? p[((size_t)b * NC + c) * 2] : 0.0;
+        vl[u][c] = b < count ? p[((size_t)b * NC + c) * 2 + 1] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kFoldAhead; ++u)
+      if (b0 + u * T < count) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) dd_add(h[c], l[c], vh[u][c], vl[u][c]);
+      }
+  }
+}
+
 // One workgroup folds the partials of one reduction (NA components per
 // slot at pa, ca slots) or of two (and NB at pb, cb slots).  Every component
 // is folded in the same fixed tree: per thread the slots b = tid, tid +
@@ -97,15 +127,8 @@ __device__ bool finish_reduce2(const double *__restrict__ pa, int ca,
   double h[N], l[N];
 #pragma unroll
   for (int c = 0; c < N; ++c) h[c] = l[c] = 0.0;
-  for (int b = threadIdx.x; b < ca; b += blockDim.x) {
-#pragma unroll
-    for (int c = 0; c < NA; ++c) dd_add(h[c], l[c], pa[(b * NA + c) * 2], pa[(b * NA + c) * 2 + 1]);
-  }
-  for (int b = threadIdx.x; b < cb; b += blockDim.x) {
-#pragma unroll
-    for (int c = 0; c < NB; ++c)
-      dd_add(h[NA + c], l[NA + c], pb[(b * NB + c) * 2], pb[(b * NB + c) * 2 + 1]);
-  }
+  fold_slots<NA>(pa, ca, h, l);
+  fold_slots<NB>(pb, cb, h + NA, l + NA);
   wave_dd_reduce<N>(h, l);
   if (lane == 0) {
 #pragma unroll

@@ -95,6 +95,35 @@ __device__ void block_partial_cm(double (&h)[NC], double (&l)[NC], double *part,
   }
 }
 
+// Adds this thread's slots b = tid, tid + T, tid + 2T, ... (T threads) of NC
+// component-major partials, in ascending order, into (h, l).  The loads of
+// kFoldAhead slots go out before their adds: one slot at a time, a finish
+// over 2 x 4,096 slots waited for 32 memory round trips in a row.
+template <int NC>
+__device__ __forceinline__ void fold_slots_cm(const double *__restrict__ p, int count,
+                                              int stride, double *h, double *l) {
+  if (NC == 0) return;
+  const int T = blockDim.x;
+  for (int b0 = threadIdx.x; b0 < count; b0 += kFoldAhead * T) {
+    double vh[kFoldAhead][NC > 0 ? NC : 1], vl[kFoldAhead][NC > 0 ? NC : 1];
+#pragma unroll
+    for (int u = 0; u < kFoldAhead; ++u) {
+      const int b = b0 + u * T;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        vh[u][c] = b < count ? p[((size_t)c * stride + b) * 2] : 0.0;
+        vl[u][c] = b < count ? p[((size_t)c * stride + b) * 2 + 1] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kFoldAhead; ++u)
+      if (b0 + u * T < count) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) dd_add(h[c], l[c], vh[u][c], vl[u][c]);
+      }
+  }
+}
+
 // finish_reduce2 over component-major partials (ca slots of stride sa at
 // pa, cb of stride sb at pb): the same fold, in the same order
 template <int NA, int NB>
@@ -109,16 +138,8 @@ __device__ bool finish_reduce2_cm(const double *__restrict__ pa, int ca, int sa,
   double h[N], l[N];
 #pragma unroll
   for (int c = 0; c < N; ++c) h[c] = l[c] = 0.0;
-  for (int b = threadIdx.x; b < ca; b += blockDim.x) {
-#pragma unroll
-    for (int c = 0; c < NA; ++c)
-      dd_add(h[c], l[c], pa[((size_t)c * sa + b) * 2], pa[((size_t)c * sa + b) * 2 + 1]);
-  }
-  for (int b = threadIdx.x; b < cb; b += blockDim.x) {
-#pragma unroll
-    for (int c = 0; c < NB; ++c)
-      dd_add(h[NA + c], l[NA + c], pb[((size_t)c * sb + b) * 2], pb[((size_t)c * sb + b) * 2 + 1]);
-  }
+  fold_slots_cm<NA>(pa, ca, sa, h, l);
+  fold_slots_cm<NB>(pb, cb, sb, h + NA, l + NA);
   wave_dd_reduce<N>(h, l);
   if (lane == 0) {
 #pragma unroll
