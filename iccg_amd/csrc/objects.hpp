@@ -80,6 +80,13 @@ inline T *dalloc(size_t count) {
   const DevicePool P = device_pool();
   void *p = nullptr;
   hipError_t e = hipMallocFromPoolAsync(&p, bytes, P.pool, P.st);
+  if (e == hipErrorOutOfMemory) {
+    // the pool's idle memory goes back to the device, then one more try
+    (void)hipGetLastError();
+    e = hipStreamSynchronize(P.st);
+    if (e == hipSuccess) e = hipMemPoolTrimTo(P.pool, 0);
+    if (e == hipSuccess) e = hipMallocFromPoolAsync(&p, bytes, P.pool, P.st);
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(P.st);  // usable on every stream
   if (e != hipSuccess)
     throw Error(ICCG_ERR_NOMEM, "device allocation of " + std::to_string(bytes) +
