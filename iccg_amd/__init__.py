@@ -11,7 +11,7 @@ from .api import (CRS, CGResult, Batch, ColorSchedule, DeviceCSR, IC0, Identity,
                   Permute_PtAP_LowerCRS_to_LowerCRS, ReadBlockColorFile_1Based,
                   ReadBlockFile_1Based, ReadColorFile_1Based, abmc_blocking,
                   abmc_blocking_grid, bench_copy, check_hazards, check_interblock,
-                  conjugate_gradient, coo_to_crs, device_count, expand_lower_to_full,
+                  conjugate_gradient, coo_to_crs, device_count, device_trim, expand_lower_to_full,
                   gen_grid, gen_normal, greedy_color, read_matrix_market_crs,
                   write_block_color_file, write_block_file, write_color_file)
 from ._capi import IccgError

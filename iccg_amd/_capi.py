@@ -65,6 +65,7 @@ _SIGS = {
     "iccg_status_string": (C.c_char_p, [_i]),
     "iccg_device_count": (_i, [_ip]),
     "iccg_device_set": (_i, [_i]),
+    "iccg_device_trim": (_i, []),
     "iccg_mm_read": (_i, [C.c_char_p, _ip, _lp, C.POINTER(_ip), C.POINTER(_ip), C.POINTER(_dp)]),
     "iccg_free_host": (None, [_vp]),
     "iccg_coo_to_crs": (_i, [_i, _l, _vp, _vp, _vp, _vp, _vp, _vp, _lp]),

@@ -622,6 +622,13 @@ def device_count():
     return c.value
 
 
+def device_trim():
+    """Hands the device memory and pinned host staging that the library
+    keeps idle for reuse (from objects already freed) back to the driver
+    (iccg_device_trim); live objects keep theirs."""
+    check(lib().iccg_device_trim())
+
+
 def bench_copy(nbytes, reps):
     ms = C.c_double(0)
     check(lib().iccg_bench_copy(nbytes, reps, C.byref(ms)))

@@ -822,6 +822,16 @@ int iccg_device_get(int *device) {
   });
 }
 
+int iccg_device_trim(void) {
+  return guard([&] {
+    HIPCHK(hipSetDevice(g_device));
+    const DevicePool P = device_pool();
+    HIPCHK(hipStreamSynchronize(P.st));  // frees queued on the pool's stream have run
+    HIPCHK(hipMemPoolTrimTo(P.pool, 0));
+    pinned_trim();
+  });
+}
+
 int iccg_csr_create(int n, long nnz, const int *rowptr, const int *colind, const double *val,
                     iccg_csr_t *out) {
   return guard([&] {

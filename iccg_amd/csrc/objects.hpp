@@ -180,6 +180,22 @@ inline void hfree(void *p) {
   for (void *q : drop) (void)hipHostFree(q);
 }
 
+// frees every idle block of the pinned cache
+inline void pinned_trim() {
+  PinnedCache &c = pinned_cache();
+  std::vector<void *> drop;
+  {
+    std::lock_guard<std::mutex> lk(c.mu);
+    for (auto &kv : c.idle) {
+      c.blocks.erase(kv.second);
+      drop.push_back(kv.second);
+    }
+    c.idle.clear();
+    c.idle_bytes = 0;
+  }
+  for (void *q : drop) (void)hipHostFree(q);
+}
+
 // Waits for the work queued on st: through a blocking-sync event when ev is
 // set (the host thread sleeps until the device signals it), else with
 // hipStreamSynchronize, which spins while the device works.  Threads that

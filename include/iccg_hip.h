@@ -79,6 +79,13 @@ int iccg_device_count(int *count);
 int iccg_device_set(int device);
 /* Device used by this thread's calls (iccg_device_set, default 0). */
 int iccg_device_get(int *device);
+/* No counterpart in the reference (its vectors are std::vector).  The
+ * library keeps device memory freed by its objects in a pool of its own and
+ * reuses it for later objects without a device-wide wait, and keeps up to
+ * 4 GiB of pinned host staging the same way.  This hands the idle part of
+ * both back (device memory of this thread's device); objects still alive
+ * keep theirs.  For callers that share the GPU with other allocators. */
+int iccg_device_trim(void);
 
 /* ---- L0: CRS I/O (crs_io.hpp:10-56) ---------------------------------------- */
 
