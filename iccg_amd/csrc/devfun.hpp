@@ -211,6 +211,18 @@ static inline int quad_grid(long n) { return red_grid((n + 4L * kWG - 1) / (4L *
 // grid of a kernel that takes one 64-row slice per wavefront per step
 static inline int slice_grid(long nslices) { return red_grid((nslices + 3) / 4); }
 
+// Row-width class WM of the SpMV kernels (for_rows_Ax and its kinds): the
+// paired 16-bit layout holds WM / 2 pairs plus an odd last entry, so rows of
+// up to WM + 1 entries fit class WM, and 5-point rows take class 4 (8 before:
+// 2D 1980^2, four right-hand sides, 131.8 -> 117.7 us per launch); the 32-bit
+// columns fit up to WM.  7-point rows stay in class 8: a class 6 measured
+// slower (242.5 -> 257.0 us at 192^3, four right-hand sides).  0: CSR rows.
+static inline int spmv_wm(const DevCsr &A) {
+  const int w = A.sell.maxw;
+  if (A.sell.col16) return w <= 5 ? 4 : w <= 9 ? 8 : w <= 17 ? 16 : 0;
+  return w <= 4 ? 4 : w <= 8 ? 8 : w <= 16 ? 16 : 0;
+}
+
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return fma(a, b, c); }
 

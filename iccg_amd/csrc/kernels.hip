@@ -1068,7 +1068,7 @@ static int ax_grid(const DevCsr &A) {
 #define DISPATCH_WM(A, KERNEL, ...)                                              \
   do {                                                                           \
     const bool _tv = (A).sell.col16 && (A).sell.toff;                            \
-    switch (wm_of((A).sell.maxw)) {                                              \
+    switch (spmv_wm(A)) {                                                        \
       case 4:                                                                    \
         if (_tv) hipLaunchKernelGGL((KERNEL<4, true>), __VA_ARGS__);             \
         else hipLaunchKernelGGL((KERNEL<4, false>), __VA_ARGS__);                \

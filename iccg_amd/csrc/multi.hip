@@ -308,9 +308,10 @@ __device__ __forceinline__ void for_rows_AXk(const DevCsr &A, const double *__re
 // AP = A P with p.Ap, ||p||^2, ||Ap||^2 per member (components 3m + c),
 // as k_spmv_cg; the next slice's loads go out before the current slice's
 // gathers (192^3, four members: 269.7 -> 243.7 us per launch at 166 VGPRs
-// and 3 waves per SIMD, against 124 VGPRs and 4 waves without)
+// and 3 waves per SIMD, against 124 VGPRs and 4 waves without).  Class 4
+// (5-point rows) is held to 4 waves per SIMD (130 -> 128 VGPRs, no spills).
 template <int WM, bool TV, int K>
-__global__ __launch_bounds__(kWG) void km_spmv_cg(DevCsr A, const double *__restrict__ P,
+__global__ __launch_bounds__(kWG, WM == 4 ? 4 : 1) void km_spmv_cg(DevCsr A, const double *__restrict__ P,
                                                   double *__restrict__ AP, const Scal *S,
                                                   double *part) {
   if (all_done<K>(S)) return;
@@ -911,10 +912,8 @@ static bool multi_plan(const iccg_csr_s &A, const iccg_precond_s &M) {
          std::max(F.ls.maxw, F.us.maxw) <= 16;
 }
 
-static int mwm(const DevCsr &A) {
-  const int w = A.sell.maxw;
-  return w <= 4 ? 4 : w <= 8 ? 8 : 16;
-}
+// Row-width class of the SpMV kernels (spmv_wm)
+static int mwm(const DevCsr &A) { return spmv_wm(A); }
 
 #define MDISPATCH_A(A, KERNEL, K, ...)                                          \
   do {                                                                          \
