@@ -785,7 +785,7 @@ struct Slot {
   iccg_cg_result res{};
 };
 
-int wm_class(int w) { return w <= 4 ? 4 : w <= 8 ? 8 : 16; }
+int wm_class(int w) { return w <= 4 ? 4 : w <= 5 ? 5 : w <= 8 ? 8 : 16; }
 
 }  // namespace
 
@@ -984,11 +984,15 @@ int km_class(int k) { return k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : 8; }
 // K<WM, KM, ...extra> over the template widths of the set
 #define BATCH_DISPATCH(wm, km, K, EXTRA, ...)                                                  \
   do {                                                                                        \
-    const int _w = (wm) <= 4 ? 4 : (wm) <= 8 ? 8 : 16, _k = km_class(km);                     \
+    const int _w = (wm) <= 4 ? 4 : (wm) <= 5 ? 5 : (wm) <= 8 ? 8 : 16, _k = km_class(km);     \
     if (_w == 4 && _k == 1) hipLaunchKernelGGL((K<4, 1 EXTRA>), __VA_ARGS__);                  \
     else if (_w == 4 && _k == 2) hipLaunchKernelGGL((K<4, 2 EXTRA>), __VA_ARGS__);             \
     else if (_w == 4 && _k == 4) hipLaunchKernelGGL((K<4, 4 EXTRA>), __VA_ARGS__);             \
     else if (_w == 4) hipLaunchKernelGGL((K<4, 8 EXTRA>), __VA_ARGS__);                        \
+    else if (_w == 5 && _k == 1) hipLaunchKernelGGL((K<5, 1 EXTRA>), __VA_ARGS__);             \
+    else if (_w == 5 && _k == 2) hipLaunchKernelGGL((K<5, 2 EXTRA>), __VA_ARGS__);             \
+    else if (_w == 5 && _k == 4) hipLaunchKernelGGL((K<5, 4 EXTRA>), __VA_ARGS__);             \
+    else if (_w == 5) hipLaunchKernelGGL((K<5, 8 EXTRA>), __VA_ARGS__);                        \
     else if (_w == 8 && _k == 1) hipLaunchKernelGGL((K<8, 1 EXTRA>), __VA_ARGS__);             \
     else if (_w == 8 && _k == 2) hipLaunchKernelGGL((K<8, 2 EXTRA>), __VA_ARGS__);             \
     else if (_w == 8 && _k == 4) hipLaunchKernelGGL((K<8, 4 EXTRA>), __VA_ARGS__);             \
