@@ -800,7 +800,8 @@ struct iccg_batch_s {
   SysScal *h_sc = nullptr;    // pinned
   double *part = nullptr;
   size_t part_slots = 0;
-  int wm = 4;                 // template width of the running set
+  int wm = 4;                 // template width of A's rows in the running set (kb_ax)
+  int wmf = 4;                // and of its factors' rows (kb_sweep)
   int chunk = 16;             // slices per chunk item
   int cw = 1;                 // slices (MC) or blocks (ABMC) per sweep wavefront
   bool autotune = true;
@@ -1061,10 +1062,10 @@ static void launch_color(iccg_batch_s &B, const ItemSet &I, int c) {
   }
   if (ni > n0) {
     if (FWD)
-      BATCH_DISPATCH(B.wm, I.kmax, kb_sweep, FWD_T, dim3((ni - n0 + 3) / 4), dim3(kWG), 0, B.st,
+      BATCH_DISPATCH(B.wmf, I.kmax, kb_sweep, FWD_T, dim3((ni - n0 + 3) / 4), dim3(kWG), 0, B.st,
                      I.d_groups, B.d_desc, items + n0, ni - n0, B.d_sc);
     else
-      BATCH_DISPATCH(B.wm, I.kmax, kb_sweep, BWD_T, dim3((ni - n0 + 3) / 4), dim3(kWG), 0, B.st,
+      BATCH_DISPATCH(B.wmf, I.kmax, kb_sweep, BWD_T, dim3((ni - n0 + 3) / 4), dim3(kWG), 0, B.st,
                      I.d_groups, B.d_desc, items + n0, ni - n0, B.d_sc);
   }
 }
@@ -1254,11 +1255,14 @@ void admit_pending(iccg_batch_s &B) {
   }
   HIPCHK(hipMemcpyAsync(B.d_desc, B.h_desc, sizeof(SysDesc) * (size_t)B.cap,
                         hipMemcpyHostToDevice, B.st));
-  int wm = 4;
+  int wm = 4, wmf = 4;
   for (const Slot &s : B.slots)
-    if (s.state == 1 || s.state == 2)
-      wm = std::max({wm, s.A->d.sell.maxw, s.M->F.ls.maxw, s.M->F.us.maxw});
+    if (s.state == 1 || s.state == 2) {
+      wm = std::max(wm, s.A->d.sell.maxw);
+      wmf = std::max({wmf, s.M->F.ls.maxw, s.M->F.us.maxw});
+    }
   B.wm = wm_class(wm);
+  B.wmf = wm_class(wmf);
   ItemSet I;
   build_items(B, ids, I, B.chunk, B.cw);
   launch_resid_items(B, I, 1);
