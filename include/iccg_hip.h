@@ -306,7 +306,7 @@ int iccg_solver_get_timing(iccg_solver_t S, int cls, long *launches,
 int iccg_solver_reset_timing(iccg_solver_t S);
 
 /* ---- several right-hand sides of one system (no reference counterpart) -----
- * The north star's multi-RHS pattern: nrhs right-hand sides of one matrix and
+ * Requests that carry several right-hand sides: nrhs of one matrix and
  * preconditioner solved together.  Each right-hand side keeps its own CG
  * recurrence (cg_crs.cpp:89-162: its own alpha, beta, stop test, iteration
  * count and true-residual restarts) and its iterates are those of
