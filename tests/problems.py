@@ -72,5 +72,5 @@ def orc_sched(s: I.ColorSchedule) -> O.Schedule:
 
 
 def rhs_ones_times_A(A):
-    """b = A * 1 (the C1 right-hand side) computed by the oracle's SpMV."""
+    """b = A * 1 (the 100x100 Laplacian case's right-hand side) computed by the oracle's SpMV."""
     return O.spmv(orc(A), np.ones(A.n))

@@ -31,7 +31,7 @@ def _ulps2(got, want):
     return abs(got - want) <= 4.5e-16 * want
 
 
-def test_config4_512_system_batch():
+def test_batch_of_512_2d_systems_sides_32_to_256():
     sides, systems = bench.batch_problem(512)
     assert len(systems) == 512 and 1024 <= min(A.n for A, _, _ in systems)
     assert max(A.n for A, _, _ in systems) <= 65536
@@ -85,7 +85,7 @@ def _collect(S, ids):
     return out
 
 
-def test_config5_serving_stream_requests():
+def test_serving_stream_requests_up_to_1_8m_rows():
     stream = bench.serve_requests(max(PICK) + 1, bench.SEED)
     reqs = [stream[k] for k in PICK]
     assert [A.n for A, _, _ in reqs] == [76176, 1745041, 1815848, 10648]

@@ -78,7 +78,7 @@ def _err(x, xr):
     return float(np.linalg.norm(x - xr) / np.linalg.norm(xr))
 
 
-def test_config2_128cube_red_black_fp64():
+def test_128cube_red_black_fp64():
     """3D 7-point Laplacian 128^3 (n = 2,097,152), red-black IC(0), FP64,
     b ~ N(0,1), tol 1e-8.  The two colors' sweeps run 4,096 workgroups, so
     the r.z and |r|^2 partial regions of the fused multicolor iteration both
@@ -144,7 +144,7 @@ def test_multicolor_three_colors_over_2048_sweep_groups():
     _assert_true_resid(Ao, x, b, res)
 
 
-def test_config3_192cube_abmc_mixed_and_fp64():
+def test_192cube_abmc_boxes_mixed_and_fp64():
     """3D variable-coefficient 192^3 (n = 7,077,888), ABMC 4x4x4 boxes, FP64
     and mixed precision, b ~ N(0,1), tol 1e-8 (true residual for mixed),
     against the reference's abmc_crs CG on the same permuted system."""

@@ -354,8 +354,8 @@ def test_cg_matches_oracle(kind, name):
     assert err < 1e-6
 
 
-def test_cg_c1_shape():
-    """2D 100x100 Laplacian (the CPU configuration), b = A*1, x0 = 0, tol 1e-8."""
+def test_cg_100x100_laplacian_b_is_a_times_ones():
+    """2D 100x100 Laplacian (the reference's CPU-sized case), b = A*1, x0 = 0, tol 1e-8."""
     A = P.grid(100, 100)
     b = P.rhs_ones_times_A(A)
     for kind in ("none", "ic0"):

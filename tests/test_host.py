@@ -217,8 +217,8 @@ def test_generator_matches_restatement():
     L = I.gen_grid(100, 100)  # the 100x100 Laplacian: diag 4, off-diagonal -1
     assert L.n == 10000 and L.nnz == 49600
     assert set(np.unique(L.val)) == {-1.0, 4.0}
-    C3 = I.gen_grid(12, 12, 12)
-    assert C3.nnz == 7 * 12 ** 3 - 6 * 12 ** 2 and set(np.unique(C3.val)) == {-1.0, 6.0}
+    G3 = I.gen_grid(12, 12, 12)
+    assert G3.nnz == 7 * 12 ** 3 - 6 * 12 ** 2 and set(np.unique(G3.val)) == {-1.0, 6.0}
 
 
 def test_normal_generator_box_muller():
