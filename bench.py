@@ -5,8 +5,9 @@ Workload (the reference's abmc_crs path at the configuration its metric is
 quoted on): 3D 7-point variable-coefficient diffusion on a 192^3 Dirichlet
 grid (n = 7,077,888), face coefficients 10^U[-2,2], diagonal = sum of the
 cell's face coefficients + 1e-6, b ~ N(0,1), x0 = 0; ABMC ordering with
-block size 64 (breadth-first blocks) and first-fit greedy block coloring;
-the permuted system P^T A P is solved as abmc_crs does (cg_crs.cpp:413-508).
+blocks of 64 rows (4x4x4 boxes, two colors; --block 64 takes breadth-first
+blocks with first-fit greedy block coloring instead); the permuted system
+P^T A P is solved as abmc_crs does (cg_crs.cpp:413-508).
 
 A step is one complete solve to tolerance with every input resident in
 HBM (the CG phase, as the reference times it, cg_crs.cpp:507-510).  The
