@@ -884,8 +884,12 @@ def main():
         "vs_baseline": None, "dtype": "f64 CG + f32 preconditioner", "data": "synthetic",
         "config": {"workload": (f"abmc_crs: {args.dims}D {2 * args.dims + 1}-pt "
                                 f"variable-coefficient diffusion {N}^{args.dims} "
-                                f"(n={n}, nnz={nnz}), ABMC B={args.block} greedy "
-                                f"({info['nb']} blocks, {info['nc']} colors), b~N(0,1), x0=0, "
+                                f"(n={n}, nnz={nnz}), ABMC "
+                                + (("4x4x4 boxes" if args.dims == 3 else "8x8 boxes")
+                                   + " of 64 rows" if args.block == "box" else
+                                   f"breadth-first blocks of {args.block} rows")
+                                + f" ({info['nb']} blocks, {info['nc']} colors by first-fit "
+                                f"greedy coloring), b~N(0,1), x0=0, "
                                 f"tol {args.tol:g} on the FP64 true residual"),
                    "n": n, "nnz": nnz, "tol": args.tol},
         "iters": r32.iters, "true_rel_resid": r32.true_rel_resid, "restarts": r32.restarts,
