@@ -912,7 +912,8 @@ static bool multi_plan(const iccg_csr_s &A, const iccg_precond_s &M) {
          std::max(F.ls.maxw, F.us.maxw) <= 16;
 }
 
-// Row-width class of the SpMV kernels (spmv_wm)
+// Row-width class of the SpMV kernels (spmv_wm): 4, 8 or 16, since
+// multi_plan takes rows of at most 16 entries
 static int mwm(const DevCsr &A) { return spmv_wm(A); }
 
 #define MDISPATCH_A(A, KERNEL, K, ...)                                          \
